@@ -1,0 +1,569 @@
+// procell_api.hip -- host side of libprocell: validation, problem set-up,
+// device buffers, launches and the merge of tallies into the output
+// histogram.  Replaces the host halves of create_cells_population
+// (src/simulation/cells_population.cu:18-54) and proliferate
+// (src/simulation/proliferation.cu:26-215), and the bin-set construction of
+// load_fluorescences (src/io/parser.cu:68-154).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "procell_internal.hpp"
+
+namespace procell {
+
+// ---------------------------------------------------------------------------
+// Validation and set-up (mirrors the reference's parse-time rules).
+// ---------------------------------------------------------------------------
+static int generation_cap(double v, double phi)
+{
+    // proliferation.cu:323: divide only while f/2 > phi; f halves exactly.
+    int k = 0;
+    double f = v;
+    while (f / 2 > phi) {
+        f = f / 2;
+        if (++k > kMaxGeneration)
+            return k;
+    }
+    return k;
+}
+
+static double resolve_phi(const procell_params *p)
+{
+    if (p->phi > 0.0)
+        return p->phi;
+    // parser.cu:80-96: smallest value among rows with a nonzero count.
+    double best = 0.0;
+    bool have = false;
+    for (uint64_t i = 0; i < p->n_rows; ++i) {
+        const procell_row &r = p->rows[i];
+        if (r.count > 0 && (!have || r.value < best)) {
+            best = r.value;
+            have = true;
+        }
+    }
+    return best;
+}
+
+int build_setup(const procell_params *p, Setup &s, bool want_bins)
+{
+    if (!p || (p->n_rows && !p->rows) || (p->n_types && !p->types))
+        return PROCELL_E_INVALID;
+    if (isnan(p->t_max) || p->t_max < 0.0)
+        return PROCELL_E_INVALID;
+    s = Setup();
+    s.t_max = p->t_max;
+    uint64_t total = 0, n_nz = 0;
+    for (uint64_t i = 0; i < p->n_rows; ++i) {
+        if (!isfinite(p->rows[i].value))
+            return PROCELL_E_NONFINITE;
+        const uint64_t c = p->rows[i].count;
+        if (c == 0)
+            continue;
+        ++n_nz;
+        if (total + c < total || total + c > kMaxCells)
+            return PROCELL_E_TOO_MANY_CELLS;
+        total += c;
+    }
+    if (n_nz > kMaxRows)
+        return PROCELL_E_TOO_MANY_CELLS;
+    if (p->n_types == 0)
+        return total ? PROCELL_E_PROPORTIONS : PROCELL_OK;
+    if (p->n_types > kMaxTypes)
+        return PROCELL_E_TOO_MANY_CELLS;
+    // parser.cu:46-66: proportions (left fold, file order) must sum to 1.
+    double sum = 0.0;
+    for (uint32_t j = 0; j < p->n_types; ++j) {
+        const procell_type &t = p->types[j];
+        if (!isfinite(t.proportion) || !isfinite(t.mean) || !isfinite(t.sd))
+            return PROCELL_E_NONFINITE;
+        sum = sum + t.proportion;
+    }
+    if (fabs(1.0 - sum) > 1.0 / pow(10.0, 8.0))
+        return PROCELL_E_PROPORTIONS;
+    for (uint32_t j = 0; j < p->n_types; ++j)
+        if (p->types[j].mean == 0.0)
+            return PROCELL_E_ZERO_MEAN;  // cell.cu:110 would leave the timer unset
+    const double phi = resolve_phi(p);
+    if (total > 0 && !(phi > 0.0))
+        return PROCELL_E_PHI;
+    s.phi = phi;
+    s.n_cells = total;
+    s.n_types = p->n_types;
+    s.v.reserve(n_nz);
+    s.start.reserve(n_nz + 1);
+    s.row_info.reserve(n_nz);
+    uint64_t acc = 0;
+    for (uint64_t i = 0; i < p->n_rows; ++i) {
+        const procell_row &r = p->rows[i];
+        if (r.count == 0)
+            continue;
+        const int kc = generation_cap(r.value, phi);
+        if (kc > kMaxGeneration)
+            return PROCELL_E_DEPTH;
+        s.kmax = std::max(s.kmax, kc);
+        s.v.push_back(r.value);
+        s.start.push_back(acc);
+        s.row_info.push_back((uint32_t)kc | ((r.value >= phi ? 1u : 0u) << 8));
+        acc += r.count;
+    }
+    s.start.push_back(acc);
+    // parser.cu:184: descending proportion; ties keep file order.
+    s.type_file.resize(s.n_types);
+    for (uint32_t j = 0; j < s.n_types; ++j)
+        s.type_file[j] = j;
+    std::stable_sort(s.type_file.begin(), s.type_file.end(), [&](uint32_t x, uint32_t y) {
+        return p->types[x].proportion > p->types[y].proportion;
+    });
+    // cell.cu:86-101: running sum in sorted order.
+    double c = 0.0;
+    for (uint32_t j = 0; j < s.n_types; ++j) {
+        const procell_type &t = p->types[s.type_file[j]];
+        c = c + t.proportion;
+        s.type_cum.push_back(c);
+        s.type_mean.push_back(t.mean);
+        s.type_sd.push_back(t.sd);
+    }
+    if (want_bins) {
+        // parser.cu:126-151: every v / 2^k >= phi, exact halving, distinct.
+        for (double v : s.v) {
+            double f = v;
+            while (f >= phi) {
+                s.bins.push_back(f);
+                f = f / 2;
+            }
+        }
+        std::sort(s.bins.begin(), s.bins.end());
+        s.bins.erase(std::unique(s.bins.begin(), s.bins.end()), s.bins.end());
+    }
+    return PROCELL_OK;
+}
+
+}  // namespace procell
+
+using namespace procell;
+
+// ---------------------------------------------------------------------------
+// Plan: device buffers for one problem, reused across runs.
+// ---------------------------------------------------------------------------
+struct procell_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool timing = false;
+    int32_t track_ratio = 0;
+    Setup setup;
+    LaunchShape shape{};
+    RunArgs args{};
+    void *d_block = nullptr;  // one allocation holding every table
+    size_t tally_len = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool ran = false;
+};
+
+namespace {
+
+thread_local hipError_t g_last_hip = hipSuccess;
+
+#define HIP_TRY(expr)                        \
+    do {                                     \
+        hipError_t e_ = (expr);              \
+        if (e_ != hipSuccess) {              \
+            g_last_hip = e_;                 \
+            return PROCELL_E_HIP;            \
+        }                                    \
+    } while (0)
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int check_device(int device, int *n_cu)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count)
+        return PROCELL_E_NO_DEVICE;
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return PROCELL_E_NO_DEVICE;
+    *n_cu = prop.multiProcessorCount;
+    return PROCELL_OK;
+}
+
+void destroy_plan(procell_plan *pl)
+{
+    if (!pl)
+        return;
+    (void)hipSetDevice(pl->device);
+    for (hipEvent_t &e : pl->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (pl->d_block)
+        (void)hipFree(pl->d_block);
+    if (pl->own_stream && pl->stream)
+        (void)hipStreamDestroy(pl->stream);
+    delete pl;
+}
+
+int create_plan(const procell_params *p, procell_plan **out)
+{
+    if (!p || !out)
+        return PROCELL_E_INVALID;
+    *out = nullptr;
+    const procell_tuning &tu = p->tuning;
+    if (tu.block_size && (tu.block_size % 64 || tu.block_size > 256))
+        return PROCELL_E_INVALID;
+    procell_plan *pl = new (std::nothrow) procell_plan();
+    if (!pl)
+        return PROCELL_E_OUT_OF_MEMORY;
+    int rc = build_setup(p, pl->setup, true);
+    if (rc != PROCELL_OK) {
+        delete pl;
+        return rc;
+    }
+    int n_cu = 0;
+    rc = check_device(p->device, &n_cu);
+    if (rc != PROCELL_OK) {
+        delete pl;
+        return rc;
+    }
+    pl->device = p->device;
+    pl->timing = (tu.flags & PROCELL_FLAG_TIMING) != 0;
+    pl->track_ratio = p->track_ratio;
+    const Setup &s = pl->setup;
+    const uint32_t n_rows = (uint32_t)s.v.size();
+    const uint32_t T = s.n_types ? s.n_types : 1;
+    const uint32_t kdim = (uint32_t)s.kmax + 1;
+    pl->tally_len = (size_t)n_rows * kdim * T;
+
+    // one device allocation: counters | tally | row_start | row_info | types | pool
+    size_t off = 0;
+    const size_t o_cnt = off; off = align_up(off + kNumCounters * 8, 256);
+    const size_t o_tally = off; off = align_up(off + pl->tally_len * 8, 256);
+    const size_t o_rs = off; off = align_up(off + (size_t)(n_rows + 1) * 8, 256);
+    const size_t o_ri = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
+    const size_t o_cum = off; off = align_up(off + (size_t)T * 8, 256);
+    const size_t o_mean = off; off = align_up(off + (size_t)T * 8, 256);
+    const size_t o_sd = off; off = align_up(off + (size_t)T * 8, 256);
+    const size_t o_tf = off; off = align_up(off + (size_t)T * 4, 256);
+    const size_t o_pool = off; off = align_up(off + (size_t)(s.n_cells + 1) * sizeof(PoolEntry), 256);
+    auto fail = [&](int code) {
+        destroy_plan(pl);
+        return code;
+    };
+    if (p->stream) {
+        pl->stream = (hipStream_t)p->stream;
+    } else {
+        if (hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(PROCELL_E_HIP);
+        pl->own_stream = true;
+    }
+    if (hipMalloc(&pl->d_block, off) != hipSuccess)
+        return fail(PROCELL_E_OUT_OF_MEMORY);
+    char *base = (char *)pl->d_block;
+    RunArgs &a = pl->args;
+    a.t_max = s.t_max;
+    a.n_cells = s.n_cells;
+    a.n_rows = n_rows;
+    a.n_types = T;
+    a.kdim = kdim;
+    a.counters = (unsigned long long *)(base + o_cnt);
+    a.tally = (unsigned long long *)(base + o_tally);
+    a.row_start = (const uint64_t *)(base + o_rs);
+    a.row_info = (const uint32_t *)(base + o_ri);
+    a.type_cum = (const double *)(base + o_cum);
+    a.type_mean = (const double *)(base + o_mean);
+    a.type_sd = (const double *)(base + o_sd);
+    a.type_file = (const uint32_t *)(base + o_tf);
+    a.pool = (PoolEntry *)(base + o_pool);
+    if (n_rows) {
+        if (hipMemcpyAsync(base + o_rs, s.start.data(), (n_rows + 1) * 8, hipMemcpyHostToDevice, pl->stream) != hipSuccess ||
+            hipMemcpyAsync(base + o_ri, s.row_info.data(), n_rows * 4, hipMemcpyHostToDevice, pl->stream) != hipSuccess)
+            return fail(PROCELL_E_HIP);
+    }
+    if (s.n_types) {
+        if (hipMemcpyAsync(base + o_cum, s.type_cum.data(), s.n_types * 8, hipMemcpyHostToDevice, pl->stream) != hipSuccess ||
+            hipMemcpyAsync(base + o_mean, s.type_mean.data(), s.n_types * 8, hipMemcpyHostToDevice, pl->stream) != hipSuccess ||
+            hipMemcpyAsync(base + o_sd, s.type_sd.data(), s.n_types * 8, hipMemcpyHostToDevice, pl->stream) != hipSuccess ||
+            hipMemcpyAsync(base + o_tf, s.type_file.data(), s.n_types * 4, hipMemcpyHostToDevice, pl->stream) != hipSuccess)
+            return fail(PROCELL_E_HIP);
+    }
+    if (hipStreamSynchronize(pl->stream) != hipSuccess)
+        return fail(PROCELL_E_HIP);
+
+    // launch shape
+    LaunchShape &ls = pl->shape;
+    ls.roots_block_size = 256;
+    ls.roots_window_rows = 16;
+    int rb = 0;
+    if (roots_occupancy(ls.roots_block_size, (size_t)ls.roots_window_rows * T * 4, &rb) != hipSuccess || rb < 1)
+        rb = 1;
+    const uint64_t tiles = (s.n_cells + ls.roots_block_size - 1) / ls.roots_block_size;
+    ls.roots_blocks = (int)std::min<uint64_t>(std::max<uint64_t>(tiles, 1), (uint64_t)n_cu * rb);
+    ls.tree_block_size = tu.block_size ? (int)tu.block_size : 256;
+    ls.tree_lds_bytes = (size_t)kdim * ls.tree_block_size * sizeof(double);
+    if (ls.tree_lds_bytes > 64 * 1024) {
+        // deep trees: shrink the workgroup so its stack fits
+        while (ls.tree_block_size > 64 && (size_t)kdim * ls.tree_block_size * 8 > 64 * 1024)
+            ls.tree_block_size -= 64;
+        ls.tree_lds_bytes = (size_t)kdim * ls.tree_block_size * sizeof(double);
+    }
+    int tb = 0;
+    if (tree_occupancy(ls.tree_block_size, ls.tree_lds_bytes, &tb) != hipSuccess || tb < 1)
+        tb = 1;
+    if (tu.blocks_per_cu)
+        tb = std::min<int>(tb, (int)tu.blocks_per_cu);
+    ls.tree_blocks = n_cu * tb;
+    ls.chunk = tu.chunk ? tu.chunk : 64;
+    if (pl->timing) {
+        for (hipEvent_t &e : pl->ev)
+            if (hipEventCreate(&e) != hipSuccess)
+                return fail(PROCELL_E_HIP);
+    }
+    *out = pl;
+    return PROCELL_OK;
+}
+
+int run_plan(procell_plan *pl, uint64_t seed)
+{
+    HIP_TRY(hipSetDevice(pl->device));
+    pl->args.seed = seed;
+    // counters and tally are contiguous at the start of the block
+    const uint64_t n_zero = (uint64_t)((char *)pl->args.tally - (char *)pl->args.counters) / 8 + pl->tally_len;
+    HIP_TRY(launch_zero(pl->args.counters, n_zero, pl->stream));
+    if (pl->timing)
+        HIP_TRY(hipEventRecord(pl->ev[0], pl->stream));
+    HIP_TRY(launch_roots(pl->args, pl->shape, pl->stream));
+    if (pl->timing)
+        HIP_TRY(hipEventRecord(pl->ev[1], pl->stream));
+    HIP_TRY(launch_tree(pl->args, pl->shape, pl->stream));
+    if (pl->timing)
+        HIP_TRY(hipEventRecord(pl->ev[2], pl->stream));
+    pl->ran = true;
+    return PROCELL_OK;
+}
+
+int plan_result(procell_plan *pl, procell_result *r)
+{
+    if (!r || !pl->ran)
+        return PROCELL_E_INVALID;
+    const Setup &s = pl->setup;
+    const uint32_t T = pl->args.n_types;
+    const uint32_t kdim = pl->args.kdim;
+    const uint64_t n_rows = s.v.size();
+    const uint64_t n_bins = s.bins.size();
+    if (r->capacity && !(r->value && r->count))
+        return PROCELL_E_INVALID;
+    HIP_TRY(hipSetDevice(pl->device));
+    std::vector<unsigned long long> cnt(kNumCounters);
+    std::vector<int64_t> tally(pl->tally_len);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), pl->args.counters, kNumCounters * 8, hipMemcpyDeviceToHost, pl->stream));
+    if (pl->tally_len)
+        HIP_TRY(hipMemcpyAsync(tally.data(), pl->args.tally, pl->tally_len * 8, hipMemcpyDeviceToHost, pl->stream));
+    HIP_TRY(hipStreamSynchronize(pl->stream));
+    procell_stats &st = r->stats;
+    memset(&st, 0, sizeof st);
+    st.divisions = cnt[kDivisions];
+    st.recorded = cnt[kRecorded];
+    st.dropped = cnt[kDropped];
+    st.lost = cnt[kLost];
+    st.rejected = cnt[kRejected];
+    st.cells = cnt[kCells];
+    st.pool = cnt[kPoolCount];
+    if (pl->timing) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, pl->ev[0], pl->ev[1]));
+        st.ms_roots = ms;
+        HIP_TRY(hipEventElapsedTime(&ms, pl->ev[1], pl->ev[2]));
+        st.ms_tree = ms;
+        HIP_TRY(hipEventElapsedTime(&ms, pl->ev[0], pl->ev[2]));
+        st.ms_run = ms;
+    }
+    if (r->raw_tally) {
+        if (r->raw_tally_len < pl->tally_len)
+            return PROCELL_E_CAPACITY;
+        memcpy(r->raw_tally, tally.data(), pl->tally_len * 8);
+    }
+    // (row, generation) -> output value row_value * 2^-generation, exact
+    // equality merge into the bin set (proliferation.cu:353-380 searches the
+    // same set by exact equality).
+    std::vector<int64_t> bc(n_bins, 0), bt((size_t)n_bins * T, 0);
+    for (uint64_t row = 0; row < n_rows; ++row) {
+        double f = s.v[row];
+        for (uint32_t k = 0; k < kdim; ++k, f = f / 2) {
+            const int64_t *cell = &tally[((size_t)row * kdim + k) * T];
+            int64_t tot = 0;
+            for (uint32_t j = 0; j < T; ++j)
+                tot += cell[j];
+            if (!tot)
+                continue;
+            auto it = std::lower_bound(s.bins.begin(), s.bins.end(), f);
+            if (it == s.bins.end() || *it != f)
+                continue;  // cannot happen: recorded values are >= phi
+            const size_t b = (size_t)(it - s.bins.begin());
+            bc[b] += tot;
+            for (uint32_t j = 0; j < T; ++j)
+                bt[b * T + j] += cell[j];
+        }
+    }
+    uint64_t n_out = 0;
+    for (uint64_t b = 0; b < n_bins; ++b)
+        n_out += bc[b] > 0;
+    r->n_out = n_out;
+    if (n_out > r->capacity)
+        return PROCELL_E_CAPACITY;
+    uint64_t o = 0;
+    for (uint64_t b = 0; b < n_bins; ++b) {
+        if (bc[b] <= 0)
+            continue;
+        r->value[o] = s.bins[b];
+        r->count[o] = bc[b];
+        if (r->type_count)
+            for (uint32_t j = 0; j < s.n_types; ++j)
+                r->type_count[o * s.n_types + j] = bt[b * T + j];
+        ++o;
+    }
+    return PROCELL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int procell_abi_version(void) { return PROCELL_ABI_VERSION; }
+
+const char *procell_strerror(int code)
+{
+    switch (code) {
+    case PROCELL_OK: return "success";
+    case PROCELL_E_INVALID: return "invalid argument";
+    case PROCELL_E_PROPORTIONS: return "proportion distribution of cell types does not sum to 1";
+    case PROCELL_E_ZERO_MEAN: return "a cell type has mean 0";
+    case PROCELL_E_NONFINITE: return "non-finite value in the input";
+    case PROCELL_E_PHI: return "phi must be > 0";
+    case PROCELL_E_DEPTH: return "division tree deeper than 62 generations (phi too small)";
+    case PROCELL_E_TOO_MANY_CELLS: return "input exceeds 2^32-1 cells, 2^24 rows or 256 types";
+    case PROCELL_E_CAPACITY: return "result buffer too small";
+    case PROCELL_E_HIP: return hipGetErrorString(g_last_hip);
+    case PROCELL_E_NO_DEVICE: return "no usable gfx950 device";
+    case PROCELL_E_IO: return "cannot write output file";
+    case PROCELL_E_OUT_OF_MEMORY: return "out of memory";
+    default: return "unknown error";
+    }
+}
+
+int procell_output_capacity(const procell_params *p, uint64_t *capacity)
+{
+    if (!capacity)
+        return PROCELL_E_INVALID;
+    Setup s;
+    const int rc = build_setup(p, s, true);
+    if (rc != PROCELL_OK)
+        return rc;
+    *capacity = s.bins.size();
+    return PROCELL_OK;
+}
+
+int procell_tally_shape(const procell_params *p, uint64_t *nonzero_rows, uint32_t *kdim)
+{
+    if (!nonzero_rows || !kdim)
+        return PROCELL_E_INVALID;
+    Setup s;
+    const int rc = build_setup(p, s, false);
+    if (rc != PROCELL_OK)
+        return rc;
+    *nonzero_rows = s.v.size();
+    *kdim = (uint32_t)s.kmax + 1;
+    return PROCELL_OK;
+}
+
+int procell_resolve_phi(const procell_params *p, double *phi)
+{
+    if (!p || !phi || (p->n_rows && !p->rows))
+        return PROCELL_E_INVALID;
+    *phi = resolve_phi(p);
+    return PROCELL_OK;
+}
+
+int procell_plan_create(const procell_params *p, procell_plan **plan) { return create_plan(p, plan); }
+
+int procell_plan_run(procell_plan *plan, uint64_t seed)
+{
+    if (!plan)
+        return PROCELL_E_INVALID;
+    return run_plan(plan, seed);
+}
+
+int procell_plan_result(procell_plan *plan, procell_result *r)
+{
+    if (!plan)
+        return PROCELL_E_INVALID;
+    return plan_result(plan, r);
+}
+
+int procell_plan_destroy(procell_plan *plan)
+{
+    destroy_plan(plan);
+    return PROCELL_OK;
+}
+
+void *procell_plan_stream(procell_plan *plan) { return plan ? (void *)plan->stream : nullptr; }
+
+int procell_simulate(const procell_params *p, procell_result *r)
+{
+    if (!p || !r)
+        return PROCELL_E_INVALID;
+    procell_params q = *p;
+    q.stream = nullptr;
+    procell_plan *pl = nullptr;
+    int rc = create_plan(&q, &pl);
+    if (rc != PROCELL_OK)
+        return rc;
+    rc = run_plan(pl, p->seed);
+    if (rc == PROCELL_OK)
+        rc = plan_result(pl, r);
+    destroy_plan(pl);
+    return rc;
+}
+
+int procell_debug_draws(int32_t device, uint64_t seed, const uint64_t *cell, const uint64_t *node,
+                        const uint32_t *attempt, uint64_t n, uint32_t *words, double *z)
+{
+    if (n && (!cell || !node || !attempt || !words || !z))
+        return PROCELL_E_INVALID;
+    int n_cu = 0;
+    int rc = check_device(device, &n_cu);
+    if (rc != PROCELL_OK)
+        return rc;
+    if (n == 0)
+        return PROCELL_OK;
+    void *d = nullptr;
+    const size_t bytes = n * (8 + 8 + 4 + 16 + 16);
+    if (hipMalloc(&d, bytes) != hipSuccess)
+        return PROCELL_E_OUT_OF_MEMORY;
+    char *b = (char *)d;
+    uint64_t *dc = (uint64_t *)b;
+    uint64_t *dn = (uint64_t *)(b + n * 8);
+    double *dz = (double *)(b + n * 16);
+    uint32_t *dw = (uint32_t *)(b + n * 32);
+    uint32_t *da = (uint32_t *)(b + n * 48);
+    hipError_t e = hipMemcpy(dc, cell, n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, node, n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(da, attempt, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_debug_draws(seed, dc, dn, da, n, dw, dz, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(words, dw, n * 16, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(z, dz, n * 16, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        g_last_hip = e;
+        return PROCELL_E_HIP;
+    }
+    return PROCELL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,333 @@
+"""Python binding of libprocell (ctypes over the C ABI in include/procell.h).
+
+The simulation itself runs in the gfx950 HIP kernels of libprocell.so; this
+module only marshals arrays.  There is no fallback: if the library is not
+built, or no gfx950 device is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)
+BUILD_DIR = os.path.join(PKG_ROOT, "_build")
+LIB_PATH = os.path.join(BUILD_DIR, "libprocell.so")
+CLI_PATH = os.path.join(BUILD_DIR, "procell")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_ROOT), "include", "procell.h")
+
+OK = 0
+E_INVALID = -1
+E_PROPORTIONS = -2
+E_ZERO_MEAN = -3
+E_NONFINITE = -4
+E_PHI = -5
+E_DEPTH = -6
+E_TOO_MANY_CELLS = -7
+E_CAPACITY = -8
+E_HIP = -9
+E_NO_DEVICE = -10
+E_IO = -11
+E_OUT_OF_MEMORY = -12
+
+FLAG_TIMING = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class Row(ctypes.Structure):
+    _fields_ = [("value", ctypes.c_double), ("count", ctypes.c_uint64)]
+
+
+class CellType(ctypes.Structure):
+    _fields_ = [("proportion", ctypes.c_double), ("mean", ctypes.c_double), ("sd", ctypes.c_double)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("chunk", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("rows", ctypes.POINTER(Row)), ("n_rows", ctypes.c_uint64),
+                ("types", ctypes.POINTER(CellType)), ("n_types", ctypes.c_uint32),
+                ("t_max", ctypes.c_double), ("phi", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("track_ratio", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("tuning", Tuning), ("stream", ctypes.c_void_p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("divisions", ctypes.c_uint64), ("recorded", ctypes.c_uint64),
+                ("dropped", ctypes.c_uint64), ("lost", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("cells", ctypes.c_uint64),
+                ("pool", ctypes.c_uint64), ("ms_roots", ctypes.c_double),
+                ("ms_tree", ctypes.c_double), ("ms_run", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ResultStruct(ctypes.Structure):
+    _fields_ = [("capacity", ctypes.c_uint64), ("value", _dp), ("count", _i64p),
+                ("type_count", _i64p), ("n_out", ctypes.c_uint64), ("raw_tally", _i64p),
+                ("raw_tally_len", ctypes.c_uint64), ("stats", Stats)]
+
+
+class ProcellError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        msg = strerror(code) if _lib is not None else str(code)
+        super().__init__(f"{what}: {msg} ({code})" if what else f"{msg} ({code})")
+        self.code = code
+
+
+_lib = None
+
+EXPORTS = (
+    "procell_simulate", "procell_output_capacity", "procell_tally_shape", "procell_resolve_phi",
+    "procell_strerror", "procell_abi_version", "procell_plan_create", "procell_plan_run",
+    "procell_plan_result", "procell_plan_destroy", "procell_plan_stream", "procell_read_histogram",
+    "procell_read_types", "procell_write_histogram", "procell_free", "procell_debug_draws",
+)
+
+
+def lib():
+    """Load libprocell.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with cuda-pro-cell_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER(Params)
+    R = ctypes.POINTER(ResultStruct)
+    L.procell_simulate.argtypes = [P, R]
+    L.procell_output_capacity.argtypes = [P, _u64p]
+    L.procell_tally_shape.argtypes = [P, _u64p, _u32p]
+    L.procell_resolve_phi.argtypes = [P, _dp]
+    L.procell_strerror.restype = ctypes.c_char_p
+    L.procell_strerror.argtypes = [ctypes.c_int]
+    L.procell_plan_create.argtypes = [P, ctypes.POINTER(ctypes.c_void_p)]
+    L.procell_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    L.procell_plan_result.argtypes = [ctypes.c_void_p, R]
+    L.procell_plan_destroy.argtypes = [ctypes.c_void_p]
+    L.procell_plan_stream.restype = ctypes.c_void_p
+    L.procell_plan_stream.argtypes = [ctypes.c_void_p]
+    L.procell_read_histogram.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(Row)), _u64p]
+    L.procell_read_types.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(CellType)), _u32p]
+    L.procell_write_histogram.argtypes = [ctypes.c_char_p, R, ctypes.c_uint32, ctypes.c_int32]
+    L.procell_free.argtypes = [ctypes.c_void_p]
+    L.procell_debug_draws.argtypes = [ctypes.c_int32, ctypes.c_uint64, _u64p, _u64p, _u32p,
+                                      ctypes.c_uint64, _u32p, _dp]
+    _lib = L
+    return L
+
+
+def strerror(code: int) -> str:
+    return lib().procell_strerror(int(code)).decode()
+
+
+def _check(rc: int, what: str = ""):
+    if rc != OK:
+        raise ProcellError(rc, what)
+
+
+class Problem:
+    """Inputs of one simulation, kept alive for the C structs that point at them."""
+
+    def __init__(self, values, counts, proportion, mean, sd, t_max, phi=0.0, seed=0,
+                 track_ratio=True, device=0, block_size=0, blocks_per_cu=0, chunk=0, flags=0):
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        counts = np.ascontiguousarray(counts, dtype=np.uint64)
+        self.rows = (Row * max(len(values), 1))()
+        for i, (v, c) in enumerate(zip(values, counts)):
+            self.rows[i].value = float(v)
+            self.rows[i].count = int(c)
+        n_types = len(proportion)
+        self.types = (CellType * max(n_types, 1))()
+        for j in range(n_types):
+            self.types[j].proportion = float(proportion[j])
+            self.types[j].mean = float(mean[j])
+            self.types[j].sd = float(sd[j])
+        self.n_types = n_types
+        self.params = Params()
+        self.params.rows = ctypes.cast(self.rows, ctypes.POINTER(Row))
+        self.params.n_rows = len(values)
+        self.params.types = ctypes.cast(self.types, ctypes.POINTER(CellType))
+        self.params.n_types = n_types
+        self.params.t_max = float(t_max)
+        self.params.phi = float(phi)
+        self.params.seed = int(seed) & (2**64 - 1)
+        self.params.track_ratio = int(bool(track_ratio))
+        self.params.device = int(device)
+        self.params.tuning = Tuning(block_size, blocks_per_cu, chunk, flags)
+        self.params.stream = None
+
+    @classmethod
+    def from_workload(cls, w, seed=0, **kw):
+        kw.setdefault("track_ratio", w.track_ratio)
+        return cls(w.values, w.counts, w.proportion, w.mean, w.sd, w.t_max, w.phi, seed, **kw)
+
+    def output_capacity(self) -> int:
+        cap = ctypes.c_uint64()
+        _check(lib().procell_output_capacity(ctypes.byref(self.params), ctypes.byref(cap)), "output_capacity")
+        return cap.value
+
+    def tally_shape(self):
+        rows = ctypes.c_uint64()
+        kdim = ctypes.c_uint32()
+        _check(lib().procell_tally_shape(ctypes.byref(self.params), ctypes.byref(rows), ctypes.byref(kdim)),
+               "tally_shape")
+        return rows.value, kdim.value, max(self.n_types, 1)
+
+    def resolve_phi(self) -> float:
+        phi = ctypes.c_double()
+        _check(lib().procell_resolve_phi(ctypes.byref(self.params), ctypes.byref(phi)), "resolve_phi")
+        return phi.value
+
+
+@dataclass
+class Result:
+    value: np.ndarray        # float64 [n_out]
+    count: np.ndarray        # int64 [n_out]
+    type_count: np.ndarray   # int64 [n_out, n_types] (file order)
+    raw_tally: np.ndarray    # int64 [nonzero_rows, kdim, n_types]
+    stats: dict
+
+
+class _ResultBuffers:
+    def __init__(self, prob: Problem, want_raw=True):
+        cap = prob.output_capacity()
+        T = max(prob.n_types, 1)
+        self.value = np.zeros(max(cap, 1), dtype=np.float64)
+        self.count = np.zeros(max(cap, 1), dtype=np.int64)
+        self.type_count = np.zeros((max(cap, 1), T), dtype=np.int64)
+        shape = prob.tally_shape()
+        self.raw = np.zeros(shape, dtype=np.int64) if want_raw else None
+        self.r = ResultStruct()
+        self.r.capacity = cap
+        self.r.value = self.value.ctypes.data_as(_dp)
+        self.r.count = self.count.ctypes.data_as(_i64p)
+        self.r.type_count = self.type_count.ctypes.data_as(_i64p)
+        if want_raw and self.raw.size:
+            self.r.raw_tally = self.raw.ctypes.data_as(_i64p)
+            self.r.raw_tally_len = self.raw.size
+        self.T = prob.n_types
+
+    def result(self) -> Result:
+        n = self.r.n_out
+        return Result(self.value[:n].copy(), self.count[:n].copy(),
+                      self.type_count[:n, : max(self.T, 0)].copy(),
+                      self.raw.copy() if self.raw is not None else None,
+                      self.r.stats.as_dict())
+
+
+def simulate(prob: Problem, want_raw: bool = True) -> Result:
+    """procell_simulate(): one end-to-end run on the GPU."""
+    buf = _ResultBuffers(prob, want_raw)
+    _check(lib().procell_simulate(ctypes.byref(prob.params), ctypes.byref(buf.r)), "simulate")
+    return buf.result()
+
+
+class Plan:
+    """procell_plan_*: inputs resident on the device, runs enqueued on one stream."""
+
+    def __init__(self, prob: Problem, stream=None):
+        self.prob = prob
+        if stream is not None:
+            prob.params.stream = ctypes.c_void_p(int(stream))
+        self.handle = ctypes.c_void_p()
+        try:
+            _check(lib().procell_plan_create(ctypes.byref(prob.params), ctypes.byref(self.handle)), "plan_create")
+        finally:
+            prob.params.stream = None
+        self._buf = None
+
+    def run(self, seed: int):
+        _check(lib().procell_plan_run(self.handle, int(seed) & (2**64 - 1)), "plan_run")
+
+    def result(self, want_raw: bool = True) -> Result:
+        buf = _ResultBuffers(self.prob, want_raw)
+        _check(lib().procell_plan_result(self.handle, ctypes.byref(buf.r)), "plan_result")
+        return buf.result()
+
+    @property
+    def stream(self) -> int:
+        return lib().procell_plan_stream(self.handle) or 0
+
+    def close(self):
+        if self.handle:
+            lib().procell_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def debug_draws(seed, cells, nodes, attempts, device=0):
+    """Philox blocks and Box-Muller pairs computed by a device kernel."""
+    cells = np.ascontiguousarray(cells, dtype=np.uint64)
+    nodes = np.ascontiguousarray(nodes, dtype=np.uint64)
+    attempts = np.ascontiguousarray(attempts, dtype=np.uint32)
+    n = len(cells)
+    words = np.zeros((n, 4), dtype=np.uint32)
+    z = np.zeros((n, 2), dtype=np.float64)
+    _check(lib().procell_debug_draws(device, int(seed), cells.ctypes.data_as(_u64p), nodes.ctypes.data_as(_u64p),
+                                     attempts.ctypes.data_as(_u32p), n, words.ctypes.data_as(_u32p),
+                                     z.ctypes.data_as(_dp)), "debug_draws")
+    return words, z
+
+
+def read_histogram(path: str):
+    rows = ctypes.POINTER(Row)()
+    n = ctypes.c_uint64()
+    _check(lib().procell_read_histogram(path.encode(), ctypes.byref(rows), ctypes.byref(n)), "read_histogram")
+    try:
+        values = np.array([rows[i].value for i in range(n.value)], dtype=np.float64)
+        counts = np.array([rows[i].count for i in range(n.value)], dtype=np.uint64)
+    finally:
+        lib().procell_free(rows)
+    return values, counts
+
+
+def read_types(path: str):
+    types = ctypes.POINTER(CellType)()
+    n = ctypes.c_uint32()
+    _check(lib().procell_read_types(path.encode(), ctypes.byref(types), ctypes.byref(n)), "read_types")
+    try:
+        out = np.array([[types[i].proportion, types[i].mean, types[i].sd] for i in range(n.value)],
+                       dtype=np.float64).reshape(-1, 3)
+    finally:
+        lib().procell_free(types)
+    return out[:, 0].copy(), out[:, 1].copy(), out[:, 2].copy()
+
+
+def write_histogram(path, value, count, type_count=None, track_ratio=False):
+    """procell_write_histogram() on host arrays (no device needed)."""
+    value = np.ascontiguousarray(value, dtype=np.float64)
+    count = np.ascontiguousarray(count, dtype=np.int64)
+    r = ResultStruct()
+    r.n_out = r.capacity = len(value)
+    r.value = value.ctypes.data_as(_dp)
+    r.count = count.ctypes.data_as(_i64p)
+    T = 0
+    if type_count is not None:
+        type_count = np.ascontiguousarray(type_count, dtype=np.int64)
+        T = type_count.shape[1] if type_count.ndim == 2 else 0
+        r.type_count = type_count.ctypes.data_as(_i64p)
+    _check(lib().procell_write_histogram(path.encode() if path else None, ctypes.byref(r), T,
+                                         int(bool(track_ratio))), "write_histogram")

@@ -1,0 +1,106 @@
+"""Synthetic ProCell inputs with the shapes of the standard workloads.
+
+Each workload is a fluorescence histogram (value, count rows in file order),
+a cell-type table (proportion, mean, sd in file order; mean < 0 is
+quiescent), t_max and phi (0 means "smallest value with a nonzero count",
+the README rule for an absent -p).
+
+Values are computed with Python floats (libm pow/exp), counts with numpy's
+PCG64 Poisson sampler, so a given seed yields the same bytes on every host
+of this image.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Workload:
+    name: str
+    values: np.ndarray          # float64 [rows]
+    counts: np.ndarray          # uint64 [rows]
+    proportion: np.ndarray      # float64 [types]
+    mean: np.ndarray            # float64 [types]
+    sd: np.ndarray              # float64 [types]
+    t_max: float
+    phi: float                  # 0: auto
+    track_ratio: bool = True
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_cells(self) -> int:
+        return int(self.counts.sum())
+
+
+def _lognormal_counts(values, total, mu, sigma, rng):
+    lam = [math.exp(-((math.log(v) - mu) ** 2) / (2.0 * sigma * sigma)) for v in values]
+    s = sum(lam)
+    lam = np.array([total * x / s for x in lam], dtype=np.float64)
+    return rng.poisson(lam).astype(np.uint64)
+
+
+def small_histogram(seed: int = 0) -> Workload:
+    """64 log-spaced bins from 1 to 1e4, ~1e5 cells, the README's three
+    types (0.53 48.33 21.6 / 0.29 86.3 26.8 / 0.18 -1 -1), t_max 168,
+    phi = smallest populated bin."""
+    rng = np.random.default_rng([seed, 1])
+    values = [10.0 ** (4.0 * i / 63.0) for i in range(64)]
+    counts = _lognormal_counts(values, 1e5, math.log(800.0), 1.0, rng)
+    return Workload(
+        name="hist64_1e5_types3",
+        values=np.array(values, dtype=np.float64),
+        counts=counts,
+        proportion=np.array([0.53, 0.29, 0.18]),
+        mean=np.array([48.33, 86.3, -1.0]),
+        sd=np.array([21.6, 26.8, -1.0]),
+        t_max=168.0,
+        phi=0.0,
+        track_ratio=True,
+        meta={"bins": 64, "target_cells": 1e5},
+    )
+
+
+def large_histogram(seed: int = 0) -> Workload:
+    """1024 log-spaced bins from 1 to 1e5, ~1e7 cells, 4 proliferating types
+    (means U(30,100), sd = U(0.2,0.4) x mean) plus one quiescent type, ratios
+    Dirichlet(1,...,1), t_max 240, phi 50, per-type tracking."""
+    rng = np.random.default_rng([seed, 2])
+    values = [10.0 ** (5.0 * i / 1023.0) for i in range(1024)]
+    counts = _lognormal_counts(values, 1e7, math.log(800.0), 1.0, rng)
+    means = rng.uniform(30.0, 100.0, size=4)
+    sds = rng.uniform(0.2, 0.4, size=4) * means
+    ratios = rng.dirichlet(np.ones(5))
+    return Workload(
+        name="hist1024_1e7_types5",
+        values=np.array(values, dtype=np.float64),
+        counts=counts,
+        proportion=ratios.astype(np.float64),
+        mean=np.concatenate([means, [-1.0]]).astype(np.float64),
+        sd=np.concatenate([sds, [-1.0]]).astype(np.float64),
+        t_max=240.0,
+        phi=50.0,
+        track_ratio=True,
+        meta={"bins": 1024, "target_cells": 1e7},
+    )
+
+
+def scaled(w: Workload, factor: float, name: str | None = None) -> Workload:
+    """Same shape with every count multiplied by `factor` (rounded down)."""
+    counts = np.floor(w.counts.astype(np.float64) * factor).astype(np.uint64)
+    return Workload(name or f"{w.name}_x{factor:g}", w.values.copy(), counts,
+                    w.proportion.copy(), w.mean.copy(), w.sd.copy(), w.t_max,
+                    w.phi, w.track_ratio, dict(w.meta, scale=factor))
+
+
+def write_files(w: Workload, histogram_path: str, types_path: str) -> None:
+    """Write the two reference input files (floats in repr form, which
+    round-trips exactly)."""
+    with open(histogram_path, "w") as f:
+        for v, c in zip(w.values, w.counts):
+            f.write(f"{float(v)!r} {int(c)}\n")
+    with open(types_path, "w") as f:
+        for p, m, s in zip(w.proportion, w.mean, w.sd):
+            f.write(f"{float(p)!r} {float(m)!r} {float(s)!r}\n")

@@ -1,0 +1,170 @@
+/*
+ * procell.h -- C ABI of the MI355X (gfx950) ProCell proliferation library,
+ * libprocell.so.  Plain C types only; no HIP or torch types in signatures
+ * (streams travel as void*).
+ *
+ * The reference (cuda-pro-cell) has no C ABI; its seam between the driver
+ * (src/simulation/simulator.cu:10-56) and the simulation is a pair of C++
+ * calls plus the text-format helpers.  Each entry point below names the
+ * reference interface it replaces.
+ *
+ * Semantics (what one simulation computes) follow the reference's plain
+ * method: every initial cell of a fluorescence histogram draws a type by
+ * proportion; a proliferating cell (mean >= 0) whose next division completes
+ * by t_max and whose halved fluorescence stays above phi divides into two
+ * daughters with fresh Normal(mean, sd) timers truncated to > 0; a cell out
+ * of time is tallied at its fluorescence; a cell in time whose halved
+ * fluorescence would not exceed phi is dropped.  Random draws come from
+ * Philox4x32-10 keyed on (seed, cell id, tree path), so results are a pure
+ * function of the inputs and the seed and do not depend on the device, the
+ * tuning knobs or the launch shape.
+ */
+#ifndef PROCELL_H
+#define PROCELL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PROCELL_ABI_VERSION 1
+
+/* ---- status codes (never exit(); every entry point returns one) ---- */
+#define PROCELL_OK 0
+#define PROCELL_E_INVALID (-1)        /* null pointer, bad size or bad knob */
+#define PROCELL_E_PROPORTIONS (-2)    /* proportions do not sum to 1 within 1e-8 */
+#define PROCELL_E_ZERO_MEAN (-3)      /* a type with mean == 0 (timer undefined) */
+#define PROCELL_E_NONFINITE (-4)      /* NaN / inf in values, proportions, means, sds */
+#define PROCELL_E_PHI (-5)            /* resolved phi is not > 0 */
+#define PROCELL_E_DEPTH (-6)          /* a row would need more than 62 generations */
+#define PROCELL_E_TOO_MANY_CELLS (-7) /* more than 2^32-1 cells, 2^24 rows or 256 types */
+#define PROCELL_E_CAPACITY (-8)       /* result buffers too small */
+#define PROCELL_E_HIP (-9)            /* a HIP runtime call failed */
+#define PROCELL_E_NO_DEVICE (-10)     /* no usable gfx950 device */
+#define PROCELL_E_IO (-11)            /* file could not be written */
+#define PROCELL_E_OUT_OF_MEMORY (-12) /* host or device allocation failed */
+
+/* One histogram row: fluorescence value and number of cells.
+ * Replaces struct fluorescence (src/simulation/data_types.h:29-33). */
+typedef struct {
+    double value;
+    uint64_t count;
+} procell_row;
+
+/* One cell type, in file order (its index is the reference's `name`).
+ * Replaces struct cell_type (src/simulation/data_types.h:21-27); mean < 0
+ * marks a quiescent type (README.md:84-90). */
+typedef struct {
+    double proportion;
+    double mean;
+    double sd;
+} procell_type;
+
+/* Launch knobs.  0 means "choose automatically".  Results never depend on
+ * them; only speed does. */
+typedef struct {
+    uint32_t block_size;    /* threads per workgroup of the tree kernel: 64..256, multiple of 64 */
+    uint32_t blocks_per_cu; /* resident workgroups per CU of the persistent tree kernel */
+    uint32_t chunk;         /* initial cells a wavefront claims from the work queue at once */
+    uint32_t flags;         /* PROCELL_FLAG_* */
+} procell_tuning;
+
+#define PROCELL_FLAG_TIMING 1u   /* record per-kernel device times into procell_stats */
+
+typedef struct {
+    const procell_row *rows;     /* histogram rows in file order; zero counts are skipped */
+    uint64_t n_rows;
+    const procell_type *types;   /* cell types in file order */
+    uint32_t n_types;
+    double t_max;                /* >= 0 (may be +inf) */
+    double phi;                  /* > 0, or <= 0 for "smallest value with a nonzero count" */
+    uint64_t seed;
+    int32_t track_ratio;         /* nonzero: fill per-type columns */
+    int32_t device;              /* HIP device ordinal */
+    procell_tuning tuning;
+    void *stream;                /* hipStream_t for the plan API; NULL = library-owned stream */
+} procell_params;
+
+typedef struct {
+    uint64_t divisions;   /* cells that divided into two daughters */
+    uint64_t recorded;    /* cells tallied into the output histogram */
+    uint64_t dropped;     /* cells in time whose halved fluorescence would not exceed phi */
+    uint64_t lost;        /* initial cells out of time with value < phi (not in the output bins) */
+    uint64_t rejected;    /* normal draws rejected because the timer was <= 0 */
+    uint64_t cells;       /* initial cells */
+    uint64_t pool;        /* initial cells that divided at least once */
+    double ms_roots;      /* device time of the initial-cell kernel (PROCELL_FLAG_TIMING) */
+    double ms_tree;       /* device time of the division-tree kernel (PROCELL_FLAG_TIMING) */
+    double ms_run;        /* device time of the whole run (PROCELL_FLAG_TIMING) */
+} procell_stats;
+
+typedef struct {
+    uint64_t capacity;     /* rows available in value / count (/ type_count) */
+    double *value;         /* [capacity] output fluorescence values, ascending */
+    int64_t *count;        /* [capacity] cells per value */
+    int64_t *type_count;   /* [capacity * n_types] per-type counts, file order; may be NULL */
+    uint64_t n_out;        /* out: rows written (only values with count > 0) */
+    int64_t *raw_tally;    /* optional [nonzero_rows][kdim][n_types] counts by (row, generation, type) */
+    uint64_t raw_tally_len;/* elements available in raw_tally */
+    procell_stats stats;   /* out */
+} procell_result;
+
+/* Run one simulation end to end (upload, kernels, download, merge).
+ * Replaces create_cells_population (src/simulation/cells_population.h:12-18)
+ * followed by proliferate (src/simulation/proliferation.h:12-19).  Unlike the
+ * reference, results are reproducible for a given seed, counts are 64-bit and
+ * errors are returned instead of ignored. */
+int procell_simulate(const procell_params *p, procell_result *r);
+
+/* Number of distinct output values (the output bin set: every v / 2^k >= phi
+ * of every nonzero row), an upper bound for r->capacity.
+ * Replaces the bin-set part of load_fluorescences (src/io/parser.h:15-21). */
+int procell_output_capacity(const procell_params *p, uint64_t *capacity);
+
+/* Shape of raw_tally: number of nonzero rows and generations + 1.
+ * No reference counterpart (the reference tallies by value only). */
+int procell_tally_shape(const procell_params *p, uint64_t *nonzero_rows, uint32_t *kdim);
+
+/* Resolved phi (the README rule for an absent -p, src/io/parser.cu:80-96). */
+int procell_resolve_phi(const procell_params *p, double *phi);
+
+const char *procell_strerror(int code);
+int procell_abi_version(void);
+
+/* ---- plan API: inputs uploaded once, many runs on one stream ----------
+ * No reference counterpart; it splits procell_simulate so that callers
+ * (benchmarks, batch drivers) can keep inputs resident in device memory. */
+typedef struct procell_plan procell_plan;
+int procell_plan_create(const procell_params *p, procell_plan **plan);
+int procell_plan_run(procell_plan *plan, uint64_t seed);            /* enqueue only */
+int procell_plan_result(procell_plan *plan, procell_result *r);     /* wait, download, merge */
+int procell_plan_destroy(procell_plan *plan);
+void *procell_plan_stream(procell_plan *plan);                       /* hipStream_t */
+
+/* ---- text formats ------------------------------------------------------
+ * Replace load_fluorescences' file reading (src/io/parser.cu:68-124),
+ * load_cell_types (src/io/parser.cu:156-185) and save_fluorescences
+ * (src/io/parser.cu:187-217).  Arrays are malloc'ed; release with
+ * procell_free.  A file that cannot be opened reads as empty, as in the
+ * reference. */
+int procell_read_histogram(const char *path, procell_row **rows, uint64_t *n_rows);
+int procell_read_types(const char *path, procell_type **types, uint32_t *n_types);
+/* path == NULL writes to stdout.  Format: "%.10g\t%lld[\t%lld...]\n" per row. */
+int procell_write_histogram(const char *path, const procell_result *r, uint32_t n_types,
+                            int32_t track_ratio);
+void procell_free(void *ptr);
+
+/* ---- diagnostics: the random stream evaluated on the device ------------
+ * For each i: words[4i..4i+3] = Philox block of (seed, cell[i], node[i],
+ * attempt[i]); z[2i], z[2i+1] = the Box-Muller pair of that block.  Host
+ * pointers; runs one small kernel.  No reference counterpart. */
+int procell_debug_draws(int32_t device, uint64_t seed, const uint64_t *cell, const uint64_t *node,
+                        const uint32_t *attempt, uint64_t n, uint32_t *words, double *z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PROCELL_H */
