@@ -161,6 +161,7 @@ struct procell_plan {
     RunArgs args{};
     void *d_block = nullptr;  // one allocation holding every table
     size_t tally_len = 0;
+    size_t zero_words = 0;    // counters, tally and segment cursors, cleared per run
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool ran = false;
 };
@@ -180,7 +181,7 @@ thread_local hipError_t g_last_hip = hipSuccess;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-int check_device(int device, int *n_cu)
+int check_device(int device, int *n_cu, size_t *lds_per_cu = nullptr, size_t *lds_per_block = nullptr)
 {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count)
@@ -191,6 +192,10 @@ int check_device(int device, int *n_cu)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return PROCELL_E_NO_DEVICE;
     *n_cu = prop.multiProcessorCount;
+    if (lds_per_cu)
+        *lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
+    if (lds_per_block)
+        *lds_per_block = prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 65536;
     return PROCELL_OK;
 }
 
@@ -207,6 +212,73 @@ void destroy_plan(procell_plan *pl)
     if (pl->own_stream && pl->stream)
         (void)hipStreamDestroy(pl->stream);
     delete pl;
+}
+
+// Expected work of one initial cell of a row, in units of one division: the
+// root step plus the expected number of divisions, with division k+1
+// completing at D = tau0 (1 + U) + tau_1 + ... + tau_k approximated as normal
+// (truncation ignored).  Only used to balance the workgroups' segments.
+double row_cost(const Setup &s, uint32_t row)
+{
+    const int kcap = (int)(s.row_info[row] & 0xFFu);
+    double w = 1.3;
+    double p_prev = 0.0;
+    for (uint32_t j = 0; j < s.n_types; ++j) {
+        const double m = s.type_mean[j];
+        const double sd = fabs(s.type_sd[j]);
+        const double p = s.type_cum[j] - p_prev;
+        p_prev = s.type_cum[j];
+        if (m < 0.0)
+            continue;
+        double e = 0.0;
+        for (int k = 0; k < kcap; ++k) {
+            const double mu = 1.5 * m + k * m;
+            const double var = (7.0 / 3.0) * sd * sd + m * m / 12.0 + k * sd * sd;
+            const double z = (s.t_max - mu) / sqrt(var > 0.0 ? var : 1e-300);
+            const double prob = 0.5 * erfc(-z / sqrt(2.0));
+            const double add = ldexp(prob, k);
+            e += add;
+            if (prob < 1e-12)
+                break;
+        }
+        w += p * e;
+    }
+    return w;
+}
+
+// Split the initial cells into `g` contiguous segments of about equal cost.
+void split_segments(const Setup &s, uint32_t g, std::vector<uint64_t> &begin, std::vector<uint32_t> &row_of)
+{
+    const uint32_t n_rows = (uint32_t)s.v.size();
+    begin.assign(g + 1, s.n_cells);
+    row_of.assign(g, n_rows ? n_rows - 1 : 0);
+    begin[0] = 0;
+    std::vector<double> w(n_rows);
+    double total = 0.0;
+    for (uint32_t r = 0; r < n_rows; ++r) {
+        w[r] = row_cost(s, r);
+        total += w[r] * (double)(s.start[r + 1] - s.start[r]);
+    }
+    uint32_t next = 1;
+    double acc = 0.0;
+    for (uint32_t r = 0; r < n_rows && next < g; ++r) {
+        const uint64_t n = s.start[r + 1] - s.start[r];
+        const double row_total = w[r] * (double)n;
+        while (next < g && acc + row_total >= total * next / g) {
+            const double want = total * next / g - acc;
+            uint64_t cut = s.start[r] + (uint64_t)(want / w[r]);
+            if (cut > s.start[r + 1])
+                cut = s.start[r + 1];
+            if (cut < begin[next - 1])
+                cut = begin[next - 1];
+            begin[next++] = cut;
+        }
+        acc += row_total;
+    }
+    for (uint32_t i = 0; i < g; ++i) {
+        const uint64_t c = begin[i] < s.n_cells ? begin[i] : (s.n_cells ? s.n_cells - 1 : 0);
+        row_of[i] = (uint32_t)(std::upper_bound(s.start.begin(), s.start.end() - 1, c) - s.start.begin()) - 1;
+    }
 }
 
 int create_plan(const procell_params *p, procell_plan **out)
@@ -226,7 +298,8 @@ int create_plan(const procell_params *p, procell_plan **out)
         return rc;
     }
     int n_cu = 0;
-    rc = check_device(p->device, &n_cu);
+    size_t lds_per_cu = 0, lds_per_block = 0;
+    rc = check_device(p->device, &n_cu, &lds_per_cu, &lds_per_block);
     if (rc != PROCELL_OK) {
         delete pl;
         return rc;
@@ -239,22 +312,91 @@ int create_plan(const procell_params *p, procell_plan **out)
     const uint32_t T = s.n_types ? s.n_types : 1;
     const uint32_t kdim = (uint32_t)s.kmax + 1;
     pl->tally_len = (size_t)n_rows * kdim * T;
+    auto fail = [&](int code) {
+        destroy_plan(pl);
+        return code;
+    };
 
-    // one device allocation: counters | tally | row_start | row_info | types | pool
+    // ---- launch shape: block size, resident workgroups, LDS split ----
+    LaunchShape &ls = pl->shape;
+    RunArgs &a = pl->args;
+    a.stack_levels = kdim > 1 ? kdim - 1 : 1;  // levels 1 .. kmax
+    int bs = tu.block_size ? (int)tu.block_size : 256;
+    int occ = 0;
+    for (;;) {
+        const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs);
+        int occ_v = 0;
+        if (tree_occupancy(bs, base, &occ_v) != hipSuccess)
+            return fail(PROCELL_E_HIP);
+        if (occ_v >= 1) {
+            occ = occ_v;
+            break;
+        }
+        if (bs == 64)
+            return fail(PROCELL_E_DEPTH);
+        bs -= 64;  // deep trees: a smaller workgroup keeps the stack within LDS
+    }
+    if (tu.blocks_per_cu)
+        occ = std::min<int>(occ, (int)tu.blocks_per_cu);
+    ls.tree_block_size = bs;
+    const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs);
+    const size_t budget = std::min(lds_per_cu / (size_t)occ, lds_per_block);
+    const size_t row_bytes = (size_t)kdim * T * 4;
+    uint32_t win_rows = budget > base + 16 ? (uint32_t)((budget - base - 16) / row_bytes) : 0;
+    const uint32_t g = (uint32_t)(n_cu * occ);
+    std::vector<uint64_t> seg_begin;
+    std::vector<uint32_t> seg_row;
+    split_segments(s, g, seg_begin, seg_row);
+    // no more window rows than a segment spans
+    uint32_t span = 1;
+    for (uint32_t i = 0; i < g; ++i) {
+        const uint64_t last = seg_begin[i + 1] > seg_begin[i] ? seg_begin[i + 1] - 1 : seg_begin[i];
+        const uint64_t lc = last < s.n_cells ? last : (s.n_cells ? s.n_cells - 1 : 0);
+        const uint32_t lr = (uint32_t)(std::upper_bound(s.start.begin(), s.start.end() - 1, lc) - s.start.begin()) - 1;
+        span = std::max(span, lr - seg_row[i] + 1);
+    }
+    win_rows = std::min(win_rows, span);
+    // u32 window counters must not overflow: bound the leaves any window can see
+    for (uint32_t i = 0; i < g && win_rows; ++i) {
+        double bound = 0.0;
+        for (uint32_t r = seg_row[i]; r < std::min<uint32_t>(n_rows, seg_row[i] + win_rows); ++r)
+            bound += (double)(s.start[r + 1] - s.start[r]) * ldexp(1.0, (int)(s.row_info[r] & 0xFFu));
+        if (bound >= 4.0e9)
+            win_rows = 0;
+    }
+    // the window must not cost a resident workgroup (LDS allocation granularity)
+    for (;;) {
+        ls.tree_lds_bytes = tree_lds_bytes(win_rows, kdim, T, a.stack_levels, (uint32_t)bs);
+        int occ_w = 0;
+        if (tree_occupancy(bs, ls.tree_lds_bytes, &occ_w) != hipSuccess)
+            return fail(PROCELL_E_HIP);
+        // keep a margin for the per-workgroup LDS allocation granularity
+        const bool fits = (size_t)occ * align_up(ls.tree_lds_bytes, 2048) <= lds_per_cu;
+        if ((occ_w >= occ && fits) || win_rows == 0)
+            break;
+        win_rows = win_rows > 8 ? win_rows - win_rows / 8 : 0;
+    }
+    ls.tree_blocks = (int)g;
+    a.win_rows = win_rows;
+    a.claim = tu.chunk ? tu.chunk : 1024;
+    a.steal_claim = std::max<uint32_t>(64, a.claim / 4);
+    a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 16;
+
+    // ---- one device allocation: counters | tally | seg_next | tables ----
     size_t off = 0;
-    const size_t o_cnt = off; off = align_up(off + kNumCounters * 8, 256);
-    const size_t o_tally = off; off = align_up(off + pl->tally_len * 8, 256);
+    const size_t o_cnt = off; off += kNumCounters * 8;
+    const size_t o_tally = off; off += pl->tally_len * 8;
+    const size_t o_next = off; off += (size_t)g * 8;
+    pl->zero_words = off / 8;
+    off = align_up(off, 256);
     const size_t o_rs = off; off = align_up(off + (size_t)(n_rows + 1) * 8, 256);
     const size_t o_ri = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
     const size_t o_cum = off; off = align_up(off + (size_t)T * 8, 256);
     const size_t o_mean = off; off = align_up(off + (size_t)T * 8, 256);
     const size_t o_sd = off; off = align_up(off + (size_t)T * 8, 256);
     const size_t o_tf = off; off = align_up(off + (size_t)T * 4, 256);
-    const size_t o_pool = off; off = align_up(off + (size_t)(s.n_cells + 1) * sizeof(PoolEntry), 256);
-    auto fail = [&](int code) {
-        destroy_plan(pl);
-        return code;
-    };
+    const size_t o_sb = off; off = align_up(off + (size_t)(g + 1) * 8, 256);
+    const size_t o_sr = off; off = align_up(off + (size_t)g * 4, 256);
     if (p->stream) {
         pl->stream = (hipStream_t)p->stream;
     } else {
@@ -264,61 +406,33 @@ int create_plan(const procell_params *p, procell_plan **out)
     }
     if (hipMalloc(&pl->d_block, off) != hipSuccess)
         return fail(PROCELL_E_OUT_OF_MEMORY);
-    char *base = (char *)pl->d_block;
-    RunArgs &a = pl->args;
+    char *dbase = (char *)pl->d_block;
     a.t_max = s.t_max;
     a.n_cells = s.n_cells;
     a.n_rows = n_rows;
     a.n_types = T;
     a.kdim = kdim;
-    a.counters = (unsigned long long *)(base + o_cnt);
-    a.tally = (unsigned long long *)(base + o_tally);
-    a.row_start = (const uint64_t *)(base + o_rs);
-    a.row_info = (const uint32_t *)(base + o_ri);
-    a.type_cum = (const double *)(base + o_cum);
-    a.type_mean = (const double *)(base + o_mean);
-    a.type_sd = (const double *)(base + o_sd);
-    a.type_file = (const uint32_t *)(base + o_tf);
-    a.pool = (PoolEntry *)(base + o_pool);
-    if (n_rows) {
-        if (hipMemcpyAsync(base + o_rs, s.start.data(), (n_rows + 1) * 8, hipMemcpyHostToDevice, pl->stream) != hipSuccess ||
-            hipMemcpyAsync(base + o_ri, s.row_info.data(), n_rows * 4, hipMemcpyHostToDevice, pl->stream) != hipSuccess)
-            return fail(PROCELL_E_HIP);
-    }
-    if (s.n_types) {
-        if (hipMemcpyAsync(base + o_cum, s.type_cum.data(), s.n_types * 8, hipMemcpyHostToDevice, pl->stream) != hipSuccess ||
-            hipMemcpyAsync(base + o_mean, s.type_mean.data(), s.n_types * 8, hipMemcpyHostToDevice, pl->stream) != hipSuccess ||
-            hipMemcpyAsync(base + o_sd, s.type_sd.data(), s.n_types * 8, hipMemcpyHostToDevice, pl->stream) != hipSuccess ||
-            hipMemcpyAsync(base + o_tf, s.type_file.data(), s.n_types * 4, hipMemcpyHostToDevice, pl->stream) != hipSuccess)
-            return fail(PROCELL_E_HIP);
-    }
+    a.counters = (unsigned long long *)(dbase + o_cnt);
+    a.tally = (unsigned long long *)(dbase + o_tally);
+    a.seg_next = (unsigned long long *)(dbase + o_next);
+    a.row_start = (const uint64_t *)(dbase + o_rs);
+    a.row_info = (const uint32_t *)(dbase + o_ri);
+    a.type_cum = (const double *)(dbase + o_cum);
+    a.type_mean = (const double *)(dbase + o_mean);
+    a.type_sd = (const double *)(dbase + o_sd);
+    a.type_file = (const uint32_t *)(dbase + o_tf);
+    a.seg_begin = (const uint64_t *)(dbase + o_sb);
+    a.seg_row = (const uint32_t *)(dbase + o_sr);
+    auto up = [&](size_t o, const void *src, size_t bytes) {
+        return bytes == 0 || hipMemcpyAsync(dbase + o, src, bytes, hipMemcpyHostToDevice, pl->stream) == hipSuccess;
+    };
+    if (!up(o_rs, s.start.data(), s.start.size() * 8) || !up(o_ri, s.row_info.data(), n_rows * 4) ||
+        !up(o_cum, s.type_cum.data(), s.type_cum.size() * 8) || !up(o_mean, s.type_mean.data(), s.type_mean.size() * 8) ||
+        !up(o_sd, s.type_sd.data(), s.type_sd.size() * 8) || !up(o_tf, s.type_file.data(), s.type_file.size() * 4) ||
+        !up(o_sb, seg_begin.data(), seg_begin.size() * 8) || !up(o_sr, seg_row.data(), seg_row.size() * 4))
+        return fail(PROCELL_E_HIP);
     if (hipStreamSynchronize(pl->stream) != hipSuccess)
         return fail(PROCELL_E_HIP);
-
-    // launch shape
-    LaunchShape &ls = pl->shape;
-    ls.roots_block_size = 256;
-    ls.roots_window_rows = 16;
-    int rb = 0;
-    if (roots_occupancy(ls.roots_block_size, (size_t)ls.roots_window_rows * T * 4, &rb) != hipSuccess || rb < 1)
-        rb = 1;
-    const uint64_t tiles = (s.n_cells + ls.roots_block_size - 1) / ls.roots_block_size;
-    ls.roots_blocks = (int)std::min<uint64_t>(std::max<uint64_t>(tiles, 1), (uint64_t)n_cu * rb);
-    ls.tree_block_size = tu.block_size ? (int)tu.block_size : 256;
-    ls.tree_lds_bytes = (size_t)kdim * ls.tree_block_size * sizeof(double);
-    if (ls.tree_lds_bytes > 64 * 1024) {
-        // deep trees: shrink the workgroup so its stack fits
-        while (ls.tree_block_size > 64 && (size_t)kdim * ls.tree_block_size * 8 > 64 * 1024)
-            ls.tree_block_size -= 64;
-        ls.tree_lds_bytes = (size_t)kdim * ls.tree_block_size * sizeof(double);
-    }
-    int tb = 0;
-    if (tree_occupancy(ls.tree_block_size, ls.tree_lds_bytes, &tb) != hipSuccess || tb < 1)
-        tb = 1;
-    if (tu.blocks_per_cu)
-        tb = std::min<int>(tb, (int)tu.blocks_per_cu);
-    ls.tree_blocks = n_cu * tb;
-    ls.chunk = tu.chunk ? tu.chunk : 64;
     if (pl->timing) {
         for (hipEvent_t &e : pl->ev)
             if (hipEventCreate(&e) != hipSuccess)
@@ -332,12 +446,9 @@ int run_plan(procell_plan *pl, uint64_t seed)
 {
     HIP_TRY(hipSetDevice(pl->device));
     pl->args.seed = seed;
-    // counters and tally are contiguous at the start of the block
-    const uint64_t n_zero = (uint64_t)((char *)pl->args.tally - (char *)pl->args.counters) / 8 + pl->tally_len;
-    HIP_TRY(launch_zero(pl->args.counters, n_zero, pl->stream));
     if (pl->timing)
         HIP_TRY(hipEventRecord(pl->ev[0], pl->stream));
-    HIP_TRY(launch_roots(pl->args, pl->shape, pl->stream));
+    HIP_TRY(launch_zero(pl->args.counters, pl->zero_words, pl->stream));
     if (pl->timing)
         HIP_TRY(hipEventRecord(pl->ev[1], pl->stream));
     HIP_TRY(launch_tree(pl->args, pl->shape, pl->stream));
@@ -374,10 +485,12 @@ int plan_result(procell_plan *pl, procell_result *r)
     st.rejected = cnt[kRejected];
     st.cells = cnt[kCells];
     st.pool = cnt[kPoolCount];
+    st.workgroups = (uint32_t)pl->shape.tree_blocks;
+    st.block_size = (uint32_t)pl->shape.tree_block_size;
+    st.lds_bytes = (uint32_t)pl->shape.tree_lds_bytes;
+    st.window_rows = pl->args.win_rows;
     if (pl->timing) {
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, pl->ev[0], pl->ev[1]));
-        st.ms_roots = ms;
         HIP_TRY(hipEventElapsedTime(&ms, pl->ev[1], pl->ev[2]));
         st.ms_tree = ms;
         HIP_TRY(hipEventElapsedTime(&ms, pl->ev[0], pl->ev[2]));

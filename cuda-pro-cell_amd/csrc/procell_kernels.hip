@@ -2,25 +2,32 @@
 //
 // The reference materialises every tree level as a dense array of 40-byte
 // cells and recurses with device-side launches (src/simulation/
-// proliferation.cu:289-402).  Here the work is split into two kernels and no
-// cell is ever stored:
+// proliferation.cu:289-402), creating the initial cells in a separate pass
+// (cells_population.cu:74-117).  Here one persistent kernel does both and no
+// cell is ever stored in device memory:
 //
-//  roots_kernel  one lane per initial cell (cell.cu:25-143,
-//                cells_population.cu:97-117): type draw, truncated first
-//                timer, phase; tallies cells that never divide and appends
-//                the others (16 B each) to a work queue.
-//  tree_kernel   persistent: each lane walks one initial cell's division tree
-//                depth first (proliferation.cu:317-381).  A step is one
-//                division: one Philox block and one Box-Muller pair give both
-//                daughters' timers; daughters that will not divide are
-//                tallied or dropped on the spot, so the lane's next node is
-//                always a dividing one (the left daughter, else the right,
-//                else the deepest pending right daughter from the lane's
-//                column of an LDS stack, else a fresh queue entry).  Wavefronts
-//                claim queue entries in chunks with one atomic.
-//
-// Tallies are int64 counts by (nonzero row, generation, type); the host maps
-// (row, generation) to the output value row * 2^-generation.
+//  * Work queue.  The initial cells are split on the host into one
+//    contiguous segment per workgroup, of roughly equal expected work.  A
+//    wavefront claims cells from its workgroup's segment with one atomic per
+//    claim; when the segment is exhausted it claims from the following
+//    segments (work stealing across workgroups).
+//  * Root step (cell.cu:25-143).  All 64 lanes set up one claimed initial
+//    cell each: type draw, truncated first timer, phase.  Cells that never
+//    divide are tallied or dropped at once; dividing ones go to the
+//    wavefront's ring buffer in LDS.
+//  * Division step (proliferation.cu:317-381).  Each busy lane divides its
+//    current cell: one Philox block and one Box-Muller pair give both
+//    daughters' timers; daughters that will not divide are tallied or dropped
+//    on the spot, so the lane's next cell is always a dividing one (the left
+//    daughter, else the right, else the deepest pending right daughter from
+//    the lane's column of the LDS stack, else a buffered root).
+//  * Once no initial cell is left anywhere, idle lanes take the shallowest
+//    pending right daughter of a busy lane in the same wavefront (work
+//    stealing inside the wavefront).
+//  * Tallies (counts by nonzero row, generation, type) accumulate in a u32
+//    LDS window over the first rows of the workgroup's own segment and are
+//    added to the int64 device tally at the end; rows outside the window use
+//    64-bit device atomics directly.
 #include "procell_internal.hpp"
 #include "procell_stream.hpp"
 
@@ -28,6 +35,10 @@ namespace procell {
 namespace {
 
 using dev::Words;
+
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kRing = 128;  // buffered dividing roots per wavefront
+constexpr uint32_t kNoWindow = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
 {
@@ -43,9 +54,28 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
     return x;
 }
 
-__device__ __forceinline__ uint32_t find_row(const uint64_t *row_start, uint32_t n_rows, uint64_t cell)
+// Index of the r-th set bit of m (r < popcount(m)).
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r)
 {
-    uint32_t lo = 0, hi = n_rows;
+    uint32_t base = 0;
+#pragma unroll
+    for (uint32_t width = 32; width >= 1; width >>= 1) {
+        const uint64_t low = m & ((width == 64 ? ~0ull : (1ull << width)) - 1ull);
+        const uint32_t c = (uint32_t)__popcll(low);
+        if (r >= c) {
+            r -= c;
+            m >>= width;
+            base += width;
+        } else {
+            m = low;
+        }
+    }
+    return base;
+}
+
+// Last row in [lo, hi) whose first cell is <= cell (row_start[lo] <= cell).
+__device__ __forceinline__ uint32_t find_row(const uint64_t *row_start, uint32_t lo, uint32_t hi, uint64_t cell)
+{
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (row_start[mid] <= cell)
@@ -56,9 +86,9 @@ __device__ __forceinline__ uint32_t find_row(const uint64_t *row_start, uint32_t
     return lo;
 }
 
-// Redraw candidates of a node's truncated-normal stream from attempt `first`
-// until one timer is > 0 (cell.cu:106-122 redraws while the timer is <= 0).
-// Every rejected candidate is counted.
+// Redraws from a node's truncated-normal stream, attempts 1, 2, ..., until a
+// timer is > 0 (cell.cu:106-122 redraws while the timer is <= 0).  Every
+// rejected candidate is counted.
 __device__ __forceinline__ double truncated_timer(uint64_t seed, uint64_t cell, uint64_t node,
                                                   double mean, double sd, uint32_t &rejected)
 {
@@ -78,9 +108,6 @@ __device__ __forceinline__ double truncated_timer(uint64_t seed, uint64_t cell, 
     return mean;
 }
 
-// ---------------------------------------------------------------------------
-// Zero fill (used instead of hipMemsetAsync so that runs can be captured).
-// ---------------------------------------------------------------------------
 __global__ void zero_kernel(unsigned long long *p, uint64_t n)
 {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -88,240 +115,420 @@ __global__ void zero_kernel(unsigned long long *p, uint64_t n)
         p[i] = 0ull;
 }
 
-// ---------------------------------------------------------------------------
-// Initial cells.  Tiles of blockDim consecutive cells; cells of a tile span
-// few rows, so generation-0 tallies go through an LDS window of `win_rows`
-// rows starting at the tile's first row (global atomics beyond it).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) roots_kernel(RunArgs a, uint32_t win_rows)
+struct RootSlot {
+    double s0;
+    uint32_t cell;
+    uint32_t row_type;  // row | sorted type << 24
+    uint32_t kcap;
+    uint32_t pad[3];
+};
+
+// First row of a claimed range: 64-ary search with one load per lane per
+// round over rows [lo, hi) (row_start[lo] <= cell < row_start[hi]).
+__device__ __forceinline__ uint32_t find_row_wave(const uint64_t *row_start, uint32_t lo, uint32_t hi,
+                                                  uint64_t cell, uint32_t lane)
 {
-    extern __shared__ unsigned int win[];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t bdim = blockDim.x;
-    const uint32_t T = a.n_types;
-    const uint32_t win_len = win_rows * T;
-    const uint64_t n_tiles = (a.n_cells + bdim - 1) / bdim;
-    uint32_t n_rec = 0, n_drop = 0, n_lost = 0, n_rej = 0, n_cells = 0;
-
-    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        const uint64_t c0 = tile * bdim;
-        const uint32_t row0 = find_row(a.row_start, a.n_rows, c0);
-        for (uint32_t i = tid; i < win_len; i += bdim)
-            win[i] = 0u;
-        __syncthreads();
-
-        const uint64_t cell = c0 + tid;
-        bool divides = false;
-        double s0 = 0.0;
-        uint32_t row = 0, jt = 0;
-        if (cell < a.n_cells) {
-            ++n_cells;
-            row = find_row(a.row_start, a.n_rows, cell);
-            const uint32_t info = a.row_info[row];
-            const uint32_t kcap = info & 0xFFu;
-            const bool ge_phi = (info >> 8) & 1u;
-            const Words w = dev::draw_block(a.seed, cell, 0, 0);
-            const double ut = dev::uniform_co(w.w0, w.w1);
-            const double U = dev::uniform_oc(w.w2, w.w3);
-            while (jt + 1 < T && !(ut < a.type_cum[jt]))
-                ++jt;
-            const double mean = a.type_mean[jt];
-            bool record = false;
-            if (mean < 0.0) {
-                record = true;
-            } else {
-                const double tau0 = truncated_timer(a.seed, cell, 1, mean, a.type_sd[jt], n_rej);
-                const double t0 = tau0 * U;
-                s0 = t0 + tau0;
-                if (s0 > a.t_max)
-                    record = true;
-                else if (kcap == 0)
-                    ++n_drop;
-                else
-                    divides = true;
-            }
-            if (record) {
-                if (ge_phi) {
-                    ++n_rec;
-                    const uint32_t tf = a.type_file[jt];
-                    const uint32_t off = row - row0;
-                    if (off < win_rows)
-                        atomicAdd(&win[off * T + tf], 1u);
-                    else
-                        atomicAdd(&a.tally[(uint64_t)row * a.kdim * T + tf], 1ull);
-                } else {
-                    ++n_lost;
-                }
-            }
-        }
-        // append dividing cells to the queue: one atomic per wavefront
-        const uint64_t dmask = __ballot(divides);
-        if (dmask) {
-            unsigned long long base = 0;
-            if (__lane_id() == 0)
-                base = atomicAdd(&a.counters[kPoolCount], (unsigned long long)__popcll(dmask));
-            base = __shfl(base, 0);
-            if (divides) {
-                PoolEntry e;
-                e.s0 = s0;
-                e.cell = (uint32_t)cell;
-                e.row_type = row | (jt << 24);
-                a.pool[base + lane_rank(dmask)] = e;
-            }
-        }
-        __syncthreads();
-        for (uint32_t i = tid; i < win_len; i += bdim) {
-            const uint32_t c = win[i];
-            if (c) {
-                const uint32_t r = row0 + i / T;
-                atomicAdd(&a.tally[(uint64_t)r * a.kdim * T + (i % T)], (unsigned long long)c);
-            }
-        }
-        __syncthreads();
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + kWave - 1) / kWave;
+        const uint32_t cand = lo + lane * step;
+        const bool le = cand < hi && row_start[cand] <= cell;
+        const uint64_t m = __ballot(le);
+        const uint32_t top = 63u - (uint32_t)__builtin_clzll(m);  // lane 0 always qualifies
+        const uint32_t new_lo = lo + top * step;
+        const uint32_t new_hi = new_lo + step < hi ? new_lo + step : hi;
+        lo = new_lo;
+        hi = new_hi;
     }
-
-    n_rec = wave_sum(n_rec);
-    n_drop = wave_sum(n_drop);
-    n_lost = wave_sum(n_lost);
-    n_rej = wave_sum(n_rej);
-    n_cells = wave_sum(n_cells);
-    if (__lane_id() == 0) {
-        if (n_rec) atomicAdd(&a.counters[kRecorded], (unsigned long long)n_rec);
-        if (n_drop) atomicAdd(&a.counters[kDropped], (unsigned long long)n_drop);
-        if (n_lost) atomicAdd(&a.counters[kLost], (unsigned long long)n_lost);
-        if (n_rej) atomicAdd(&a.counters[kRejected], (unsigned long long)n_rej);
-        if (n_cells) atomicAdd(&a.counters[kCells], (unsigned long long)n_cells);
-    }
+    return lo;
 }
 
-// ---------------------------------------------------------------------------
-// Division trees, depth first, one tree per lane at a time.
-// LDS: stack[level * blockDim + tid] = completion time of the pending right
-// daughter at `level` (bit `level` of the lane's `pend` mask).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) tree_kernel(RunArgs a, uint32_t chunk)
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x)
 {
-    extern __shared__ double stack[];
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)x);
+}
+
+// Lane state: the dividing cell a lane currently holds.
+struct Lane {
+    uint64_t cell, node, pend;
+    uint64_t tb_glob;   // device tally index of (row, generation 0, type)
+    double s, mean, sd;
+    uint32_t k, kcap;
+    uint32_t tb_lds;    // window index of (row, generation 0, type), or kNoWindow
+    bool active;
+};
+
+struct Counts {
+    uint32_t div, rec, drop, rej, lost, cells, pool;
+};
+
+// LDS layout (dynamic): type tables (cum, mean, sd: double [T]; file: u32
+// [T]) | window rows (first cell: u64 [win_rows + 1]; info: u32 [win_rows])
+// | window tally u32 [win_rows * kdim * T] | stack double [stack_levels *
+// blockDim] | ring RootSlot [waves * kRing]
+__global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t bdim = blockDim.x;
+    const uint32_t lane = tid & (kWave - 1);
+    const uint32_t wave = tid / kWave;
     const uint32_t T = a.n_types;
-    const uint64_t pool_n = __atomic_load_n(&a.counters[kPoolCount], __ATOMIC_RELAXED);
+    const uint32_t kdim = a.kdim;
+    const uint32_t win_len = a.win_rows * kdim * T;
+    double *t_cum = (double *)lds;
+    double *t_mean = t_cum + T;
+    double *t_sd = t_mean + T;
+    uint32_t *t_file = (uint32_t *)(t_sd + T);
+    uint64_t *w_start = (uint64_t *)(lds + ((size_t)T * 28 + 15) / 16 * 16);
+    uint32_t *w_info = (uint32_t *)(w_start + a.win_rows + 1);
+    unsigned int *win = (unsigned int *)((unsigned char *)w_start + ((size_t)a.win_rows * 12 + 8 + 15) / 16 * 16);
+    double *stack = (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
+    RootSlot *ring = (RootSlot *)(stack + (size_t)a.stack_levels * bdim) + wave * (kRing + 1);
 
-    uint64_t q_cur = 0, q_end = 0;
-    bool q_done = false;
+    const uint32_t seg0 = blockIdx.x;
+    const uint32_t n_seg = gridDim.x;
+    const uint32_t row_lo = a.seg_row[seg0];
+    const uint32_t row_hi = row_lo + a.win_rows < a.n_rows ? row_lo + a.win_rows : a.n_rows;
 
-    bool active = false;
-    uint64_t cell = 0, node = 0, pend = 0;
-    uint32_t k = 0, kcap = 0;
-    uint64_t tally_base = 0;  // (row * kdim) * T + type_file
-    double s = 0.0, mean = 0.0, sd = 0.0;
-    uint32_t n_div = 0, n_rec = 0, n_drop = 0, n_rej = 0;
+    for (uint32_t i = tid; i < T; i += bdim) {
+        t_cum[i] = a.type_cum[i];
+        t_mean[i] = a.type_mean[i];
+        t_sd[i] = a.type_sd[i];
+        t_file[i] = a.type_file[i];
+    }
+    // window rows [row_lo, row_hi): their first cells and generation caps
+    for (uint32_t i = tid; i <= row_hi - row_lo; i += bdim)
+        w_start[i] = a.row_start[row_lo + i];
+    for (uint32_t i = tid; i < row_hi - row_lo; i += bdim)
+        w_info[i] = a.row_info[row_lo + i];
+    for (uint32_t i = tid; i <= win_len; i += bdim)
+        win[i] = 0u;  // win[win_len] absorbs the adds of lanes outside the window
+    __syncthreads();
+
+    // wavefront-uniform queue state (scalar registers)
+    uint64_t c_cur = 0, c_end = 0;
+    uint32_t row_cur = 0;
+    uint32_t victim = 0;      // 0: own segment; else offset of the next segments to scan
+    uint32_t roots_done = 0;
+    uint32_t rb_head = 0, rb_count = 0;
+
+    Lane L;
+    L.cell = L.node = L.pend = L.tb_glob = 0;
+    L.s = L.mean = L.sd = 0.0;
+    L.k = L.kcap = 0;
+    L.tb_lds = kNoWindow;
+    L.active = false;
+    Counts n = {0, 0, 0, 0, 0, 0, 0};
+    const uint32_t idle_limit = a.idle_limit;
 
     for (;;) {
-        const uint64_t idle = __ballot(!active);
-        if (idle != 0 && !q_done) {
-            if (q_cur >= q_end) {
-                unsigned long long base = 0;
-                if (__lane_id() == 0)
-                    base = atomicAdd(&a.counters[kPoolNext], (unsigned long long)chunk);
-                base = __shfl(base, 0);
-                if (base >= pool_n) {
-                    q_done = true;
-                } else {
-                    q_cur = base;
-                    q_end = base + chunk < pool_n ? base + chunk : pool_n;
+        // ======== refill: initial cells, buffered roots, stealing ========
+        uint64_t idle = __ballot(!L.active);
+        uint32_t n_idle = uni((uint32_t)__popcll(idle));
+        while (!roots_done && rb_count < n_idle) {
+            // ---- claim a range of initial cells if the current one is used up ----
+            while (c_cur >= c_end && !roots_done) {
+                // own segment while it lasts; then the segment with the most
+                // cells left among the next 64 (one load per lane)
+                uint32_t sg = seg0;
+                if (victim) {
+                    const uint32_t cand = (seg0 + victim + lane) % n_seg;
+                    const uint64_t len = a.seg_begin[cand + 1] - a.seg_begin[cand];
+                    const uint64_t used = __atomic_load_n(&a.seg_next[cand], __ATOMIC_RELAXED);
+                    uint64_t left = used < len && victim + lane < n_seg ? len - used : 0;
+                    uint32_t best = lane;
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) {
+                        const uint64_t ol = __shfl_xor(left, off);
+                        const uint32_t ob = __shfl_xor(best, off);
+                        if (ol > left || (ol == left && ob < best)) {
+                            left = ol;
+                            best = ob;
+                        }
+                    }
+                    if (uni64(left) == 0) {
+                        victim = uni(victim + kWave);
+                        if (victim >= n_seg)
+                            roots_done = 1;
+                        continue;
+                    }
+                    sg = uni((seg0 + victim + best) % n_seg);
+                }
+                const uint64_t sb = a.seg_begin[sg];
+                const uint64_t len = a.seg_begin[sg + 1] - sb;
+                const uint32_t want = victim ? a.steal_claim : a.claim;
+                unsigned long long base = ~0ull;
+                if (lane == 0)
+                    base = atomicAdd(&a.seg_next[sg], (unsigned long long)want);
+                base = uni64(__shfl(base, 0));
+                if (base < len) {
+                    c_cur = uni64(sb + base);
+                    c_end = uni64(sb + (base + want < len ? base + want : len));
+                    if (row_hi > row_lo && c_cur >= w_start[0] && c_cur < w_start[row_hi - row_lo])
+                        row_cur = uni(row_lo + find_row_wave(w_start, 0, row_hi - row_lo, c_cur, lane));
+                    else
+                        row_cur = uni(find_row_wave(a.row_start, 0, a.n_rows, c_cur, lane));
+                } else if (!victim) {
+                    victim = 1;
                 }
             }
-            if (!q_done) {
-                const uint64_t avail = q_end - q_cur;
-                const uint32_t rank = lane_rank(idle);
-                if (!active && rank < avail) {
-                    const PoolEntry e = a.pool[q_cur + rank];
-                    const uint32_t row = e.row_type & 0xFFFFFFu;
-                    const uint32_t jt = e.row_type >> 24;
-                    cell = e.cell;
-                    s = e.s0;
-                    node = 1;
-                    k = 0;
-                    pend = 0;
-                    kcap = a.row_info[row] & 0xFFu;
-                    mean = a.type_mean[jt];
-                    sd = a.type_sd[jt];
-                    tally_base = (uint64_t)row * a.kdim * T + a.type_file[jt];
-                    active = true;
+            if (c_cur >= c_end)
+                break;
+            // ---- root step: one initial cell per lane (cell.cu:25-143) ----
+            // branch-free apart from rare paths behind wavefront-wide checks
+            const uint32_t n_valid = uni(c_end - c_cur < kWave ? (uint32_t)(c_end - c_cur) : kWave);
+            const uint64_t c = c_cur + lane;
+            const bool valid = lane < n_valid;
+            const bool in_win = row_cur >= row_lo && row_cur + 1 < row_hi;
+            const uint64_t next_start = row_cur + 1 >= a.n_rows ? ~0ull
+                                        : (in_win ? w_start[row_cur + 1 - row_lo] : a.row_start[row_cur + 1]);
+            uint32_t row = row_cur;
+            if (uni64(next_start) <= c_cur + n_valid - 1) {
+                // rare: the 64 cells span several rows
+                if (valid) {
+                    const uint32_t hi = row_cur + kWave + 1 < a.n_rows ? row_cur + kWave + 1 : a.n_rows;
+                    if (in_win && hi <= row_hi)
+                        row = row_lo + find_row(w_start, row_cur - row_lo, hi - row_lo, c);
+                    else
+                        row = find_row(a.row_start, row_cur, hi, c);
                 }
-                const uint64_t taken = (uint64_t)__popcll(idle);
-                q_cur += taken < avail ? taken : avail;
             }
-        }
-        if (__ballot(active) == 0)
-            break;
-        if (!active)
-            continue;
-
-        // one division: both daughters' timers from one block
-        ++n_div;
-        double tl, tr;
-        {
-            const Words w = dev::draw_block(a.seed, cell, node, 0);
+            const bool row_in_win = row >= row_lo && row < row_hi;
+            uint32_t info = w_info[row_in_win ? row - row_lo : 0u];
+            if (__builtin_expect(__ballot(valid && !row_in_win) != 0, 0)) {
+                if (valid && !row_in_win)
+                    info = a.row_info[row];
+            }
+            const uint32_t kc = info & 0xFFu;
+            const bool ge_phi = (info >> 8) & 1u;
+            const Words w = dev::draw_block(a.seed, c, 0, 0);
+            const double ut = dev::uniform_co(w.w0, w.w1);
+            const double U = dev::uniform_oc(w.w2, w.w3);
+            // first type with ut < cumulative proportion (cell.cu:90-101)
+            uint32_t jt = 0;
+            bool found = false;
+            for (uint32_t i = 0; i + 1 < T; ++i) {
+                const bool hit = !found && ut < t_cum[i];
+                jt = hit ? i : jt;
+                found = found || hit;
+            }
+            jt = found ? jt : T - 1;
+            const double m = t_mean[jt];
+            const double sdv = t_sd[jt];
+            const bool quiescent = m < 0.0;
+            // first timer: stream of node 1 from attempt 1 (cell.cu:106-122)
+            const Words w1 = dev::draw_block(a.seed, c, 1, 1);
             double z0, z1;
-            dev::normal_pair(w, z0, z1);
-            tl = fma(z0, sd, mean);
-            tr = fma(z1, sd, mean);
-        }
-        if (!(tl > 0.0)) {
-            ++n_rej;
-            tl = truncated_timer(a.seed, cell, 2 * node, mean, sd, n_rej);
-        }
-        if (!(tr > 0.0)) {
-            ++n_rej;
-            tr = truncated_timer(a.seed, cell, 2 * node + 1, mean, sd, n_rej);
-        }
-        const double sl = s + tl;
-        const double sr = s + tr;
-        const uint32_t k1 = k + 1;
-        const bool rec_l = sl > a.t_max;
-        const bool rec_r = sr > a.t_max;
-        const bool div_l = !rec_l && k1 < kcap;
-        const bool div_r = !rec_r && k1 < kcap;
-        const uint32_t nrec = (uint32_t)rec_l + (uint32_t)rec_r;
-        n_drop += (uint32_t)(!rec_l && !div_l) + (uint32_t)(!rec_r && !div_r);
-        if (nrec) {
-            n_rec += nrec;
-            atomicAdd(&a.tally[tally_base + (uint64_t)k1 * T], (unsigned long long)nrec);
-        }
-        if (div_l) {
-            if (div_r) {
-                stack[k1 * bdim + tid] = sr;
-                pend |= 1ull << k1;
+            dev::normal_pair(w1, z0, z1);
+            const double c0 = fma(z0, sdv, m);
+            const double c1 = fma(z1, sdv, m);
+            const bool live = valid && !quiescent;
+            n.rej += live ? (uint32_t)!(c0 > 0.0) + (uint32_t)(!(c0 > 0.0) && !(c1 > 0.0)) : 0u;
+            double tau0 = c0 > 0.0 ? c0 : c1;
+            if (__builtin_expect(__ballot(live && !(tau0 > 0.0)) != 0, 0)) {
+                if (live && !(tau0 > 0.0)) {
+                    // rare: keep drawing from attempt 2
+                    tau0 = m;
+                    for (uint32_t att = 2; att < 65536u; ++att) {
+                        const Words wr = dev::draw_block(a.seed, c, 1, att);
+                        double y0, y1;
+                        dev::normal_pair(wr, y0, y1);
+                        const double d0 = fma(y0, sdv, m);
+                        if (d0 > 0.0) { tau0 = d0; break; }
+                        ++n.rej;
+                        const double d1 = fma(y1, sdv, m);
+                        if (d1 > 0.0) { tau0 = d1; break; }
+                        ++n.rej;
+                    }
+                }
             }
-            node = 2 * node;
-            k = k1;
-            s = sl;
-        } else if (div_r) {
-            node = 2 * node + 1;
-            k = k1;
-            s = sr;
-        } else if (pend) {
-            const uint32_t lvl = 63u - (uint32_t)__builtin_clzll(pend);
-            pend &= ~(1ull << lvl);
-            s = stack[lvl * bdim + tid];
-            node = (node >> (k - lvl)) | 1ull;
-            k = lvl;
-        } else {
-            active = false;
+            const double t0 = tau0 * U;
+            const double s0 = t0 + tau0;
+            // quiescent or out of time: tallied if its value is in the output
+            // bin set; in time with f/2 <= phi: dropped; else it divides
+            const bool record = valid && (quiescent || s0 > a.t_max);
+            const bool rec_ok = record && ge_phi;
+            const bool divides = valid && !record && kc != 0;
+            n.cells += valid ? 1u : 0u;
+            n.rec += rec_ok ? 1u : 0u;
+            n.lost += record && !ge_phi ? 1u : 0u;
+            n.drop += valid && !record && kc == 0 ? 1u : 0u;
+            n.pool += divides ? 1u : 0u;
+            const uint32_t tf = t_file[jt];
+            const uint32_t woff = row - row_lo;
+            const bool rec_win = rec_ok && woff < a.win_rows;
+            atomicAdd(&win[rec_win ? woff * kdim * T + tf : win_len], rec_win ? 1u : 0u);
+            if (__builtin_expect(__ballot(rec_ok && !rec_win) != 0, 0)) {
+                if (rec_ok && !rec_win)
+                    atomicAdd(&a.tally[(uint64_t)row * kdim * T + tf], 1ull);
+            }
+            // dividing roots go to the ring; the others write the spare slot
+            const uint64_t dmask = __ballot(divides);
+            RootSlot &slot = ring[divides ? (rb_head + rb_count + lane_rank(dmask)) & (kRing - 1) : kRing];
+            slot.s0 = s0;
+            slot.cell = (uint32_t)c;
+            slot.row_type = row | (jt << 24);
+            slot.kcap = kc;
+            rb_count = uni(rb_count + (uint32_t)__popcll(dmask));
+            row_cur = uni(__shfl(row, (int)(n_valid - 1)));
+            c_cur = uni64(c_cur + n_valid);
+            if (c_cur >= c_end && victim == 0 && c_end == a.seg_begin[seg0 + 1])
+                victim = 1;  // own segment exhausted: next claims steal
         }
+        if (rb_count && n_idle) {
+            // ---- hand buffered roots to idle lanes (branch-free) ----
+            const uint32_t rank = lane_rank(idle);
+            const bool take = !L.active && rank < rb_count;
+            const RootSlot e = ring[(rb_head + (take ? rank : 0u)) & (kRing - 1)];
+            const uint32_t row = e.row_type & 0xFFFFFFu;
+            const uint32_t jt = min(e.row_type >> 24, T - 1);
+            const uint32_t tf = t_file[jt];
+            const uint32_t off = row - row_lo;
+            L.cell = take ? e.cell : L.cell;
+            L.s = take ? e.s0 : L.s;
+            L.node = take ? 1ull : L.node;
+            L.k = take ? 0u : L.k;
+            L.pend = take ? 0ull : L.pend;
+            L.kcap = take ? e.kcap : L.kcap;
+            L.mean = take ? t_mean[jt] : L.mean;
+            L.sd = take ? t_sd[jt] : L.sd;
+            L.tb_lds = take ? (off < a.win_rows ? off * kdim * T + tf : kNoWindow) : L.tb_lds;
+            L.tb_glob = take ? (uint64_t)row * kdim * T + tf : L.tb_glob;
+            L.active = L.active || take;
+            const uint32_t taken = n_idle < rb_count ? n_idle : rb_count;
+            rb_head = uni((rb_head + taken) & (kRing - 1));
+            rb_count = uni(rb_count - taken);
+        } else if (roots_done && n_idle) {
+            // ---- steal pending daughters inside the wavefront ----
+            const uint64_t donors = __ballot(L.active && L.pend != 0);
+            const uint32_t nd = uni(min((uint32_t)__popcll(donors), n_idle));
+            if (nd) {
+                const uint32_t my_lvl = L.pend ? (uint32_t)__builtin_ctzll(L.pend) : 0u;
+                const uint32_t r = lane_rank(idle);
+                const bool thief = !L.active && r < nd;
+                const uint32_t src = thief ? nth_set_bit(donors, r) : lane;
+                const uint32_t lvl = __shfl(my_lvl, (int)src);
+                const uint64_t d_node = __shfl(L.node, (int)src);
+                const uint32_t d_k = __shfl(L.k, (int)src);
+                const uint64_t d_cell = __shfl(L.cell, (int)src);
+                const uint32_t d_kcap = __shfl(L.kcap, (int)src);
+                const uint32_t d_tbl = __shfl(L.tb_lds, (int)src);
+                const uint64_t d_tbg = __shfl(L.tb_glob, (int)src);
+                const double d_mean = __shfl(L.mean, (int)src);
+                const double d_sd = __shfl(L.sd, (int)src);
+                if (L.active && L.pend && lane_rank(donors) < nd)
+                    L.pend &= ~(1ull << my_lvl);
+                if (thief) {
+                    L.s = stack[(size_t)(lvl - 1) * bdim + wave * kWave + src];
+                    L.node = (d_node >> (d_k - lvl)) | 1ull;
+                    L.k = lvl;
+                    L.cell = d_cell;
+                    L.kcap = d_kcap;
+                    L.tb_lds = d_tbl;
+                    L.tb_glob = d_tbg;
+                    L.mean = d_mean;
+                    L.sd = d_sd;
+                    L.pend = 0;
+                    L.active = true;
+                }
+            }
+        }
+        if (__ballot(L.active) == 0) {
+            if (roots_done && rb_count == 0)
+                break;
+            continue;
+        }
+
+        // ======== division steps until enough lanes are idle ========
+        // Straight-line code: every outcome is a select, so the step costs a
+        // handful of scalar instructions (one scalar unit serves the whole
+        // CU).  Idle lanes run it too with their results discarded.
+        do {
+            const bool act = L.active;
+            double tl, tr;
+            {
+                const Words w = dev::draw_block(a.seed, L.cell, L.node, 0);
+                double z0, z1;
+                dev::normal_pair(w, z0, z1);
+                tl = fma(z0, L.sd, L.mean);
+                tr = fma(z1, L.sd, L.mean);
+            }
+            const bool need_l = act && !(tl > 0.0);
+            const bool need_r = act && !(tr > 0.0);
+            if (__builtin_expect(__ballot(need_l || need_r) != 0, 0)) {
+                // rare: truncated-normal redraws (cell.cu:106-122)
+                if (need_l) {
+                    ++n.rej;
+                    tl = truncated_timer(a.seed, L.cell, 2 * L.node, L.mean, L.sd, n.rej);
+                }
+                if (need_r) {
+                    ++n.rej;
+                    tr = truncated_timer(a.seed, L.cell, 2 * L.node + 1, L.mean, L.sd, n.rej);
+                }
+            }
+            // daughters (proliferation.cu:317-381): out of time -> tallied,
+            // f/2 <= phi -> dropped, else divides later
+            const double sl = L.s + tl;
+            const double sr = L.s + tr;
+            const uint32_t k1 = L.k + 1;
+            const bool rec_l = sl > a.t_max;
+            const bool rec_r = sr > a.t_max;
+            const bool div_l = !rec_l && k1 < L.kcap;
+            const bool div_r = !rec_r && k1 < L.kcap;
+            const uint32_t nrec = act ? (uint32_t)rec_l + (uint32_t)rec_r : 0u;
+            n.div += act ? 1u : 0u;
+            n.rec += nrec;
+            n.drop += act ? (uint32_t)(!rec_l && !div_l) + (uint32_t)(!rec_r && !div_r) : 0u;
+            const bool in_win = L.tb_lds != kNoWindow;
+            // idle lanes add 0 to the spare slot win[win_len]
+            atomicAdd(&win[act && in_win ? L.tb_lds + k1 * T : win_len], nrec);
+            if (__builtin_expect(__ballot(!in_win && nrec != 0) != 0, 0)) {
+                if (!in_win && nrec != 0)
+                    atomicAdd(&a.tally[L.tb_glob + (uint64_t)k1 * T], (unsigned long long)nrec);
+            }
+            // right daughter's completion time; level k1 holds no pending
+            // entry, so the store is harmless when it is not pushed
+            stack[(size_t)(act ? k1 - 1 : 0u) * bdim + tid] = sr;
+            const bool go_l = act && div_l;
+            const bool go_r = act && !div_l && div_r;
+            const bool pop = act && !div_l && !div_r;
+            const uint64_t pend2 = L.pend | (go_l && div_r ? 1ull << k1 : 0ull);
+            const bool has = pend2 != 0;
+            const uint32_t lvl = has ? 63u - (uint32_t)__builtin_clzll(pend2) : 1u;
+            const double sp = stack[(size_t)(lvl - 1) * bdim + tid];
+            const uint64_t up = (L.node >> ((L.k - lvl) & 63u)) | 1ull;  // pending right daughter
+            L.node = go_l ? 2 * L.node : (go_r ? 2 * L.node + 1 : (pop ? up : L.node));
+            L.s = go_l ? sl : (go_r ? sr : (pop ? sp : L.s));
+            L.pend = pop ? pend2 & ~(1ull << lvl) : pend2;
+            L.k = (go_l || go_r) ? k1 : (pop ? lvl : L.k);
+            L.active = act && (!pop || has);
+        } while (uni((uint32_t)__popcll(__ballot(!L.active))) <= idle_limit);
     }
 
-    n_div = wave_sum(n_div);
-    n_rec = wave_sum(n_rec);
-    n_drop = wave_sum(n_drop);
-    n_rej = wave_sum(n_rej);
-    if (__lane_id() == 0) {
+    const uint32_t n_div = wave_sum(n.div);
+    const uint32_t n_rec = wave_sum(n.rec);
+    const uint32_t n_drop = wave_sum(n.drop);
+    const uint32_t n_rej = wave_sum(n.rej);
+    const uint32_t n_lost = wave_sum(n.lost);
+    const uint32_t n_cells = wave_sum(n.cells);
+    const uint32_t n_pool = wave_sum(n.pool);
+    if (lane == 0) {
         if (n_div) atomicAdd(&a.counters[kDivisions], (unsigned long long)n_div);
         if (n_rec) atomicAdd(&a.counters[kRecorded], (unsigned long long)n_rec);
         if (n_drop) atomicAdd(&a.counters[kDropped], (unsigned long long)n_drop);
         if (n_rej) atomicAdd(&a.counters[kRejected], (unsigned long long)n_rej);
+        if (n_lost) atomicAdd(&a.counters[kLost], (unsigned long long)n_lost);
+        if (n_cells) atomicAdd(&a.counters[kCells], (unsigned long long)n_cells);
+        if (n_pool) atomicAdd(&a.counters[kPoolCount], (unsigned long long)n_pool);
+    }
+    __syncthreads();
+    const uint64_t gbase = (uint64_t)row_lo * kdim * T;
+    for (uint32_t i = tid; i < win_len; i += bdim) {
+        const uint32_t c = win[i];
+        if (c)
+            atomicAdd(&a.tally[gbase + i], (unsigned long long)c);
     }
 }
 
@@ -344,6 +551,16 @@ __global__ void debug_draws_kernel(uint64_t seed, const uint64_t *cell, const ui
 
 }  // namespace
 
+size_t tree_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32_t stack_levels,
+                      uint32_t block_size)
+{
+    const size_t tables = ((size_t)n_types * 28 + 15) / 16 * 16 + ((size_t)win_rows * 12 + 8 + 15) / 16 * 16;
+    const size_t win = ((size_t)win_rows * kdim * n_types * 4 + 4 + 15) / 16 * 16;
+    const size_t stk = (size_t)stack_levels * block_size * sizeof(double);
+    const size_t ring = (size_t)(block_size / kWave) * (kRing + 1) * sizeof(RootSlot);
+    return tables + win + stk + ring;
+}
+
 hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t st)
 {
     if (n == 0)
@@ -355,22 +572,11 @@ hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_roots(const RunArgs &a, const LaunchShape &ls, hipStream_t st)
-{
-    if (a.n_cells == 0)
-        return hipSuccess;
-    const size_t lds = (size_t)ls.roots_window_rows * a.n_types * sizeof(unsigned int);
-    hipLaunchKernelGGL(roots_kernel, dim3(ls.roots_blocks), dim3(ls.roots_block_size), lds, st, a,
-                       ls.roots_window_rows);
-    return hipGetLastError();
-}
-
 hipError_t launch_tree(const RunArgs &a, const LaunchShape &ls, hipStream_t st)
 {
     if (a.n_cells == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(tree_kernel, dim3(ls.tree_blocks), dim3(ls.tree_block_size), ls.tree_lds_bytes,
-                       st, a, ls.chunk);
+    hipLaunchKernelGGL(tree_kernel, dim3(ls.tree_blocks), dim3(ls.tree_block_size), ls.tree_lds_bytes, st, a);
     return hipGetLastError();
 }
 
@@ -388,11 +594,6 @@ hipError_t launch_debug_draws(uint64_t seed, const uint64_t *cell, const uint64_
 hipError_t tree_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu)
 {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tree_kernel, block_size, lds_bytes);
-}
-
-hipError_t roots_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu)
-{
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, roots_kernel, block_size, lds_bytes);
 }
 
 }  // namespace procell

@@ -51,7 +51,7 @@ class CellType(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
-                ("chunk", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+                ("chunk", ctypes.c_uint32), ("idle_lanes", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
 class Params(ctypes.Structure):
@@ -66,8 +66,9 @@ class Stats(ctypes.Structure):
     _fields_ = [("divisions", ctypes.c_uint64), ("recorded", ctypes.c_uint64),
                 ("dropped", ctypes.c_uint64), ("lost", ctypes.c_uint64),
                 ("rejected", ctypes.c_uint64), ("cells", ctypes.c_uint64),
-                ("pool", ctypes.c_uint64), ("ms_roots", ctypes.c_double),
-                ("ms_tree", ctypes.c_double), ("ms_run", ctypes.c_double)]
+                ("pool", ctypes.c_uint64), ("ms_tree", ctypes.c_double), ("ms_run", ctypes.c_double),
+                ("workgroups", ctypes.c_uint32), ("block_size", ctypes.c_uint32),
+                ("lds_bytes", ctypes.c_uint32), ("window_rows", ctypes.c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -141,7 +142,7 @@ class Problem:
     """Inputs of one simulation, kept alive for the C structs that point at them."""
 
     def __init__(self, values, counts, proportion, mean, sd, t_max, phi=0.0, seed=0,
-                 track_ratio=True, device=0, block_size=0, blocks_per_cu=0, chunk=0, flags=0):
+                 track_ratio=True, device=0, block_size=0, blocks_per_cu=0, chunk=0, idle_lanes=0, flags=0):
         values = np.ascontiguousarray(values, dtype=np.float64)
         counts = np.ascontiguousarray(counts, dtype=np.uint64)
         self.rows = (Row * max(len(values), 1))()
@@ -165,7 +166,7 @@ class Problem:
         self.params.seed = int(seed) & (2**64 - 1)
         self.params.track_ratio = int(bool(track_ratio))
         self.params.device = int(device)
-        self.params.tuning = Tuning(block_size, blocks_per_cu, chunk, flags)
+        self.params.tuning = Tuning(block_size, blocks_per_cu, chunk, idle_lanes, flags)
         self.params.stream = None
 
     @classmethod

@@ -68,6 +68,7 @@ typedef struct {
     uint32_t block_size;    /* threads per workgroup of the tree kernel: 64..256, multiple of 64 */
     uint32_t blocks_per_cu; /* resident workgroups per CU of the persistent tree kernel */
     uint32_t chunk;         /* initial cells a wavefront claims from the work queue at once */
+    uint32_t idle_lanes;    /* idle lanes a wavefront tolerates before refilling them (0..63) */
     uint32_t flags;         /* PROCELL_FLAG_* */
 } procell_tuning;
 
@@ -95,9 +96,12 @@ typedef struct {
     uint64_t rejected;    /* normal draws rejected because the timer was <= 0 */
     uint64_t cells;       /* initial cells */
     uint64_t pool;        /* initial cells that divided at least once */
-    double ms_roots;      /* device time of the initial-cell kernel (PROCELL_FLAG_TIMING) */
-    double ms_tree;       /* device time of the division-tree kernel (PROCELL_FLAG_TIMING) */
-    double ms_run;        /* device time of the whole run (PROCELL_FLAG_TIMING) */
+    double ms_tree;       /* device time of the simulation kernel (PROCELL_FLAG_TIMING) */
+    double ms_run;        /* device time of the whole run incl. clearing (PROCELL_FLAG_TIMING) */
+    uint32_t workgroups;  /* launch shape chosen for the run */
+    uint32_t block_size;
+    uint32_t lds_bytes;
+    uint32_t window_rows; /* rows of the per-workgroup LDS tally window */
 } procell_stats;
 
 typedef struct {
