@@ -1,0 +1,201 @@
+"""Benchmark: cell divisions per second of the plain proliferation method on
+one MI355X.
+
+A step is one complete simulation (procell_plan_run: clear the tallies, run
+the persistent division-tree kernel) of the 1024-bin, ~1e7-cell histogram
+with five cell types (four proliferating, one quiescent), t_max 240, phi 50
+and per-type tracking.  Inputs are uploaded once before timing (resident in
+HBM); every step uses a different seed.  value = divisions summed over the
+timed steps / wall time between the two synchronisations.
+
+Prints one JSON line on stdout; everything else goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "cuda-pro-cell_amd")
+for _p in (REPO, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "cell divisions/s on 1 GPU (plain) + calib grad-steps/s batched vs plain"
+# Nominal VALU work per division (lane operations): Philox4x32-10 block,
+# two 53-bit uniforms, FP64 Box-Muller pair, timer update and compares.
+LANE_OPS_PER_DIVISION = 180
+# MI355X VALU issue peak: 256 CUs x 64 lanes/clk x 2.4 GHz (lane-ops/s).
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "tree_kernel_hbm_traffic.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def cpu_baseline(workload, budget_s: float):
+    """The CPU oracle (OpenMP, all host cores of this box) on the same
+    workload, seeds 0, 1, ... until about budget_s of wall time."""
+    from oracle import binding as ob
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    divs, seeds, t0 = 0, 0, time.perf_counter()
+    while True:
+        _, _, st = ob.simulate(ob.Problem.from_workload(workload, seeds), threads)
+        divs += st["divisions"]
+        seeds += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    # one core on a 1/20 sample of the cells
+    from procell import workloads
+
+    small = workloads.scaled(workload, 0.05)
+    t1 = time.perf_counter()
+    _, _, st1 = ob.simulate(ob.Problem.from_workload(small, 0), 1)
+    dt1 = time.perf_counter() - t1
+    return {
+        "value": divs / dt,
+        "unit": "divisions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"full workload, seeds 0..{seeds - 1} ({divs} divisions, {dt:.2f} s wall)",
+        "one_core_value": st1["divisions"] / dt1,
+        "one_core_sample": f"5% of the cells, seed 0 ({st1['divisions']} divisions, {dt1:.2f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=1.5, help="seconds of CPU-baseline wall time")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device once (reference kernel for HBM counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not supported",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import procell
+    from procell import workloads
+
+    if not torch.cuda.is_available():
+        raise SystemExit("no GPU visible")
+    if args.copy_calib:
+        n = args.copy_calib << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {n} bytes read, {n} bytes written")
+        del src, dst
+    w = workloads.large_histogram(0)
+    log(f"workload {w.name}: {w.n_cells} cells, {len(w.values)} bins, {len(w.proportion)} types")
+    stream = torch.cuda.Stream()
+    prob = procell.Problem.from_workload(w, 0, flags=procell.FLAG_TIMING)
+    plan = procell.Plan(prob, stream=stream.cuda_stream)
+    K, W = args.steps, args.warmup
+    seeds = [1000 + i for i in range(K)]
+    for i in range(W):
+        plan.run(i)
+    torch.cuda.synchronize()
+    plan.kernel_times()  # drop warm-up timings
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for s in seeds:
+        plan.run(s)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = start.elapsed_time(stop)
+    kt = plan.kernel_times()
+    # divisions of each timed seed (re-run untimed; results are a pure
+    # function of the seed)
+    total_div = 0
+    stats = None
+    for s in seeds:
+        plan.run(s)
+        stats = plan.result(want_raw=False).stats
+        total_div += stats["divisions"]
+    plan.close()
+    elapsed = ev_ms * 1e-3
+    value = total_div / elapsed
+    kernel_ms = float(np.mean(kt)) if len(kt) else float("nan")
+    div_per_run = total_div / K
+    achieved = div_per_run * LANE_OPS_PER_DIVISION / (kernel_ms * 1e-3)
+    log(f"timed {K} steps: {ev_ms:.2f} ms events, {wall * 1e3:.2f} ms wall, kernel {kernel_ms:.4f} ms avg, "
+        f"{total_div} divisions, {value:.4g} div/s")
+    traffic = traffic_from_profile()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "divisions/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ev_ms / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "hist1024_1e7_types5: 1024 log-spaced bins 1..1e5, lognormal Poisson counts "
+                        "(~1e7 cells), 4 proliferating types + 1 quiescent, Dirichlet ratios, t_max 240, "
+                        "phi 50, per-type tracking",
+            "cells": w.n_cells,
+            "bins": int(len(w.values)),
+            "types": int(len(w.proportion)),
+            "divisions_per_step": div_per_run,
+            "seeds": f"{seeds[0]}..{seeds[-1]}",
+            "workgroups": stats["workgroups"],
+            "block_size": stats["block_size"],
+            "lds_bytes": stats["lds_bytes"],
+            "calibration": "not built (grad-steps/s not measured)",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved / 1e12,
+            "peak": VALU_PEAK_LANE_OPS / 1e12,
+            "unit": "Tlane-op/s",
+            "frac": achieved / VALU_PEAK_LANE_OPS,
+            "traffic": traffic,
+            "kernel": "tree_kernel",
+            "kernel_ms": kernel_ms,
+            "work_per_unit": f"{LANE_OPS_PER_DIVISION} lane-ops per division",
+        },
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(w, args.cpu_budget)
+        out["cpu_baseline"] = cb
+        log(f"cpu baseline: {cb['value']:.4g} div/s on {cb['cores']} cores; 1 core {cb['one_core_value']:.4g}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

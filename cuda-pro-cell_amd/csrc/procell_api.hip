@@ -163,8 +163,12 @@ struct procell_plan {
     size_t tally_len = 0;
     size_t zero_words = 0;    // counters, tally and segment cursors, cleared per run
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> kev;   // PROCELL_FLAG_TIMING: start/stop of the tree kernel per run (ring)
+    uint64_t runs = 0;             // runs since the last kernel-time read
     bool ran = false;
 };
+
+constexpr uint32_t kTimedRuns = 256;
 
 namespace {
 
@@ -205,6 +209,9 @@ void destroy_plan(procell_plan *pl)
         return;
     (void)hipSetDevice(pl->device);
     for (hipEvent_t &e : pl->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    for (hipEvent_t &e : pl->kev)
         if (e)
             (void)hipEventDestroy(e);
     if (pl->d_block)
@@ -321,10 +328,13 @@ int create_plan(const procell_params *p, procell_plan **out)
     LaunchShape &ls = pl->shape;
     RunArgs &a = pl->args;
     a.stack_levels = kdim > 1 ? kdim - 1 : 1;  // levels 1 .. kmax
+    // row tables (8 bytes per row) go to LDS when they are small
+    const uint32_t lds_rows = (size_t)(2 * n_rows + 1) * 4 <= 16384 ? n_rows : 0u;
+    a.rows_in_lds = lds_rows ? 1u : 0u;
     int bs = tu.block_size ? (int)tu.block_size : 256;
     int occ = 0;
     for (;;) {
-        const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs);
+        const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
         int occ_v = 0;
         if (tree_occupancy(bs, base, &occ_v) != hipSuccess)
             return fail(PROCELL_E_HIP);
@@ -339,7 +349,7 @@ int create_plan(const procell_params *p, procell_plan **out)
     if (tu.blocks_per_cu)
         occ = std::min<int>(occ, (int)tu.blocks_per_cu);
     ls.tree_block_size = bs;
-    const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs);
+    const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
     const size_t budget = std::min(lds_per_cu / (size_t)occ, lds_per_block);
     const size_t row_bytes = (size_t)kdim * T * 4;
     uint32_t win_rows = budget > base + 16 ? (uint32_t)((budget - base - 16) / row_bytes) : 0;
@@ -366,7 +376,7 @@ int create_plan(const procell_params *p, procell_plan **out)
     }
     // the window must not cost a resident workgroup (LDS allocation granularity)
     for (;;) {
-        ls.tree_lds_bytes = tree_lds_bytes(win_rows, kdim, T, a.stack_levels, (uint32_t)bs);
+        ls.tree_lds_bytes = tree_lds_bytes(win_rows, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
         int occ_w = 0;
         if (tree_occupancy(bs, ls.tree_lds_bytes, &occ_w) != hipSuccess)
             return fail(PROCELL_E_HIP);
@@ -378,9 +388,9 @@ int create_plan(const procell_params *p, procell_plan **out)
     }
     ls.tree_blocks = (int)g;
     a.win_rows = win_rows;
-    a.claim = tu.chunk ? tu.chunk : 1024;
-    a.steal_claim = std::max<uint32_t>(64, a.claim / 4);
-    a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 16;
+    a.claim = tu.chunk ? tu.chunk : 1024;  // cells per claim from the own segment
+    a.steal_claim = std::max<uint32_t>(1, std::min<uint32_t>(a.claim, 64));
+    a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 8;
 
     // ---- one device allocation: counters | tally | seg_next | tables ----
     size_t off = 0;
@@ -389,7 +399,7 @@ int create_plan(const procell_params *p, procell_plan **out)
     const size_t o_next = off; off += (size_t)g * 8;
     pl->zero_words = off / 8;
     off = align_up(off, 256);
-    const size_t o_rs = off; off = align_up(off + (size_t)(n_rows + 1) * 8, 256);
+    const size_t o_rs = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
     const size_t o_ri = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
     const size_t o_cum = off; off = align_up(off + (size_t)T * 8, 256);
     const size_t o_mean = off; off = align_up(off + (size_t)T * 8, 256);
@@ -415,7 +425,7 @@ int create_plan(const procell_params *p, procell_plan **out)
     a.counters = (unsigned long long *)(dbase + o_cnt);
     a.tally = (unsigned long long *)(dbase + o_tally);
     a.seg_next = (unsigned long long *)(dbase + o_next);
-    a.row_start = (const uint64_t *)(dbase + o_rs);
+    a.row_start = (const uint32_t *)(dbase + o_rs);
     a.row_info = (const uint32_t *)(dbase + o_ri);
     a.type_cum = (const double *)(dbase + o_cum);
     a.type_mean = (const double *)(dbase + o_mean);
@@ -426,7 +436,8 @@ int create_plan(const procell_params *p, procell_plan **out)
     auto up = [&](size_t o, const void *src, size_t bytes) {
         return bytes == 0 || hipMemcpyAsync(dbase + o, src, bytes, hipMemcpyHostToDevice, pl->stream) == hipSuccess;
     };
-    if (!up(o_rs, s.start.data(), s.start.size() * 8) || !up(o_ri, s.row_info.data(), n_rows * 4) ||
+    std::vector<uint32_t> start32(s.start.begin(), s.start.end());  // n_cells < 2^32
+    if (!up(o_rs, start32.data(), start32.size() * 4) || !up(o_ri, s.row_info.data(), n_rows * 4) ||
         !up(o_cum, s.type_cum.data(), s.type_cum.size() * 8) || !up(o_mean, s.type_mean.data(), s.type_mean.size() * 8) ||
         !up(o_sd, s.type_sd.data(), s.type_sd.size() * 8) || !up(o_tf, s.type_file.data(), s.type_file.size() * 4) ||
         !up(o_sb, seg_begin.data(), seg_begin.size() * 8) || !up(o_sr, seg_row.data(), seg_row.size() * 4))
@@ -435,6 +446,10 @@ int create_plan(const procell_params *p, procell_plan **out)
         return fail(PROCELL_E_HIP);
     if (pl->timing) {
         for (hipEvent_t &e : pl->ev)
+            if (hipEventCreate(&e) != hipSuccess)
+                return fail(PROCELL_E_HIP);
+        pl->kev.assign(2 * kTimedRuns, nullptr);
+        for (hipEvent_t &e : pl->kev)
             if (hipEventCreate(&e) != hipSuccess)
                 return fail(PROCELL_E_HIP);
     }
@@ -449,11 +464,17 @@ int run_plan(procell_plan *pl, uint64_t seed)
     if (pl->timing)
         HIP_TRY(hipEventRecord(pl->ev[0], pl->stream));
     HIP_TRY(launch_zero(pl->args.counters, pl->zero_words, pl->stream));
-    if (pl->timing)
+    const size_t slot = (size_t)(pl->runs % kTimedRuns);
+    if (pl->timing) {
         HIP_TRY(hipEventRecord(pl->ev[1], pl->stream));
+        HIP_TRY(hipEventRecord(pl->kev[2 * slot], pl->stream));
+    }
     HIP_TRY(launch_tree(pl->args, pl->shape, pl->stream));
-    if (pl->timing)
+    if (pl->timing) {
+        HIP_TRY(hipEventRecord(pl->kev[2 * slot + 1], pl->stream));
         HIP_TRY(hipEventRecord(pl->ev[2], pl->stream));
+    }
+    ++pl->runs;
     pl->ran = true;
     return PROCELL_OK;
 }
@@ -625,6 +646,27 @@ int procell_plan_destroy(procell_plan *plan)
 }
 
 void *procell_plan_stream(procell_plan *plan) { return plan ? (void *)plan->stream : nullptr; }
+
+int procell_plan_kernel_times(procell_plan *plan, double *ms, uint32_t max_runs, uint32_t *n_runs)
+{
+    if (!plan || !n_runs || (max_runs && !ms))
+        return PROCELL_E_INVALID;
+    if (!plan->timing)
+        return PROCELL_E_INVALID;
+    HIP_TRY(hipSetDevice(plan->device));
+    HIP_TRY(hipStreamSynchronize(plan->stream));
+    const uint64_t avail = plan->runs < kTimedRuns ? plan->runs : kTimedRuns;
+    const uint32_t n = (uint32_t)(avail < max_runs ? avail : max_runs);
+    for (uint32_t i = 0; i < n; ++i) {
+        const size_t slot = (size_t)((plan->runs - n + i) % kTimedRuns);
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, plan->kev[2 * slot], plan->kev[2 * slot + 1]));
+        ms[i] = t;
+    }
+    *n_runs = n;
+    plan->runs = 0;
+    return PROCELL_OK;
+}
 
 int procell_simulate(const procell_params *p, procell_result *r)
 {

@@ -61,7 +61,9 @@ struct RunArgs {
     uint32_t claim;         // initial cells per claim from the workgroup's own segment
     uint32_t steal_claim;   // initial cells per claim from another segment
     uint32_t idle_limit;    // idle lanes tolerated before a wavefront refills
-    const uint64_t *row_start;   // [n_rows + 1]
+    uint32_t rows_in_lds;   // nonzero: the row tables are copied to LDS
+    uint32_t pad;
+    const uint32_t *row_start;   // [n_rows + 1] first cell of each nonzero row (cells < 2^32)
     const uint32_t *row_info;    // [n_rows]: generation cap | (value >= phi) << 8
     const double *type_cum;      // [n_types] sorted order
     const double *type_mean;
@@ -81,7 +83,7 @@ struct LaunchShape {
 };
 
 size_t tree_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32_t stack_levels,
-                      uint32_t block_size);
+                      uint32_t block_size, uint32_t lds_rows);
 hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t st);
 hipError_t launch_tree(const RunArgs &a, const LaunchShape &ls, hipStream_t st);
 hipError_t launch_debug_draws(uint64_t seed, const uint64_t *cell, const uint64_t *node,

@@ -74,7 +74,7 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r)
 }
 
 // Last row in [lo, hi) whose first cell is <= cell (row_start[lo] <= cell).
-__device__ __forceinline__ uint32_t find_row(const uint64_t *row_start, uint32_t lo, uint32_t hi, uint64_t cell)
+__device__ __forceinline__ uint32_t find_row(const uint32_t *row_start, uint32_t lo, uint32_t hi, uint64_t cell)
 {
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -90,12 +90,13 @@ __device__ __forceinline__ uint32_t find_row(const uint64_t *row_start, uint32_t
 // timer is > 0 (cell.cu:106-122 redraws while the timer is <= 0).  Every
 // rejected candidate is counted.
 __device__ __forceinline__ double truncated_timer(uint64_t seed, uint64_t cell, uint64_t node,
-                                                  double mean, double sd, uint32_t &rejected)
+                                                  double mean, double sd, uint32_t &rejected,
+                                                  const dev::Coeffs &K)
 {
     for (uint32_t att = 1; att < 65536u; ++att) {
         const Words w = dev::draw_block(seed, cell, node, att);
         double z0, z1;
-        dev::normal_pair(w, z0, z1);
+        dev::normal_pair(w, z0, z1, K);
         const double c0 = fma(z0, sd, mean);
         if (c0 > 0.0)
             return c0;
@@ -119,13 +120,11 @@ struct RootSlot {
     double s0;
     uint32_t cell;
     uint32_t row_type;  // row | sorted type << 24
-    uint32_t kcap;
-    uint32_t pad[3];
 };
 
 // First row of a claimed range: 64-ary search with one load per lane per
 // round over rows [lo, hi) (row_start[lo] <= cell < row_start[hi]).
-__device__ __forceinline__ uint32_t find_row_wave(const uint64_t *row_start, uint32_t lo, uint32_t hi,
+__device__ __forceinline__ uint32_t find_row_wave(const uint32_t *row_start, uint32_t lo, uint32_t hi,
                                                   uint64_t cell, uint32_t lane)
 {
     while (hi - lo > 1) {
@@ -142,6 +141,30 @@ __device__ __forceinline__ uint32_t find_row_wave(const uint64_t *row_start, uin
     return lo;
 }
 
+// Row of the cell c_cur + lane of a root step (lanes < n_valid), given the row
+// row_cur of c_cur.  Either the whole step lies in row_cur, or the rows that
+// start inside the step are marked in the wavefront's 64 flags (one load per
+// lane) and each lane counts the starts at or below it.
+__device__ __forceinline__ uint32_t step_row(const uint32_t *row_start, uint32_t n_rows, uint32_t row_cur,
+                                             uint64_t c_cur, uint32_t n_valid, uint32_t lane, uint32_t *flags)
+{
+    const uint64_t next = row_cur + 1 < n_rows ? (uint64_t)row_start[row_cur + 1] : ~0ull;
+    if (__builtin_amdgcn_readfirstlane((uint32_t)(next > c_cur + n_valid - 1)))
+        return row_cur;
+    flags[lane] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t ri = row_cur + 1 + lane;
+    if (ri < n_rows) {
+        const uint64_t t = (uint64_t)row_start[ri] - c_cur;  // >= 0: row_cur holds c_cur
+        if (t < n_valid)
+            flags[t] = 1u;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t m = __ballot(flags[lane] != 0u);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    return row_cur + (uint32_t)__popcll(m & upto);
+}
+
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ uint64_t uni64(uint64_t x)
 {
@@ -154,8 +177,11 @@ struct Lane {
     uint64_t cell, node, pend;
     uint64_t tb_glob;   // device tally index of (row, generation 0, type)
     double s, mean, sd;
+    double tl, tr;      // daughters' timers of the division in progress
     uint32_t k, kcap;
     uint32_t tb_lds;    // window index of (row, generation 0, type), or kNoWindow
+    uint32_t redo;      // bit 0 / bit 1: left / right daughter's timer still to redraw
+    uint32_t att;       // attempt number of that redraw
     bool active;
 };
 
@@ -163,10 +189,11 @@ struct Counts {
     uint32_t div, rec, drop, rej, lost, cells, pool;
 };
 
-// LDS layout (dynamic): type tables (cum, mean, sd: double [T]; file: u32
-// [T]) | window rows (first cell: u64 [win_rows + 1]; info: u32 [win_rows])
-// | window tally u32 [win_rows * kdim * T] | stack double [stack_levels *
-// blockDim] | ring RootSlot [waves * kRing]
+// LDS layout (dynamic): stream coefficients | type tables (cum, mean, sd:
+// double [T]; file: u32 [T]) | row tables when they fit (first cell: u32
+// [n_rows + 1]; info: u32 [n_rows]) | window tally u32 [win_rows * kdim * T
+// + 1 spare] | stack double [stack_levels * blockDim] | ring RootSlot [waves
+// * (kRing + 1)] | row-start flags u32 [waves * 64]
 __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -177,35 +204,51 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
     const uint32_t T = a.n_types;
     const uint32_t kdim = a.kdim;
     const uint32_t win_len = a.win_rows * kdim * T;
-    double *t_cum = (double *)lds;
+    const uint32_t lds_rows = a.rows_in_lds ? a.n_rows : 0u;
+    dev::Coeffs *k_lds = (dev::Coeffs *)lds;
+    double *t_cum = (double *)(lds + (sizeof(dev::Coeffs) + 15) / 16 * 16);
     double *t_mean = t_cum + T;
     double *t_sd = t_mean + T;
     uint32_t *t_file = (uint32_t *)(t_sd + T);
-    uint64_t *w_start = (uint64_t *)(lds + ((size_t)T * 28 + 15) / 16 * 16);
-    uint32_t *w_info = (uint32_t *)(w_start + a.win_rows + 1);
-    unsigned int *win = (unsigned int *)((unsigned char *)w_start + ((size_t)a.win_rows * 12 + 8 + 15) / 16 * 16);
+    uint32_t *r_start = (uint32_t *)((unsigned char *)t_cum + ((size_t)T * 28 + 15) / 16 * 16);
+    uint32_t *r_info = r_start + lds_rows + 1;
+    unsigned int *win = (unsigned int *)((unsigned char *)r_start + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16);
     double *stack = (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
-    RootSlot *ring = (RootSlot *)(stack + (size_t)a.stack_levels * bdim) + wave * (kRing + 1);
+    RootSlot *ring_all = (RootSlot *)(stack + (size_t)a.stack_levels * bdim);
+    RootSlot *ring = ring_all + wave * (kRing + 1);
+    uint32_t *flags = (uint32_t *)(ring_all + (bdim / kWave) * (kRing + 1)) + wave * kWave;
 
     const uint32_t seg0 = blockIdx.x;
     const uint32_t n_seg = gridDim.x;
-    const uint32_t row_lo = a.seg_row[seg0];
-    const uint32_t row_hi = row_lo + a.win_rows < a.n_rows ? row_lo + a.win_rows : a.n_rows;
+    const uint32_t row_lo = a.seg_row[seg0];  // first row of the tally window
 
+    // first claim of the wavefront's own segment, issued before the set-up
+    // so its latency overlaps it (lane 0 holds the result)
+    unsigned long long pf_raw = 0;
+    if (lane == 0)
+        pf_raw = atomicAdd(&a.seg_next[seg0], (unsigned long long)a.claim);
+    const uint64_t own_begin = uni64(a.seg_begin[seg0]);
+    const uint64_t own_len = uni64(a.seg_begin[seg0 + 1]) - own_begin;
+    if (tid == 0)
+        *k_lds = dev::coeffs();
     for (uint32_t i = tid; i < T; i += bdim) {
         t_cum[i] = a.type_cum[i];
         t_mean[i] = a.type_mean[i];
         t_sd[i] = a.type_sd[i];
         t_file[i] = a.type_file[i];
     }
-    // window rows [row_lo, row_hi): their first cells and generation caps
-    for (uint32_t i = tid; i <= row_hi - row_lo; i += bdim)
-        w_start[i] = a.row_start[row_lo + i];
-    for (uint32_t i = tid; i < row_hi - row_lo; i += bdim)
-        w_info[i] = a.row_info[row_lo + i];
+    // row tables: first cell and generation cap of every nonzero row
+    for (uint32_t i = tid; i <= lds_rows; i += bdim)
+        r_start[i] = a.row_start[i];
+    for (uint32_t i = tid; i < lds_rows; i += bdim)
+        r_info[i] = a.row_info[i];
     for (uint32_t i = tid; i <= win_len; i += bdim)
         win[i] = 0u;  // win[win_len] absorbs the adds of lanes outside the window
     __syncthreads();
+
+    // the stream's FP64 coefficients, held in vector registers for the whole
+    // kernel (loaded from LDS, so they are not re-materialised as scalar literals)
+    const dev::Coeffs K = *k_lds;
 
     // wavefront-uniform queue state (scalar registers)
     uint64_t c_cur = 0, c_end = 0;
@@ -216,8 +259,10 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
 
     Lane L;
     L.cell = L.node = L.pend = L.tb_glob = 0;
-    L.s = L.mean = L.sd = 0.0;
+    L.s = L.mean = L.sd = L.tl = L.tr = 0.0;
     L.k = L.kcap = 0;
+    L.redo = 0;
+    L.att = 1;
     L.tb_lds = kNoWindow;
     L.active = false;
     Counts n = {0, 0, 0, 0, 0, 0, 0};
@@ -230,48 +275,62 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
         while (!roots_done && rb_count < n_idle) {
             // ---- claim a range of initial cells if the current one is used up ----
             while (c_cur >= c_end && !roots_done) {
-                // own segment while it lasts; then the segment with the most
-                // cells left among the next 64 (one load per lane)
-                uint32_t sg = seg0;
-                if (victim) {
-                    const uint32_t cand = (seg0 + victim + lane) % n_seg;
-                    const uint64_t len = a.seg_begin[cand + 1] - a.seg_begin[cand];
-                    const uint64_t used = __atomic_load_n(&a.seg_next[cand], __ATOMIC_RELAXED);
-                    uint64_t left = used < len && victim + lane < n_seg ? len - used : 0;
-                    uint32_t best = lane;
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) {
-                        const uint64_t ol = __shfl_xor(left, off);
-                        const uint32_t ob = __shfl_xor(best, off);
-                        if (ol > left || (ol == left && ob < best)) {
-                            left = ol;
-                            best = ob;
-                        }
+                if (!victim) {
+                    // own segment: take the claim issued ahead of time (its
+                    // latency overlapped the previous range) and issue the next
+                    const uint64_t sb = own_begin;
+                    const uint64_t len = own_len;
+                    const uint64_t base = uni64(__shfl(pf_raw, 0));
+                    if (base < len) {
+                        c_cur = uni64(sb + base);
+                        c_end = uni64(sb + (base + a.claim < len ? base + a.claim : len));
+                        if (lane == 0)
+                            pf_raw = atomicAdd(&a.seg_next[seg0], (unsigned long long)a.claim);
+                        if (lds_rows)
+                            row_cur = uni(find_row_wave(r_start, 0, a.n_rows, c_cur, lane));
+                        else
+                            row_cur = uni(find_row_wave(a.row_start, 0, a.n_rows, c_cur, lane));
+                    } else {
+                        victim = 1;  // own segment exhausted: steal
                     }
-                    if (uni64(left) == 0) {
-                        victim = uni(victim + kWave);
-                        if (victim >= n_seg)
-                            roots_done = 1;
-                        continue;
-                    }
-                    sg = uni((seg0 + victim + best) % n_seg);
+                    continue;
                 }
+                // the segment with the most unclaimed cells among the next 64
+                // (one load per lane); move on when all 64 are empty
+                const uint32_t cand = (seg0 + victim + lane) % n_seg;
+                const uint64_t clen = a.seg_begin[cand + 1] - a.seg_begin[cand];
+                const uint64_t used = __atomic_load_n(&a.seg_next[cand], __ATOMIC_RELAXED);
+                uint64_t left = used < clen && victim + lane < n_seg ? clen - used : 0;
+                uint32_t best = lane;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const uint64_t ol = __shfl_xor(left, off);
+                    const uint32_t ob = __shfl_xor(best, off);
+                    if (ol > left || (ol == left && ob < best)) {
+                        left = ol;
+                        best = ob;
+                    }
+                }
+                if (uni64(left) == 0) {
+                    victim = uni(victim + kWave);
+                    if (victim >= n_seg)
+                        roots_done = 1;
+                    continue;
+                }
+                const uint32_t sg = uni((seg0 + victim + best) % n_seg);
                 const uint64_t sb = a.seg_begin[sg];
                 const uint64_t len = a.seg_begin[sg + 1] - sb;
-                const uint32_t want = victim ? a.steal_claim : a.claim;
-                unsigned long long base = ~0ull;
+                unsigned long long sbase = ~0ull;
                 if (lane == 0)
-                    base = atomicAdd(&a.seg_next[sg], (unsigned long long)want);
-                base = uni64(__shfl(base, 0));
+                    sbase = atomicAdd(&a.seg_next[sg], (unsigned long long)a.steal_claim);
+                const uint64_t base = uni64(__shfl(sbase, 0));
                 if (base < len) {
                     c_cur = uni64(sb + base);
-                    c_end = uni64(sb + (base + want < len ? base + want : len));
-                    if (row_hi > row_lo && c_cur >= w_start[0] && c_cur < w_start[row_hi - row_lo])
-                        row_cur = uni(row_lo + find_row_wave(w_start, 0, row_hi - row_lo, c_cur, lane));
+                    c_end = uni64(sb + (base + a.steal_claim < len ? base + a.steal_claim : len));
+                    if (lds_rows)
+                        row_cur = uni(find_row_wave(r_start, 0, a.n_rows, c_cur, lane));
                     else
                         row_cur = uni(find_row_wave(a.row_start, 0, a.n_rows, c_cur, lane));
-                } else if (!victim) {
-                    victim = 1;
                 }
             }
             if (c_cur >= c_end)
@@ -281,25 +340,13 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             const uint32_t n_valid = uni(c_end - c_cur < kWave ? (uint32_t)(c_end - c_cur) : kWave);
             const uint64_t c = c_cur + lane;
             const bool valid = lane < n_valid;
-            const bool in_win = row_cur >= row_lo && row_cur + 1 < row_hi;
-            const uint64_t next_start = row_cur + 1 >= a.n_rows ? ~0ull
-                                        : (in_win ? w_start[row_cur + 1 - row_lo] : a.row_start[row_cur + 1]);
-            uint32_t row = row_cur;
-            if (uni64(next_start) <= c_cur + n_valid - 1) {
-                // rare: the 64 cells span several rows
-                if (valid) {
-                    const uint32_t hi = row_cur + kWave + 1 < a.n_rows ? row_cur + kWave + 1 : a.n_rows;
-                    if (in_win && hi <= row_hi)
-                        row = row_lo + find_row(w_start, row_cur - row_lo, hi - row_lo, c);
-                    else
-                        row = find_row(a.row_start, row_cur, hi, c);
-                }
-            }
-            const bool row_in_win = row >= row_lo && row < row_hi;
-            uint32_t info = w_info[row_in_win ? row - row_lo : 0u];
-            if (__builtin_expect(__ballot(valid && !row_in_win) != 0, 0)) {
-                if (valid && !row_in_win)
-                    info = a.row_info[row];
+            uint32_t row, info;
+            if (lds_rows) {
+                row = step_row(r_start, a.n_rows, row_cur, c_cur, n_valid, lane, flags);
+                info = r_info[valid ? row : row_cur];
+            } else {
+                row = step_row(a.row_start, a.n_rows, row_cur, c_cur, n_valid, lane, flags);
+                info = a.row_info[valid ? row : row_cur];
             }
             const uint32_t kc = info & 0xFFu;
             const bool ge_phi = (info >> 8) & 1u;
@@ -321,7 +368,7 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             // first timer: stream of node 1 from attempt 1 (cell.cu:106-122)
             const Words w1 = dev::draw_block(a.seed, c, 1, 1);
             double z0, z1;
-            dev::normal_pair(w1, z0, z1);
+            dev::normal_pair(w1, z0, z1, K);
             const double c0 = fma(z0, sdv, m);
             const double c1 = fma(z1, sdv, m);
             const bool live = valid && !quiescent;
@@ -334,7 +381,7 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
                     for (uint32_t att = 2; att < 65536u; ++att) {
                         const Words wr = dev::draw_block(a.seed, c, 1, att);
                         double y0, y1;
-                        dev::normal_pair(wr, y0, y1);
+                        dev::normal_pair(wr, y0, y1, K);
                         const double d0 = fma(y0, sdv, m);
                         if (d0 > 0.0) { tau0 = d0; break; }
                         ++n.rej;
@@ -370,12 +417,9 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             slot.s0 = s0;
             slot.cell = (uint32_t)c;
             slot.row_type = row | (jt << 24);
-            slot.kcap = kc;
             rb_count = uni(rb_count + (uint32_t)__popcll(dmask));
             row_cur = uni(__shfl(row, (int)(n_valid - 1)));
             c_cur = uni64(c_cur + n_valid);
-            if (c_cur >= c_end && victim == 0 && c_end == a.seg_begin[seg0 + 1])
-                victim = 1;  // own segment exhausted: next claims steal
         }
         if (rb_count && n_idle) {
             // ---- hand buffered roots to idle lanes (branch-free) ----
@@ -391,7 +435,9 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             L.node = take ? 1ull : L.node;
             L.k = take ? 0u : L.k;
             L.pend = take ? 0ull : L.pend;
-            L.kcap = take ? e.kcap : L.kcap;
+            L.redo = take ? 0u : L.redo;
+            const uint32_t kcap_e = (lds_rows ? r_info[min(row, a.n_rows - 1)] : a.row_info[min(row, a.n_rows - 1)]) & 0xFFu;
+            L.kcap = take ? kcap_e : L.kcap;
             L.mean = take ? t_mean[jt] : L.mean;
             L.sd = take ? t_sd[jt] : L.sd;
             L.tb_lds = take ? (off < a.win_rows ? off * kdim * T + tf : kNoWindow) : L.tb_lds;
@@ -431,6 +477,7 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
                     L.mean = d_mean;
                     L.sd = d_sd;
                     L.pend = 0;
+                    L.redo = 0;
                     L.active = true;
                 }
             }
@@ -446,64 +493,85 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
         // handful of scalar instructions (one scalar unit serves the whole
         // CU).  Idle lanes run it too with their results discarded.
         do {
-            const bool act = L.active;
-            double tl, tr;
-            {
-                const Words w = dev::draw_block(a.seed, L.cell, L.node, 0);
-                double z0, z1;
-                dev::normal_pair(w, z0, z1);
-                tl = fma(z0, L.sd, L.mean);
-                tr = fma(z1, L.sd, L.mean);
-            }
-            const bool need_l = act && !(tl > 0.0);
-            const bool need_r = act && !(tr > 0.0);
-            if (__builtin_expect(__ballot(need_l || need_r) != 0, 0)) {
-                // rare: truncated-normal redraws (cell.cu:106-122)
-                if (need_l) {
-                    ++n.rej;
-                    tl = truncated_timer(a.seed, L.cell, 2 * L.node, L.mean, L.sd, n.rej);
-                }
-                if (need_r) {
-                    ++n.rej;
-                    tr = truncated_timer(a.seed, L.cell, 2 * L.node + 1, L.mean, L.sd, n.rej);
-                }
-            }
+            // One Philox block + Box-Muller pair per lane and step: either the
+            // pair block of a new division (timers of both daughters) or one
+            // block of a daughter's truncated-normal redraw stream
+            // (cell.cu:106-122); a lane with a redraw pending does not advance
+            // its tree in that step.  Flags are 0/1 integers so the step's
+            // logic stays in vector instructions.
+            const uint32_t act = L.active ? 1u : 0u;
+            const uint32_t fresh = L.redo == 0 ? 1u : 0u;
+            const uint32_t rright = (L.redo & 1u) ? 0u : 1u;   // redraw the right daughter?
+            const uint64_t bnode = fresh ? L.node : (L.node << 1) | (uint64_t)rright;
+            const Words w = dev::draw_block(a.seed, L.cell, bnode, fresh ? 0u : L.att);
+            double z0, z1;
+            dev::normal_pair(w, z0, z1, K);
+            const double c0 = fma(z0, L.sd, L.mean);
+            const double c1 = fma(z1, L.sd, L.mean);
+            const uint32_t r0 = c0 > 0.0 ? 0u : 1u;
+            const uint32_t r1 = c1 > 0.0 ? 0u : 1u;
+            // redraw: first candidate > 0 of this block; after 65535 blocks the mean
+            const uint32_t got = (r0 & r1) ^ 1u;
+            const uint32_t give_up = (got ^ 1u) & (L.att >= 65535u ? 1u : 0u);
+            const double v = give_up ? L.mean : (r0 ? c1 : c0);
+            const uint32_t done = got | give_up;
+            const uint32_t rbit = rright ? 2u : 1u;
+            const double tl = fresh ? c0 : ((done & (rright ^ 1u)) ? v : L.tl);
+            const double tr = fresh ? c1 : ((done & rright) ? v : L.tr);
+            const uint32_t redo = fresh ? (r0 | (r1 << 1)) : (done ? L.redo & ~rbit : L.redo);
+            n.div += act & fresh;
+            n.rej += act * (fresh ? r0 + r1 : r0 + (r0 & r1));
+            L.tl = tl;
+            L.tr = tr;
+            L.att = fresh | done ? 1u : L.att + 1u;
+            L.redo = act ? redo : 0u;
+            const uint32_t cmp = act & (redo == 0 ? 1u : 0u);  // division complete this step
+
             // daughters (proliferation.cu:317-381): out of time -> tallied,
             // f/2 <= phi -> dropped, else divides later
             const double sl = L.s + tl;
             const double sr = L.s + tr;
             const uint32_t k1 = L.k + 1;
-            const bool rec_l = sl > a.t_max;
-            const bool rec_r = sr > a.t_max;
-            const bool div_l = !rec_l && k1 < L.kcap;
-            const bool div_r = !rec_r && k1 < L.kcap;
-            const uint32_t nrec = act ? (uint32_t)rec_l + (uint32_t)rec_r : 0u;
-            n.div += act ? 1u : 0u;
+            const uint32_t rec_l = sl > a.t_max ? 1u : 0u;
+            const uint32_t rec_r = sr > a.t_max ? 1u : 0u;
+            const uint32_t can = k1 < L.kcap ? 1u : 0u;
+            const uint32_t div_l = (rec_l ^ 1u) & can;
+            const uint32_t div_r = (rec_r ^ 1u) & can;
+            const uint32_t nrec = cmp * (rec_l + rec_r);
             n.rec += nrec;
-            n.drop += act ? (uint32_t)(!rec_l && !div_l) + (uint32_t)(!rec_r && !div_r) : 0u;
+            n.drop += cmp * ((rec_l | div_l) ^ 1u) + cmp * ((rec_r | div_r) ^ 1u);
             const bool in_win = L.tb_lds != kNoWindow;
-            // idle lanes add 0 to the spare slot win[win_len]
-            atomicAdd(&win[act && in_win ? L.tb_lds + k1 * T : win_len], nrec);
+            // lanes without a completed division add 0 to the spare slot win[win_len]
+            atomicAdd(&win[(cmp && in_win) ? L.tb_lds + k1 * T : win_len], nrec);
             if (__builtin_expect(__ballot(!in_win && nrec != 0) != 0, 0)) {
                 if (!in_win && nrec != 0)
                     atomicAdd(&a.tally[L.tb_glob + (uint64_t)k1 * T], (unsigned long long)nrec);
             }
             // right daughter's completion time; level k1 holds no pending
-            // entry, so the store is harmless when it is not pushed
+            // entry of an active lane (idle lanes have none at all), so the
+            // store is harmless when it is not pushed
             stack[(size_t)(act ? k1 - 1 : 0u) * bdim + tid] = sr;
-            const bool go_l = act && div_l;
-            const bool go_r = act && !div_l && div_r;
-            const bool pop = act && !div_l && !div_r;
-            const uint64_t pend2 = L.pend | (go_l && div_r ? 1ull << k1 : 0ull);
-            const bool has = pend2 != 0;
-            const uint32_t lvl = has ? 63u - (uint32_t)__builtin_clzll(pend2) : 1u;
+            const uint32_t go_l = cmp & div_l;
+            const uint32_t go_r = cmp & (div_l ^ 1u) & div_r;
+            const uint32_t pop = cmp & (div_l ^ 1u) & (div_r ^ 1u);
+            const uint64_t pend2 = L.pend | ((uint64_t)(go_l & div_r) << k1);
+            const uint32_t has = pend2 != 0 ? 1u : 0u;
+            const uint32_t lvl = has ? 63u - (uint32_t)__builtin_clzll(pend2 | 1ull) : 1u;
             const double sp = stack[(size_t)(lvl - 1) * bdim + tid];
-            const uint64_t up = (L.node >> ((L.k - lvl) & 63u)) | 1ull;  // pending right daughter
-            L.node = go_l ? 2 * L.node : (go_r ? 2 * L.node + 1 : (pop ? up : L.node));
-            L.s = go_l ? sl : (go_r ? sr : (pop ? sp : L.s));
-            L.pend = pop ? pend2 & ~(1ull << lvl) : pend2;
-            L.k = (go_l || go_r) ? k1 : (pop ? lvl : L.k);
-            L.active = act && (!pop || has);
+            // single-level selects only (nested ones turn into branches)
+            const uint32_t down = go_l | go_r;
+            const uint64_t child = (L.node << 1) | (uint64_t)go_r;
+            const uint32_t sh = pop ? ((L.k - lvl) & 63u) : 0u;
+            const uint64_t upn = (L.node >> sh) | (uint64_t)pop;  // pending right daughter
+            const double s_child = go_l ? sl : sr;
+            const double s_up = pop ? sp : L.s;
+            const uint32_t k_up = pop ? lvl : L.k;
+            const uint64_t clear = (uint64_t)pop << lvl;
+            L.node = down ? child : upn;
+            L.s = down ? s_child : s_up;
+            L.k = down ? k1 : k_up;
+            L.pend = pend2 & ~clear;
+            L.active = act && !(pop && !has);
         } while (uni((uint32_t)__popcll(__ballot(!L.active))) <= idle_limit);
     }
 
@@ -552,13 +620,16 @@ __global__ void debug_draws_kernel(uint64_t seed, const uint64_t *cell, const ui
 }  // namespace
 
 size_t tree_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32_t stack_levels,
-                      uint32_t block_size)
+                      uint32_t block_size, uint32_t lds_rows)
 {
-    const size_t tables = ((size_t)n_types * 28 + 15) / 16 * 16 + ((size_t)win_rows * 12 + 8 + 15) / 16 * 16;
+    const size_t tables = (sizeof(dev::Coeffs) + 15) / 16 * 16 + ((size_t)n_types * 28 + 15) / 16 * 16 +
+                          ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16;
     const size_t win = ((size_t)win_rows * kdim * n_types * 4 + 4 + 15) / 16 * 16;
     const size_t stk = (size_t)stack_levels * block_size * sizeof(double);
-    const size_t ring = (size_t)(block_size / kWave) * (kRing + 1) * sizeof(RootSlot);
-    return tables + win + stk + ring;
+    const size_t waves = block_size / kWave;
+    const size_t ring = waves * (kRing + 1) * sizeof(RootSlot);
+    const size_t flg = waves * kWave * sizeof(uint32_t);
+    return tables + win + stk + ring + flg;
 }
 
 hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t st)

@@ -79,40 +79,64 @@ __device__ __forceinline__ double uniform_oc(uint32_t hi, uint32_t lo)  // (0, 1
     return (mantissa53(hi, lo) + 1.0) * 0x1p-53;
 }
 
+// FP64 coefficients of the stream's log and sin/cos, gathered in one struct so
+// that a kernel can load them once into vector registers instead of
+// re-materialising every 64-bit literal in scalar registers inside its loop
+// (the CU's single scalar unit is shared by all its wavefronts).
+struct Coeffs {
+    double sqrt2;            // mantissa fold threshold of log
+    double rcp[3];           // quadratic seed of 1/(2+f): rcp[0] + rcp[1] d + rcp[2] d^2
+    double at[10];           // 2/(2n+1), n = 1..10
+    double ln2_hi, ln2_lo;   // ln 2 split: hi has 32 significant bits
+    double sn[9];            // (-1)^n (2 pi)^(2n+1) / (2n+1)!, n = 0..8
+    double cs[9];            // (-1)^n (2 pi)^(2n) / (2n)!, n = 1..9
+};
+
+__host__ __device__ constexpr Coeffs coeffs()
+{
+    return Coeffs{0x1.6a09e667f3bcdp+0,
+                  {0x1.7aa45399395aap+0, -0x1.72711bb58452fp-1, 0x1.df894e1f2af38p-4},
+                  {0x1.5555555555555p-1, 0x1.999999999999ap-2, 0x1.2492492492492p-2, 0x1.c71c71c71c71cp-3,
+                   0x1.745d1745d1746p-3, 0x1.3b13b13b13b14p-3, 0x1.1111111111111p-3, 0x1.e1e1e1e1e1e1ep-4,
+                   0x1.af286bca1af28p-4, 0x1.8618618618618p-4},
+                  0x1.62e42fee00000p-1,
+                  0x1.a39ef35793c76p-33,
+                  {0x1.921fb54442d18p+2, -0x1.4abbce625be53p+5, 0x1.466bc6775aae2p+6, -0x1.32d2cce62bd86p+6,
+                   0x1.50783487ee782p+5, -0x1.e3074fde8871fp+3, 0x1.e8f434d018d63p+1, -0x1.6fadb9f155744p-1,
+                   0x1.aaec32af93359p-4},
+                  {-0x1.3bd3cc9be45dep+4, 0x1.03c1f081b5ac4p+6, -0x1.55d3c7e3cbffap+6, 0x1.e1f506891babbp+5,
+                   -0x1.a6d1f2a204a8cp+4, 0x1.f9d38a3763cc3p+2, -0x1.b6e24f44b128fp+0, 0x1.20c62c2f2d7f5p-2,
+                   -0x1.2a0c591af8314p-5}};
+}
+
 // Natural log for positive normal x.
-__device__ __forceinline__ double log_pos(double x)
+__device__ __forceinline__ double log_pos(double x, const Coeffs &K)
 {
     const uint64_t bits = __builtin_bit_cast(uint64_t, x);
     int expo = (int)(bits >> 52) - 1023;
     double m = __builtin_bit_cast(double, (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
-    if (m > 0x1.6a09e667f3bcdp+0) {
+    if (m > K.sqrt2) {
         m = m * 0.5;
         expo = expo + 1;
     }
     const double f = m - 1.0;
     const double d = 2.0 + f;
-    double r = fma(fma(0x1.df894e1f2af38p-4, d, -0x1.72711bb58452fp-1), d, 0x1.7aa45399395aap+0);
+    double r = fma(fma(K.rcp[2], d, K.rcp[1]), d, K.rcp[0]);
     double e;
     e = fma(-d, r, 1.0); r = fma(r, e, r);
     e = fma(-d, r, 1.0); r = fma(r, e, r);
     e = fma(-d, r, 1.0); r = fma(r, e, r);
     const double s = f * r;
     const double z = s * s;
-    double p = 0x1.8618618618618p-4;        // 2/21
-    p = fma(p, z, 0x1.af286bca1af28p-4);    // 2/19
-    p = fma(p, z, 0x1.e1e1e1e1e1e1ep-4);    // 2/17
-    p = fma(p, z, 0x1.1111111111111p-3);    // 2/15
-    p = fma(p, z, 0x1.3b13b13b13b14p-3);    // 2/13
-    p = fma(p, z, 0x1.745d1745d1746p-3);    // 2/11
-    p = fma(p, z, 0x1.c71c71c71c71cp-3);    // 2/9
-    p = fma(p, z, 0x1.2492492492492p-2);    // 2/7
-    p = fma(p, z, 0x1.999999999999ap-2);    // 2/5
-    p = fma(p, z, 0x1.5555555555555p-1);    // 2/3
+    double p = K.at[9];
+#pragma unroll
+    for (int i = 8; i >= 0; --i)
+        p = fma(p, z, K.at[i]);
     const double R = p * z;
     const double hfsq = (0.5 * f) * f;
     const double dk = (double)expo;
-    const double lo_part = fma(s, hfsq + R, dk * 0x1.a39ef35793c76p-33);
-    return dk * 0x1.62e42fee00000p-1 - ((hfsq - lo_part) - f);
+    const double lo_part = fma(s, hfsq + R, dk * K.ln2_lo);
+    return dk * K.ln2_hi - ((hfsq - lo_part) - f);
 }
 
 // Square root for finite a >= 0.
@@ -132,30 +156,22 @@ __device__ __forceinline__ double sqrt_nonneg(double a)
 }
 
 // sin and cos of 2 pi u with u = uniform_co(hi, lo).
-__device__ __forceinline__ void sincos_turns(uint32_t hi, uint32_t lo, double &sn_out, double &cs_out)
+__device__ __forceinline__ void sincos_turns(uint32_t hi, uint32_t lo, double &sn_out, double &cs_out,
+                                             const Coeffs &K)
 {
     const double u = uniform_co(hi, lo);
     const uint32_t q = ((hi >> 29) + 1u) >> 1;   // nearest quarter turn, 0..4
     const double r = u - (double)q * 0.25;       // exact, |r| <= 1/8
     const double z = r * r;
-    double ps = 0x1.aaec32af93359p-4;
-    ps = fma(ps, z, -0x1.6fadb9f155744p-1);
-    ps = fma(ps, z, 0x1.e8f434d018d63p+1);
-    ps = fma(ps, z, -0x1.e3074fde8871fp+3);
-    ps = fma(ps, z, 0x1.50783487ee782p+5);
-    ps = fma(ps, z, -0x1.32d2cce62bd86p+6);
-    ps = fma(ps, z, 0x1.466bc6775aae2p+6);
-    ps = fma(ps, z, -0x1.4abbce625be53p+5);
-    const double sn = fma(r * z, ps, r * 0x1.921fb54442d18p+2);
-    double pc = -0x1.2a0c591af8314p-5;
-    pc = fma(pc, z, 0x1.20c62c2f2d7f5p-2);
-    pc = fma(pc, z, -0x1.b6e24f44b128fp+0);
-    pc = fma(pc, z, 0x1.f9d38a3763cc3p+2);
-    pc = fma(pc, z, -0x1.a6d1f2a204a8cp+4);
-    pc = fma(pc, z, 0x1.e1f506891babbp+5);
-    pc = fma(pc, z, -0x1.55d3c7e3cbffap+6);
-    pc = fma(pc, z, 0x1.03c1f081b5ac4p+6);
-    pc = fma(pc, z, -0x1.3bd3cc9be45dep+4);
+    double ps = K.sn[8];
+#pragma unroll
+    for (int i = 7; i >= 1; --i)
+        ps = fma(ps, z, K.sn[i]);
+    const double sn = fma(r * z, ps, r * K.sn[0]);
+    double pc = K.cs[8];
+#pragma unroll
+    for (int i = 7; i >= 0; --i)
+        pc = fma(pc, z, K.cs[i]);
     const double cs = fma(z, pc, 1.0);
     const uint32_t quad = q & 3u;
     const double a = (quad & 1u) ? cs : sn;   // |sin| candidate
@@ -165,14 +181,19 @@ __device__ __forceinline__ void sincos_turns(uint32_t hi, uint32_t lo, double &s
 }
 
 // Box-Muller (Box & Muller 1958) pair from one block.
-__device__ __forceinline__ void normal_pair(const Words &w, double &z0, double &z1)
+__device__ __forceinline__ void normal_pair(const Words &w, double &z0, double &z1, const Coeffs &K)
 {
     const double u1 = uniform_oc(w.w0, w.w1);
-    const double rho = sqrt_nonneg(-2.0 * log_pos(u1));
+    const double rho = sqrt_nonneg(-2.0 * log_pos(u1, K));
     double sn, cs;
-    sincos_turns(w.w2, w.w3, sn, cs);
+    sincos_turns(w.w2, w.w3, sn, cs, K);
     z0 = rho * cs;
     z1 = rho * sn;
+}
+
+__device__ __forceinline__ void normal_pair(const Words &w, double &z0, double &z1)
+{
+    normal_pair(w, z0, z1, coeffs());
 }
 
 }  // namespace dev

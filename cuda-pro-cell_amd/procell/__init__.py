@@ -92,7 +92,8 @@ _lib = None
 EXPORTS = (
     "procell_simulate", "procell_output_capacity", "procell_tally_shape", "procell_resolve_phi",
     "procell_strerror", "procell_abi_version", "procell_plan_create", "procell_plan_run",
-    "procell_plan_result", "procell_plan_destroy", "procell_plan_stream", "procell_read_histogram",
+    "procell_plan_result", "procell_plan_destroy", "procell_plan_stream", "procell_plan_kernel_times",
+    "procell_read_histogram",
     "procell_read_types", "procell_write_histogram", "procell_free", "procell_debug_draws",
 )
 
@@ -119,6 +120,7 @@ def lib():
     L.procell_plan_destroy.argtypes = [ctypes.c_void_p]
     L.procell_plan_stream.restype = ctypes.c_void_p
     L.procell_plan_stream.argtypes = [ctypes.c_void_p]
+    L.procell_plan_kernel_times.argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint32, _u32p]
     L.procell_read_histogram.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(Row)), _u64p]
     L.procell_read_types.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(CellType)), _u32p]
     L.procell_write_histogram.argtypes = [ctypes.c_char_p, R, ctypes.c_uint32, ctypes.c_int32]
@@ -256,6 +258,15 @@ class Plan:
         buf = _ResultBuffers(self.prob, want_raw)
         _check(lib().procell_plan_result(self.handle, ctypes.byref(buf.r)), "plan_result")
         return buf.result()
+
+    def kernel_times(self, max_runs: int = 256):
+        """Device ms of the simulation kernel for each run since the last call
+        (needs flags=FLAG_TIMING)."""
+        ms = np.zeros(max_runs, dtype=np.float64)
+        n = ctypes.c_uint32()
+        _check(lib().procell_plan_kernel_times(self.handle, ms.ctypes.data_as(_dp), max_runs, ctypes.byref(n)),
+               "plan_kernel_times")
+        return ms[: n.value].copy()
 
     @property
     def stream(self) -> int:

@@ -146,6 +146,11 @@ int procell_plan_run(procell_plan *plan, uint64_t seed);            /* enqueue o
 int procell_plan_result(procell_plan *plan, procell_result *r);     /* wait, download, merge */
 int procell_plan_destroy(procell_plan *plan);
 void *procell_plan_stream(procell_plan *plan);                       /* hipStream_t */
+/* With PROCELL_FLAG_TIMING: device time (ms) of the simulation kernel of each
+ * of the last min(runs, 256, max_runs) runs since the previous call, oldest
+ * first, from HIP events recorded on the plan's stream around the kernel.
+ * Waits for the stream; resets the run count. */
+int procell_plan_kernel_times(procell_plan *plan, double *ms, uint32_t max_runs, uint32_t *n_runs);
 
 /* ---- text formats ------------------------------------------------------
  * Replace load_fluorescences' file reading (src/io/parser.cu:68-124),

@@ -191,3 +191,23 @@ def test_plan_reuse_gives_same_results(procell_lib, oracle):
     assert not np.array_equal(outs[0].raw_tally, outs[1].raw_tally)
     tally, _, _ = oracle.simulate(oracle.Problem.from_workload(w, 6))
     assert np.array_equal(outs[1].raw_tally, tally)
+
+
+def test_many_rows_use_device_memory_row_tables(procell_lib, oracle):
+    # more rows than fit the LDS row tables: row lookups go to device memory
+    rng = np.random.default_rng(12)
+    n = 5000
+    values = np.exp(rng.uniform(np.log(2.0), np.log(5e4), n))
+    counts = rng.integers(0, 40, n).astype(np.uint64)
+    gp, op = _problem_pair(procell_lib, oracle, values, counts, [0.6, 0.3, 0.1], [25.0, 40.0, -1.0],
+                           [8.0, 10.0, -1.0], 110.0, 3.0, 31)
+    assert_same(procell_lib, oracle, gp, op)
+
+
+def test_sparse_rows_many_row_starts_per_step(procell_lib, oracle):
+    # one or two cells per row: every 64-cell step spans dozens of rows
+    values = [1.5 * 1.01**i for i in range(1500)]
+    counts = [1 + (i % 2) for i in range(1500)]
+    gp, op = _problem_pair(procell_lib, oracle, values, counts, [0.5, 0.5], [12.0, 30.0], [3.0, 9.0],
+                           70.0, 2.0, 17)
+    assert_same(procell_lib, oracle, gp, op)
