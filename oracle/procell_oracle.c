@@ -322,8 +322,8 @@ static int setup_build(const oracle_problem *p, oracle_setup *s)
     memset(s, 0, sizeof *s);
     if (!p || (p->n_rows && (!p->values || !p->counts)))
         return OR_E_INVALID;
-    if (!isfinite(p->t_max) && !(p->t_max > 0.0))
-        return OR_E_NONFINITE;
+    /* t_max = +inf is allowed (every division is in time); NaN and negative
+     * values are invalid arguments, like a negative -t (cmdargs.cpp:202). */
     if (isnan(p->t_max) || p->t_max < 0.0)
         return OR_E_INVALID;
     s->t_max = p->t_max;
@@ -341,8 +341,11 @@ static int setup_build(const oracle_problem *p, oracle_setup *s)
     }
     if (total >= OR_MAX_CELLS)
         return OR_E_TOO_MANY_CELLS;
-    if (p->n_types == 0 || !p->proportion || !p->mean || !p->sd)
-        return total ? OR_E_INVALID : OR_OK;
+    if (p->n_types && (!p->proportion || !p->mean || !p->sd))
+        return OR_E_INVALID;
+    /* no types: the proportions sum to 0, not 1 (parser.cu:46-66) */
+    if (p->n_types == 0)
+        return total ? OR_E_PROPORTIONS : OR_OK;
     /* parser.cu:46-66: left fold of proportions in file order. */
     double sum = 0.0;
     for (uint32_t j = 0; j < p->n_types; ++j) {
