@@ -32,6 +32,7 @@ LANE_OPS_PER_DIVISION = 180
 # MI355X VALU issue peak: 256 CUs x 64 lanes/clk x 2.4 GHz (lane-ops/s).
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "tree_kernel_hbm_traffic.txt")
+COUNTER_FILE = os.path.join(REPO, "profiles", "tree_kernel_valu_counters.txt")
 
 
 def log(*a):
@@ -46,6 +47,28 @@ def traffic_from_profile():
         return float(m.group(1)) if m else None
     except OSError:
         return None
+
+
+def issue_frac_from_profile():
+    """VALU issue utilisation (SQ_INSTS_VALU / (CUs x cycles)) from the
+    committed rocprofv3 counter summary."""
+    try:
+        with open(COUNTER_FILE) as f:
+            m = re.search(r"VALU issue utilisation.*=\s*([0-9.]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(workload, budget_s: float):
@@ -74,7 +97,8 @@ def cpu_baseline(workload, budget_s: float):
         "unit": "divisions/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"full workload, seeds 0..{seeds - 1} ({divs} divisions, {dt:.2f} s wall)",
+        "sample": f"full workload, seeds 0..{seeds - 1} ({divs} divisions, {dt:.2f} s wall), OpenMP oracle",
+        "cpu": cpu_model(),
         "one_core_value": st1["divisions"] / dt1,
         "one_core_sample": f"5% of the cells, seed 0 ({st1['divisions']} divisions, {dt1:.2f} s)",
     }
@@ -143,6 +167,13 @@ def main():
         stats = plan.result(want_raw=False).stats
         total_div += stats["divisions"]
     plan.close()
+    # end-to-end C-ABI call (host rows in, host histogram out: plan set-up,
+    # uploads, run, download, merge), reported beside the metric
+    e2e = []
+    for s in seeds[:5]:
+        t1 = time.perf_counter()
+        procell.simulate(procell.Problem.from_workload(w, s), want_raw=False)
+        e2e.append((time.perf_counter() - t1) * 1e3)
     elapsed = ev_ms * 1e-3
     value = total_div / elapsed
     kernel_ms = float(np.mean(kt)) if len(kt) else float("nan")
@@ -176,6 +207,7 @@ def main():
             "workgroups": stats["workgroups"],
             "block_size": stats["block_size"],
             "lds_bytes": stats["lds_bytes"],
+            "abi_end_to_end_ms": float(np.median(e2e)),
             "calibration": "not built (grad-steps/s not measured)",
         },
         "roofline": {
@@ -188,6 +220,9 @@ def main():
             "kernel": "tree_kernel",
             "kernel_ms": kernel_ms,
             "work_per_unit": f"{LANE_OPS_PER_DIVISION} lane-ops per division",
+            "peak_source": "MI355X FP64 vector peak 78.6 TFLOP/s / 2 = one wave64 VALU instruction "
+                           "per CU per clock (256 CUs x 64 lanes x 2.4 GHz)",
+            "issue_frac_counters": issue_frac_from_profile(),
         },
     }
     if not args.no_cpu:
