@@ -388,9 +388,18 @@ int create_plan(const procell_params *p, procell_plan **out)
     }
     ls.tree_blocks = (int)g;
     a.win_rows = win_rows;
-    a.claim = tu.chunk ? tu.chunk : 1024;  // cells per claim from the own segment
+    // cells per claim from the own segment: at least two claims per
+    // wavefront of the workgroup (a power of two in [64, 4096]) balances the
+    // atomics against the imbalance between the workgroup's wavefronts
+    uint32_t auto_claim = 64;
+    {
+        const double per_wave = (double)s.n_cells / (double)g / (double)(bs / 64);
+        while (auto_claim < 4096 && 2.0 * (double)(auto_claim * 2) <= per_wave)
+            auto_claim *= 2;
+    }
+    a.claim = tu.chunk ? tu.chunk : auto_claim;
     a.steal_claim = std::max<uint32_t>(1, std::min<uint32_t>(a.claim, 64));
-    a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 8;
+    a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 16;
 
     // ---- one device allocation: counters | tally | seg_next | tables ----
     size_t off = 0;

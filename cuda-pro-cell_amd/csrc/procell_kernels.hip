@@ -9,8 +9,9 @@
 //  * Work queue.  The initial cells are split on the host into one
 //    contiguous segment per workgroup, of roughly equal expected work.  A
 //    wavefront claims cells from its workgroup's segment with one atomic per
-//    claim; when the segment is exhausted it claims from the following
-//    segments (work stealing across workgroups).
+//    claim (the next claim issued ahead of time); when the segment is
+//    exhausted it claims from the segment with the most unclaimed cells (work
+//    stealing across workgroups), found by one parallel scan of all cursors.
 //  * Root step (cell.cu:25-143).  All 64 lanes set up one claimed initial
 //    cell each: type draw, truncated first timer, phase.  Cells that never
 //    divide are tallied or dropped at once; dividing ones go to the
@@ -38,6 +39,7 @@ using dev::Words;
 
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kRing = 128;  // buffered dividing roots per wavefront
+constexpr uint32_t kScanBatch = 8;  // segment-cursor loads in flight per lane while stealing
 constexpr uint32_t kNoWindow = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
@@ -253,7 +255,7 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
     // wavefront-uniform queue state (scalar registers)
     uint64_t c_cur = 0, c_end = 0;
     uint32_t row_cur = 0;
-    uint32_t victim = 0;      // 0: own segment; else offset of the next segments to scan
+    uint32_t victim = 0;      // 0: own segment; 1: own segment exhausted, steal
     uint32_t roots_done = 0;
     uint32_t rb_head = 0, rb_count = 0;
 
@@ -295,13 +297,31 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
                     }
                     continue;
                 }
-                // the segment with the most unclaimed cells among the next 64
-                // (one load per lane); move on when all 64 are empty
-                const uint32_t cand = (seg0 + victim + lane) % n_seg;
-                const uint64_t clen = a.seg_begin[cand + 1] - a.seg_begin[cand];
-                const uint64_t used = __atomic_load_n(&a.seg_next[cand], __ATOMIC_RELAXED);
-                uint64_t left = used < clen && victim + lane < n_seg ? clen - used : 0;
-                uint32_t best = lane;
+                // the segment with the most unclaimed cells, over all
+                // segments in one pass: lane i reads segments i, i + 64, ...,
+                // eight independent loads at a time, so the scan costs about
+                // one memory round trip; none left anywhere ends the roots
+                uint64_t left = 0;
+                uint32_t best = 0;
+                for (uint32_t base = 0; base < n_seg; base += kWave * kScanBatch) {
+                    uint64_t lens[kScanBatch], used[kScanBatch];
+#pragma unroll
+                    for (uint32_t j = 0; j < kScanBatch; ++j) {
+                        const uint32_t sg = base + j * kWave + lane;
+                        const uint32_t sc = sg < n_seg ? sg : n_seg - 1;
+                        lens[j] = sg < n_seg ? a.seg_begin[sc + 1] - a.seg_begin[sc] : 0;
+                        used[j] = __atomic_load_n(&a.seg_next[sc], __ATOMIC_RELAXED);
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < kScanBatch; ++j) {
+                        const uint64_t l = used[j] < lens[j] ? lens[j] - used[j] : 0;
+                        const uint32_t sg = base + j * kWave + lane;
+                        if (l > left) {
+                            left = l;
+                            best = sg;
+                        }
+                    }
+                }
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) {
                     const uint64_t ol = __shfl_xor(left, off);
@@ -312,12 +332,10 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
                     }
                 }
                 if (uni64(left) == 0) {
-                    victim = uni(victim + kWave);
-                    if (victim >= n_seg)
-                        roots_done = 1;
+                    roots_done = 1;
                     continue;
                 }
-                const uint32_t sg = uni((seg0 + victim + best) % n_seg);
+                const uint32_t sg = uni(best);
                 const uint64_t sb = a.seg_begin[sg];
                 const uint64_t len = a.seg_begin[sg + 1] - sb;
                 unsigned long long sbase = ~0ull;
