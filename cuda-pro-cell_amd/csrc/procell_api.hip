@@ -160,6 +160,7 @@ struct procell_plan {
     LaunchShape shape{};
     RunArgs args{};
     void *d_block = nullptr;  // one allocation holding every table
+    void *d_queue = nullptr;  // queue of dividing initial cells (16 B per initial cell)
     size_t tally_len = 0;
     size_t zero_words = 0;    // counters, tally and segment cursors, cleared per run
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -216,19 +217,23 @@ void destroy_plan(procell_plan *pl)
             (void)hipEventDestroy(e);
     if (pl->d_block)
         (void)hipFree(pl->d_block);
+    if (pl->d_queue)
+        (void)hipFree(pl->d_queue);
     if (pl->own_stream && pl->stream)
         (void)hipStreamDestroy(pl->stream);
     delete pl;
 }
 
-// Expected work of one initial cell of a row, in units of one division: the
-// root step plus the expected number of divisions, with division k+1
-// completing at D = tau0 (1 + U) + tau_1 + ... + tau_k approximated as normal
-// (truncation ignored).  Only used to balance the workgroups' segments.
+// Expected tree-kernel work of one initial cell of a row, in units of one
+// division: the expected number of divisions, with division k+1 completing
+// at D = tau0 (1 + U) + tau_1 + ... + tau_k approximated as normal
+// (truncation ignored), plus a small constant per cell.  The root kernel's
+// work is balanced separately (equal chunks).  Only used to balance the
+// tree kernel's segments.
 double row_cost(const Setup &s, uint32_t row)
 {
     const int kcap = (int)(s.row_info[row] & 0xFFu);
-    double w = 1.3;
+    double w = 0.02;
     double p_prev = 0.0;
     for (uint32_t j = 0; j < s.n_types; ++j) {
         const double m = s.type_mean[j];
@@ -249,6 +254,27 @@ double row_cost(const Setup &s, uint32_t row)
                 break;
         }
         w += p * e;
+    }
+    return w;
+}
+
+// Expected share of a row's initial cells that divide at least once (first
+// division completing by t_max, same normal approximation as row_cost).
+double row_dividing(const Setup &s, uint32_t row)
+{
+    if ((s.row_info[row] & 0xFFu) == 0)
+        return 0.0;
+    double w = 0.0, p_prev = 0.0;
+    for (uint32_t j = 0; j < s.n_types; ++j) {
+        const double m = s.type_mean[j];
+        const double sd = fabs(s.type_sd[j]);
+        const double p = s.type_cum[j] - p_prev;
+        p_prev = s.type_cum[j];
+        if (m < 0.0)
+            continue;
+        const double var = (7.0 / 3.0) * sd * sd + m * m / 12.0;
+        const double z = (s.t_max - 1.5 * m) / sqrt(var > 0.0 ? var : 1e-300);
+        w += p * 0.5 * erfc(-z / sqrt(2.0));
     }
     return w;
 }
@@ -353,7 +379,9 @@ int create_plan(const procell_params *p, procell_plan **out)
     const size_t budget = std::min(lds_per_cu / (size_t)occ, lds_per_block);
     const size_t row_bytes = (size_t)kdim * T * 4;
     uint32_t win_rows = budget > base + 16 ? (uint32_t)((budget - base - 16) / row_bytes) : 0;
-    const uint32_t g = (uint32_t)(n_cu * occ);
+    // segments: one per resident workgroup, at most kMaxSegments (the tree
+    // kernel's per-wavefront victim masks hold 32 x 64 segments)
+    const uint32_t g = (uint32_t)std::min<int64_t>((int64_t)n_cu * occ, kMaxSegments);
     std::vector<uint64_t> seg_begin;
     std::vector<uint32_t> seg_row;
     split_segments(s, g, seg_begin, seg_row);
@@ -387,27 +415,51 @@ int create_plan(const procell_params *p, procell_plan **out)
         win_rows = win_rows > 8 ? win_rows - win_rows / 8 : 0;
     }
     ls.tree_blocks = (int)g;
+    ls.root_block_size = 256;
+    ls.root_lds_bytes = root_lds_bytes(T, 256, lds_rows);
     a.win_rows = win_rows;
-    // cells per claim from the own segment: at least two claims per
+    // queue entries per claim from the own segment: at least two claims per
     // wavefront of the workgroup (a power of two in [64, 4096]) balances the
     // atomics against the imbalance between the workgroup's wavefronts
     uint32_t auto_claim = 64;
     {
-        const double per_wave = (double)s.n_cells / (double)g / (double)(bs / 64);
+        double entries = 0.0;
+        for (uint32_t r = 0; r < n_rows; ++r)
+            entries += row_dividing(s, r) * (double)(s.start[r + 1] - s.start[r]);
+        const double per_wave = entries / (double)g / (double)(bs / 64);
         while (auto_claim < 4096 && 2.0 * (double)(auto_claim * 2) <= per_wave)
             auto_claim *= 2;
     }
-    a.claim = tu.chunk ? tu.chunk : auto_claim;
+    a.claim = tu.chunk ? tu.chunk : std::max<uint32_t>(std::min<uint32_t>(auto_claim * 2, 1024), 64);
     a.steal_claim = std::max<uint32_t>(1, std::min<uint32_t>(a.claim, 64));
     a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 16;
 
     // ---- one device allocation: counters | tally | seg_next | tables ----
     size_t off = 0;
-    const size_t o_cnt = off; off += kNumCounters * 8;
+    const size_t o_cnt = off; off += (size_t)kCounterCopies * kNumCounters * 8;
     const size_t o_tally = off; off += pl->tally_len * 8;
     const size_t o_next = off; off += (size_t)g * 8;
+    const size_t o_total = off; off += (size_t)g * 8;
     pl->zero_words = off / 8;
     off = align_up(off, 256);
+    // root-kernel chunks: kChunk cells each, none spanning two segments
+    std::vector<uint32_t> chunk_first(g + 1, 0), chunk_seg;
+    for (uint32_t i = 0; i < g; ++i) {
+        const uint64_t len = seg_begin[i + 1] - seg_begin[i];
+        const uint64_t nc = (len + kChunk - 1) / kChunk;
+        chunk_first[i + 1] = chunk_first[i] + (uint32_t)nc;
+        chunk_seg.insert(chunk_seg.end(), (size_t)nc, i);
+    }
+    const size_t o_cseg = off; off = align_up(off + chunk_seg.size() * 4 + 4, 256);
+    const size_t o_cfirst = off; off = align_up(off + (size_t)(g + 1) * 4, 256);
+    a.n_chunks = (uint32_t)chunk_seg.size();
+    {
+        int occ_r = 0;
+        if (root_occupancy(256, ls.root_lds_bytes, &occ_r) != hipSuccess)
+            return fail(PROCELL_E_HIP);
+        const uint64_t resident = (uint64_t)n_cu * (uint64_t)std::max(occ_r, 1);
+        ls.root_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>(a.n_chunks, resident));
+    }
     const size_t o_rs = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
     const size_t o_ri = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
     const size_t o_cum = off; off = align_up(off + (size_t)T * 8, 256);
@@ -425,6 +477,8 @@ int create_plan(const procell_params *p, procell_plan **out)
     }
     if (hipMalloc(&pl->d_block, off) != hipSuccess)
         return fail(PROCELL_E_OUT_OF_MEMORY);
+    if (s.n_cells && hipMalloc(&pl->d_queue, (size_t)s.n_cells * sizeof(QueueEntry)) != hipSuccess)
+        return fail(PROCELL_E_OUT_OF_MEMORY);
     char *dbase = (char *)pl->d_block;
     a.t_max = s.t_max;
     a.n_cells = s.n_cells;
@@ -434,6 +488,10 @@ int create_plan(const procell_params *p, procell_plan **out)
     a.counters = (unsigned long long *)(dbase + o_cnt);
     a.tally = (unsigned long long *)(dbase + o_tally);
     a.seg_next = (unsigned long long *)(dbase + o_next);
+    a.seg_total = (unsigned long long *)(dbase + o_total);
+    a.chunk_seg = (const uint32_t *)(dbase + o_cseg);
+    a.chunk_first = (const uint32_t *)(dbase + o_cfirst);
+    a.queue = (QueueEntry *)pl->d_queue;
     a.row_start = (const uint32_t *)(dbase + o_rs);
     a.row_info = (const uint32_t *)(dbase + o_ri);
     a.type_cum = (const double *)(dbase + o_cum);
@@ -449,7 +507,8 @@ int create_plan(const procell_params *p, procell_plan **out)
     if (!up(o_rs, start32.data(), start32.size() * 4) || !up(o_ri, s.row_info.data(), n_rows * 4) ||
         !up(o_cum, s.type_cum.data(), s.type_cum.size() * 8) || !up(o_mean, s.type_mean.data(), s.type_mean.size() * 8) ||
         !up(o_sd, s.type_sd.data(), s.type_sd.size() * 8) || !up(o_tf, s.type_file.data(), s.type_file.size() * 4) ||
-        !up(o_sb, seg_begin.data(), seg_begin.size() * 8) || !up(o_sr, seg_row.data(), seg_row.size() * 4))
+        !up(o_sb, seg_begin.data(), seg_begin.size() * 8) || !up(o_sr, seg_row.data(), seg_row.size() * 4) ||
+        !up(o_cseg, chunk_seg.data(), chunk_seg.size() * 4) || !up(o_cfirst, chunk_first.data(), chunk_first.size() * 4))
         return fail(PROCELL_E_HIP);
     if (hipStreamSynchronize(pl->stream) != hipSuccess)
         return fail(PROCELL_E_HIP);
@@ -478,6 +537,7 @@ int run_plan(procell_plan *pl, uint64_t seed)
         HIP_TRY(hipEventRecord(pl->ev[1], pl->stream));
         HIP_TRY(hipEventRecord(pl->kev[2 * slot], pl->stream));
     }
+    HIP_TRY(launch_root(pl->args, pl->shape, pl->stream));
     HIP_TRY(launch_tree(pl->args, pl->shape, pl->stream));
     if (pl->timing) {
         HIP_TRY(hipEventRecord(pl->kev[2 * slot + 1], pl->stream));
@@ -500,17 +560,25 @@ int plan_result(procell_plan *pl, procell_result *r)
     if (r->capacity && !(r->value && r->count))
         return PROCELL_E_INVALID;
     HIP_TRY(hipSetDevice(pl->device));
-    std::vector<unsigned long long> cnt(kNumCounters);
+    std::vector<unsigned long long> cnt_all((size_t)kCounterCopies * kNumCounters), cnt(kNumCounters, 0);
     std::vector<int64_t> tally(pl->tally_len);
-    HIP_TRY(hipMemcpyAsync(cnt.data(), pl->args.counters, kNumCounters * 8, hipMemcpyDeviceToHost, pl->stream));
+    HIP_TRY(hipMemcpyAsync(cnt_all.data(), pl->args.counters, cnt_all.size() * 8, hipMemcpyDeviceToHost,
+                           pl->stream));
     if (pl->tally_len)
         HIP_TRY(hipMemcpyAsync(tally.data(), pl->args.tally, pl->tally_len * 8, hipMemcpyDeviceToHost, pl->stream));
     HIP_TRY(hipStreamSynchronize(pl->stream));
+    for (size_t i = 0; i < cnt_all.size(); ++i)
+        cnt[i % kNumCounters] += cnt_all[i];
     procell_stats &st = r->stats;
     memset(&st, 0, sizeof st);
     st.divisions = cnt[kDivisions];
-    st.recorded = cnt[kRecorded];
-    st.dropped = cnt[kDropped];
+    // recorded = tally sum; dropped from the event balance (every division
+    // adds one cell; every cell ends recorded, dropped or lost)
+    uint64_t recorded = 0;
+    for (size_t i = 0; i < pl->tally_len; ++i)
+        recorded += (uint64_t)tally[i];
+    st.recorded = recorded;
+    st.dropped = cnt[kCells] + cnt[kDivisions] - recorded - cnt[kLost];
     st.lost = cnt[kLost];
     st.rejected = cnt[kRejected];
     st.cells = cnt[kCells];

@@ -17,18 +17,36 @@ constexpr uint64_t kMaxCells = 0xFFFFFFFFull;
 constexpr uint64_t kMaxRows = 1ull << 24;
 constexpr uint32_t kMaxTypes = 256;
 constexpr int kMaxGeneration = 62;  // heap index of a node must fit 64 bits
+constexpr uint32_t kChunk = 1024;   // initial cells per root-kernel workgroup
+constexpr uint32_t kMaxSegments = 2048;  // tree-kernel workgroups (= segments) at most
+
+// Rows of the root kernel's generation-0 tally window (LDS, at most 8 KB).
+__host__ __device__ constexpr uint32_t root_win_rows(uint32_t n_types)
+{
+    return n_types <= 64 ? 32u : 2048u / n_types;
+}
+
+// A dividing initial cell, written by the root kernel for the tree kernel.
+struct QueueEntry {
+    double s0;          // completion time of its first division
+    uint32_t cell;      // initial cell id
+    uint32_t row_type;  // nonzero row | sorted type << 24
+};
 
 // Counters block in device memory (uint64 each).
 enum Counter : int {
     kPoolCount = 0,   // initial cells that divided at least once
     kDivisions = 1,
-    kRecorded = 2,
-    kDropped = 3,
+    kRecorded = 2,    // unused on the device: derived by the host
+    kDropped = 3,     // unused on the device: derived by the host
     kLost = 4,
     kRejected = 5,
     kCells = 6,
     kNumCounters = 8,
 };
+// Copies of the counters block: workgroup b adds into copy b % kCounterCopies,
+// so the device atomics on any one counter stay few; the host sums the copies.
+constexpr uint32_t kCounterCopies = 64;
 
 // Problem after validation: nonzero rows in file order, types sorted by
 // descending proportion (src/io/parser.cu:184), output bin set.
@@ -58,11 +76,11 @@ struct RunArgs {
     uint32_t kdim;          // generations + 1
     uint32_t stack_levels;  // per-lane LDS stack depth (levels 1 .. kdim-1)
     uint32_t win_rows;      // rows of the LDS tally window (0: none)
-    uint32_t claim;         // initial cells per claim from the workgroup's own segment
-    uint32_t steal_claim;   // initial cells per claim from another segment
+    uint32_t claim;         // queue entries per claim from the workgroup's own segment
+    uint32_t steal_claim;   // queue entries per claim from another segment
     uint32_t idle_limit;    // idle lanes tolerated before a wavefront refills
     uint32_t rows_in_lds;   // nonzero: the row tables are copied to LDS
-    uint32_t pad;
+    uint32_t n_chunks;      // root-kernel chunks
     const uint32_t *row_start;   // [n_rows + 1] first cell of each nonzero row (cells < 2^32)
     const uint32_t *row_info;    // [n_rows]: generation cap | (value >= phi) << 8
     const double *type_cum;      // [n_types] sorted order
@@ -71,24 +89,35 @@ struct RunArgs {
     const uint32_t *type_file;   // sorted position -> file index
     const uint64_t *seg_begin;   // [segments + 1] first initial cell of each workgroup's segment
     const uint32_t *seg_row;     // [segments] row of each segment's first cell (window base)
-    unsigned long long *seg_next;  // [segments] cells claimed so far per segment
+    unsigned long long *seg_next;  // [segments] queue entries claimed so far per segment
+    unsigned long long *seg_total; // [segments] queue entries per segment
+    const uint32_t *chunk_seg;   // [chunks] segment of each root-kernel chunk
+    const uint32_t *chunk_first; // [segments + 1] first chunk of each segment
+    QueueEntry *queue;           // [n_cells] dividing initial cells; segment g's entries
+                                 // start at seg_begin[g]
     unsigned long long *tally;   // [n_rows][kdim][n_types] file type order
-    unsigned long long *counters;  // [kNumCounters]
+    unsigned long long *counters;  // [kCounterCopies][kNumCounters]
 };
 
 struct LaunchShape {
-    int tree_blocks;
+    int tree_blocks;        // = segments
+    int root_blocks;        // persistent root workgroups (<= chunks)
     int tree_block_size;
     size_t tree_lds_bytes;
+    int root_block_size;
+    size_t root_lds_bytes;
 };
 
 size_t tree_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32_t stack_levels,
                       uint32_t block_size, uint32_t lds_rows);
+size_t root_lds_bytes(uint32_t n_types, uint32_t block_size, uint32_t lds_rows);
 hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t st);
+hipError_t launch_root(const RunArgs &a, const LaunchShape &ls, hipStream_t st);
 hipError_t launch_tree(const RunArgs &a, const LaunchShape &ls, hipStream_t st);
 hipError_t launch_debug_draws(uint64_t seed, const uint64_t *cell, const uint64_t *node,
                               const uint32_t *attempt, uint64_t n, uint32_t *words, double *z,
                               hipStream_t st);
 hipError_t tree_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu);
+hipError_t root_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu);
 
 }  // namespace procell

@@ -3,32 +3,39 @@
 // The reference materialises every tree level as a dense array of 40-byte
 // cells and recurses with device-side launches (src/simulation/
 // proliferation.cu:289-402), creating the initial cells in a separate pass
-// (cells_population.cu:74-117).  Here one persistent kernel does both and no
-// cell is ever stored in device memory:
+// (cells_population.cu:74-117).  Here two kernels run back to back on one
+// stream and no cell below the root is ever stored in device memory:
 //
-//  * Work queue.  The initial cells are split on the host into one
-//    contiguous segment per workgroup, of roughly equal expected work.  A
-//    wavefront claims cells from its workgroup's segment with one atomic per
-//    claim (the next claim issued ahead of time); when the segment is
-//    exhausted it claims from the segment with the most unclaimed cells (work
-//    stealing across workgroups), found by one parallel scan of all cursors.
-//  * Root step (cell.cu:25-143).  All 64 lanes set up one claimed initial
-//    cell each: type draw, truncated first timer, phase.  Cells that never
-//    divide are tallied or dropped at once; dividing ones go to the
-//    wavefront's ring buffer in LDS.
-//  * Division step (proliferation.cu:317-381).  Each busy lane divides its
-//    current cell: one Philox block and one Box-Muller pair give both
-//    daughters' timers; daughters that will not divide are tallied or dropped
-//    on the spot, so the lane's next cell is always a dividing one (the left
+//  * root_kernel (cell.cu:25-143).  One initial cell per lane, no divergence:
+//    type draw, truncated first timer, phase.  Cells that never divide are
+//    tallied (generation 0) or dropped at once; a dividing cell becomes a
+//    16-byte queue entry.  The initial cells are split on the host into one
+//    contiguous segment per tree-kernel workgroup, of roughly equal expected
+//    work, and each segment into chunks of kChunk cells, one root workgroup
+//    each, so every root workgroup has the same work.  A chunk stages its
+//    entries in LDS and appends them to its segment's part of the queue
+//    with one atomic, so each segment's entries are contiguous.
+//  * tree_kernel (proliferation.cu:317-381).  Persistent workgroups, one per
+//    segment.  A wavefront claims entries of its workgroup's segment with one
+//    atomic per claim (the next claim issued ahead of time) and keeps the
+//    next 64 entries in registers, one per lane, loaded ahead of use; when
+//    the segment is exhausted it claims from the segment with the most
+//    unclaimed entries (work stealing across workgroups), found by one
+//    parallel scan of all cursors.  Each busy lane divides its current cell
+//    per step: one Philox block and one Box-Muller pair give both daughters'
+//    timers; daughters that will not divide are tallied or dropped on the
+//    spot, so the lane's next cell is always a dividing one (the left
 //    daughter, else the right, else the deepest pending right daughter from
-//    the lane's column of the LDS stack, else a buffered root).
-//  * Once no initial cell is left anywhere, idle lanes take the shallowest
-//    pending right daughter of a busy lane in the same wavefront (work
-//    stealing inside the wavefront).
-//  * Tallies (counts by nonzero row, generation, type) accumulate in a u32
-//    LDS window over the first rows of the workgroup's own segment and are
-//    added to the int64 device tally at the end; rows outside the window use
-//    64-bit device atomics directly.
+//    the lane's column of the LDS stack, else a queue entry).  Once no entry
+//    is left anywhere, idle lanes take the shallowest pending right daughter
+//    of a busy lane in the same wavefront (work stealing inside the
+//    wavefront).
+//  * Tallies (counts by nonzero row, generation, type) of the tree kernel
+//    accumulate in a u32 LDS window over the first rows of the workgroup's
+//    segment and are added to the int64 device tally at the end; rows outside
+//    the window use 64-bit device atomics.  The root kernel aggregates its
+//    generation-0 tallies per wavefront before one device atomic per
+//    distinct (row, type).
 #include "procell_internal.hpp"
 #include "procell_stream.hpp"
 
@@ -38,8 +45,7 @@ namespace {
 using dev::Words;
 
 constexpr uint32_t kWave = 64;
-constexpr uint32_t kRing = 128;  // buffered dividing roots per wavefront
-constexpr uint32_t kScanBatch = 8;  // segment-cursor loads in flight per lane while stealing
+constexpr uint32_t kClaimAhead = 128;  // claimed entries left when the next own claim is issued
 constexpr uint32_t kNoWindow = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
@@ -75,40 +81,22 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r)
     return base;
 }
 
-// Last row in [lo, hi) whose first cell is <= cell (row_start[lo] <= cell).
-__device__ __forceinline__ uint32_t find_row(const uint32_t *row_start, uint32_t lo, uint32_t hi, uint64_t cell)
-{
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (row_start[mid] <= cell)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// Redraws from a node's truncated-normal stream, attempts 1, 2, ..., until a
-// timer is > 0 (cell.cu:106-122 redraws while the timer is <= 0).  Every
-// rejected candidate is counted.
-__device__ __forceinline__ double truncated_timer(uint64_t seed, uint64_t cell, uint64_t node,
-                                                  double mean, double sd, uint32_t &rejected,
-                                                  const dev::Coeffs &K)
+// 32-bit integer mix (the finaliser of MurmurHash3) for spreading probes.
+__device__ __forceinline__ uint32_t hash32(uint32_t x)
 {
-    for (uint32_t att = 1; att < 65536u; ++att) {
-        const Words w = dev::draw_block(seed, cell, node, att);
-        double z0, z1;
-        dev::normal_pair(w, z0, z1, K);
-        const double c0 = fma(z0, sd, mean);
-        if (c0 > 0.0)
-            return c0;
-        ++rejected;
-        const double c1 = fma(z1, sd, mean);
-        if (c1 > 0.0)
-            return c1;
-        ++rejected;
-    }
-    return mean;
+    x ^= x >> 16;
+    x *= 0x85EBCA6Bu;
+    x ^= x >> 13;
+    x *= 0xC2B2AE35u;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint64_t uni64(uint64_t x)
+{
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)x);
 }
 
 __global__ void zero_kernel(unsigned long long *p, uint64_t n)
@@ -118,14 +106,8 @@ __global__ void zero_kernel(unsigned long long *p, uint64_t n)
         p[i] = 0ull;
 }
 
-struct RootSlot {
-    double s0;
-    uint32_t cell;
-    uint32_t row_type;  // row | sorted type << 24
-};
-
-// First row of a claimed range: 64-ary search with one load per lane per
-// round over rows [lo, hi) (row_start[lo] <= cell < row_start[hi]).
+// First row of a range: 64-ary search with one load per lane per round over
+// rows [lo, hi) (row_start[lo] <= cell < row_start[hi]).
 __device__ __forceinline__ uint32_t find_row_wave(const uint32_t *row_start, uint32_t lo, uint32_t hi,
                                                   uint64_t cell, uint32_t lane)
 {
@@ -167,205 +149,125 @@ __device__ __forceinline__ uint32_t step_row(const uint32_t *row_start, uint32_t
     return row_cur + (uint32_t)__popcll(m & upto);
 }
 
-__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ uint64_t uni64(uint64_t x)
+// Per-lane event counts.  Recorded and dropped cells are not counted on the
+// device: the host takes recorded = the sum of the tally and dropped = cells
+// + divisions - recorded - lost (every division adds one cell, and every
+// cell ends recorded, dropped or lost).
+struct Counts {
+    uint32_t div, rej, lost, cells, pool;
+};
+
+// Adds a workgroup's counts to counter copy blockIdx.x % kCounterCopies:
+// wavefront sums, then one LDS atomic per wavefront and counter, then one
+// device atomic per counter.  `red` is kNumCounters u32 of LDS, zeroed and
+// followed by a barrier before the first call; every thread calls it.
+__device__ __forceinline__ void flush_counts(const Counts &n, unsigned long long *counters, uint32_t lane,
+                                             uint32_t *red)
 {
-    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
-           __builtin_amdgcn_readfirstlane((uint32_t)x);
+    constexpr uint32_t kN = 5;
+    const uint32_t v[kN] = {wave_sum(n.pool), wave_sum(n.div), wave_sum(n.lost), wave_sum(n.rej),
+                            wave_sum(n.cells)};
+    const uint32_t idx[kN] = {kPoolCount, kDivisions, kLost, kRejected, kCells};
+    if (lane < kN) {
+        uint32_t x = v[0];
+#pragma unroll
+        for (uint32_t i = 1; i < kN; ++i)
+            x = lane == i ? v[i] : x;
+        if (x)
+            atomicAdd(&red[lane], x);
+    }
+    __syncthreads();
+    if (threadIdx.x < kN) {
+        const uint32_t x = red[threadIdx.x];
+        uint32_t c = idx[0];
+#pragma unroll
+        for (uint32_t i = 1; i < kN; ++i)
+            c = threadIdx.x == i ? idx[i] : c;
+        if (x)
+            atomicAdd(&counters[(blockIdx.x % kCounterCopies) * kNumCounters + c], (unsigned long long)x);
+    }
 }
 
-// Lane state: the dividing cell a lane currently holds.
-struct Lane {
-    uint64_t cell, node, pend;
-    uint64_t tb_glob;   // device tally index of (row, generation 0, type)
-    double s, mean, sd;
-    double tl, tr;      // daughters' timers of the division in progress
-    uint32_t k, kcap;
-    uint32_t tb_lds;    // window index of (row, generation 0, type), or kNoWindow
-    uint32_t redo;      // bit 0 / bit 1: left / right daughter's timer still to redraw
-    uint32_t att;       // attempt number of that redraw
-    bool active;
-};
-
-struct Counts {
-    uint32_t div, rec, drop, rej, lost, cells, pool;
-};
-
-// LDS layout (dynamic): stream coefficients | type tables (cum, mean, sd:
-// double [T]; file: u32 [T]) | row tables when they fit (first cell: u32
-// [n_rows + 1]; info: u32 [n_rows]) | window tally u32 [win_rows * kdim * T
-// + 1 spare] | stack double [stack_levels * blockDim] | ring RootSlot [waves
-// * (kRing + 1)] | row-start flags u32 [waves * 64]
-__global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
+// ======================================================================
+// root_kernel: persistent workgroups; workgroup b takes chunks b, b + G,
+// b + 2G, ... (G workgroups), chunk c = cells [lo, hi) of segment
+// chunk_seg[c], kChunk cells (fewer at the segment's end); wavefront w takes
+// the w-th quarter of a chunk.  Chunks have equal cost, so the static
+// round-robin balances.
+// LDS: type tables (cum, mean, sd: double [T]; file: u32 [T]) | row tables
+// when they fit (first cell: u32 [n_rows + 1]; info: u32 [n_rows]) |
+// row-start flags u32 [waves * 64] | entry counter u32 | queue base u32 |
+// 2 spare u32 | counter sums u32 [8] | generation-0 tally window u32
+// [root_win_rows(T) * T + 1 spare] over the chunk's first rows | staged
+// entries QueueEntry [kChunk]
+// ======================================================================
+__global__ void __launch_bounds__(256) root_kernel(RunArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t bdim = blockDim.x;
     const uint32_t lane = tid & (kWave - 1);
     const uint32_t wave = tid / kWave;
+    const uint32_t waves = bdim / kWave;
     const uint32_t T = a.n_types;
     const uint32_t kdim = a.kdim;
-    const uint32_t win_len = a.win_rows * kdim * T;
     const uint32_t lds_rows = a.rows_in_lds ? a.n_rows : 0u;
-    dev::Coeffs *k_lds = (dev::Coeffs *)lds;
-    double *t_cum = (double *)(lds + (sizeof(dev::Coeffs) + 15) / 16 * 16);
+    const uint32_t rwin_rows = root_win_rows(T);
+    const uint32_t rwin_len = rwin_rows * T;
+    double *t_cum = (double *)lds;
     double *t_mean = t_cum + T;
     double *t_sd = t_mean + T;
     uint32_t *t_file = (uint32_t *)(t_sd + T);
-    uint32_t *r_start = (uint32_t *)((unsigned char *)t_cum + ((size_t)T * 28 + 15) / 16 * 16);
+    uint32_t *r_start = (uint32_t *)(lds + ((size_t)T * 28 + 15) / 16 * 16);
     uint32_t *r_info = r_start + lds_rows + 1;
-    unsigned int *win = (unsigned int *)((unsigned char *)r_start + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16);
-    double *stack = (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
-    RootSlot *ring_all = (RootSlot *)(stack + (size_t)a.stack_levels * bdim);
-    RootSlot *ring = ring_all + wave * (kRing + 1);
-    uint32_t *flags = (uint32_t *)(ring_all + (bdim / kWave) * (kRing + 1)) + wave * kWave;
+    uint32_t *flags_all = (uint32_t *)((unsigned char *)r_start + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16);
+    uint32_t *flags = flags_all + wave * kWave;
+    uint32_t *n_entries = flags_all + waves * kWave;
+    uint32_t *q_base = n_entries + 1;
+    uint32_t *red = n_entries + 4;    // [kNumCounters] workgroup counter sums
+    uint32_t *rwin = n_entries + 12;
+    QueueEntry *stage = (QueueEntry *)((unsigned char *)rwin + ((size_t)rwin_len * 4 + 4 + 15) / 16 * 16);
 
-    const uint32_t seg0 = blockIdx.x;
-    const uint32_t n_seg = gridDim.x;
-    const uint32_t row_lo = a.seg_row[seg0];  // first row of the tally window
-
-    // first claim of the wavefront's own segment, issued before the set-up
-    // so its latency overlaps it (lane 0 holds the result)
-    unsigned long long pf_raw = 0;
-    if (lane == 0)
-        pf_raw = atomicAdd(&a.seg_next[seg0], (unsigned long long)a.claim);
-    const uint64_t own_begin = uni64(a.seg_begin[seg0]);
-    const uint64_t own_len = uni64(a.seg_begin[seg0 + 1]) - own_begin;
-    if (tid == 0)
-        *k_lds = dev::coeffs();
     for (uint32_t i = tid; i < T; i += bdim) {
         t_cum[i] = a.type_cum[i];
         t_mean[i] = a.type_mean[i];
         t_sd[i] = a.type_sd[i];
         t_file[i] = a.type_file[i];
     }
-    // row tables: first cell and generation cap of every nonzero row
     for (uint32_t i = tid; i <= lds_rows; i += bdim)
         r_start[i] = a.row_start[i];
     for (uint32_t i = tid; i < lds_rows; i += bdim)
         r_info[i] = a.row_info[i];
-    for (uint32_t i = tid; i <= win_len; i += bdim)
-        win[i] = 0u;  // win[win_len] absorbs the adds of lanes outside the window
-    __syncthreads();
+    if (tid < kNumCounters)
+        red[tid] = 0u;
+    const uint32_t *rs = lds_rows ? r_start : a.row_start;
+    const uint32_t *ri = lds_rows ? r_info : a.row_info;
+    const dev::Coeffs K = dev::coeffs();
+    Counts n = {0, 0, 0, 0, 0};
 
-    // the stream's FP64 coefficients, held in vector registers for the whole
-    // kernel (loaded from LDS, so they are not re-materialised as scalar literals)
-    const dev::Coeffs K = *k_lds;
+    for (uint32_t ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
+        const uint32_t g = a.chunk_seg[ch];
+        const uint64_t sb = a.seg_begin[g];
+        const uint64_t lo = uni64(sb + (uint64_t)(ch - a.chunk_first[g]) * kChunk);
+        const uint64_t hi = uni64(min(lo + kChunk, a.seg_begin[g + 1]));
+        const uint32_t quarter = (kChunk + waves - 1) / waves;
+        const uint64_t wlo = uni64(min(lo + (uint64_t)wave * quarter, hi));
+        const uint64_t whi = uni64(min(wlo + quarter, hi));
+        if (tid == 0)
+            *n_entries = 0u;
+        for (uint32_t i = tid; i <= rwin_len; i += bdim)
+            rwin[i] = 0u;  // rwin[rwin_len] absorbs the adds of lanes outside the window
+        __syncthreads();
 
-    // wavefront-uniform queue state (scalar registers)
-    uint64_t c_cur = 0, c_end = 0;
-    uint32_t row_cur = 0;
-    uint32_t victim = 0;      // 0: own segment; 1: own segment exhausted, steal
-    uint32_t roots_done = 0;
-    uint32_t rb_head = 0, rb_count = 0;
+        const uint32_t row_lo = uni(find_row_wave(rs, 0, a.n_rows, lo, lane));  // first row of the window
+        uint32_t row_cur = wlo < whi ? uni(find_row_wave(rs, row_lo, a.n_rows, wlo, lane)) : 0u;
 
-    Lane L;
-    L.cell = L.node = L.pend = L.tb_glob = 0;
-    L.s = L.mean = L.sd = L.tl = L.tr = 0.0;
-    L.k = L.kcap = 0;
-    L.redo = 0;
-    L.att = 1;
-    L.tb_lds = kNoWindow;
-    L.active = false;
-    Counts n = {0, 0, 0, 0, 0, 0, 0};
-    const uint32_t idle_limit = a.idle_limit;
-
-    for (;;) {
-        // ======== refill: initial cells, buffered roots, stealing ========
-        uint64_t idle = __ballot(!L.active);
-        uint32_t n_idle = uni((uint32_t)__popcll(idle));
-        while (!roots_done && rb_count < n_idle) {
-            // ---- claim a range of initial cells if the current one is used up ----
-            while (c_cur >= c_end && !roots_done) {
-                if (!victim) {
-                    // own segment: take the claim issued ahead of time (its
-                    // latency overlapped the previous range) and issue the next
-                    const uint64_t sb = own_begin;
-                    const uint64_t len = own_len;
-                    const uint64_t base = uni64(__shfl(pf_raw, 0));
-                    if (base < len) {
-                        c_cur = uni64(sb + base);
-                        c_end = uni64(sb + (base + a.claim < len ? base + a.claim : len));
-                        if (lane == 0)
-                            pf_raw = atomicAdd(&a.seg_next[seg0], (unsigned long long)a.claim);
-                        if (lds_rows)
-                            row_cur = uni(find_row_wave(r_start, 0, a.n_rows, c_cur, lane));
-                        else
-                            row_cur = uni(find_row_wave(a.row_start, 0, a.n_rows, c_cur, lane));
-                    } else {
-                        victim = 1;  // own segment exhausted: steal
-                    }
-                    continue;
-                }
-                // the segment with the most unclaimed cells, over all
-                // segments in one pass: lane i reads segments i, i + 64, ...,
-                // eight independent loads at a time, so the scan costs about
-                // one memory round trip; none left anywhere ends the roots
-                uint64_t left = 0;
-                uint32_t best = 0;
-                for (uint32_t base = 0; base < n_seg; base += kWave * kScanBatch) {
-                    uint64_t lens[kScanBatch], used[kScanBatch];
-#pragma unroll
-                    for (uint32_t j = 0; j < kScanBatch; ++j) {
-                        const uint32_t sg = base + j * kWave + lane;
-                        const uint32_t sc = sg < n_seg ? sg : n_seg - 1;
-                        lens[j] = sg < n_seg ? a.seg_begin[sc + 1] - a.seg_begin[sc] : 0;
-                        used[j] = __atomic_load_n(&a.seg_next[sc], __ATOMIC_RELAXED);
-                    }
-#pragma unroll
-                    for (uint32_t j = 0; j < kScanBatch; ++j) {
-                        const uint64_t l = used[j] < lens[j] ? lens[j] - used[j] : 0;
-                        const uint32_t sg = base + j * kWave + lane;
-                        if (l > left) {
-                            left = l;
-                            best = sg;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    const uint64_t ol = __shfl_xor(left, off);
-                    const uint32_t ob = __shfl_xor(best, off);
-                    if (ol > left || (ol == left && ob < best)) {
-                        left = ol;
-                        best = ob;
-                    }
-                }
-                if (uni64(left) == 0) {
-                    roots_done = 1;
-                    continue;
-                }
-                const uint32_t sg = uni(best);
-                const uint64_t sb = a.seg_begin[sg];
-                const uint64_t len = a.seg_begin[sg + 1] - sb;
-                unsigned long long sbase = ~0ull;
-                if (lane == 0)
-                    sbase = atomicAdd(&a.seg_next[sg], (unsigned long long)a.steal_claim);
-                const uint64_t base = uni64(__shfl(sbase, 0));
-                if (base < len) {
-                    c_cur = uni64(sb + base);
-                    c_end = uni64(sb + (base + a.steal_claim < len ? base + a.steal_claim : len));
-                    if (lds_rows)
-                        row_cur = uni(find_row_wave(r_start, 0, a.n_rows, c_cur, lane));
-                    else
-                        row_cur = uni(find_row_wave(a.row_start, 0, a.n_rows, c_cur, lane));
-                }
-            }
-            if (c_cur >= c_end)
-                break;
-            // ---- root step: one initial cell per lane (cell.cu:25-143) ----
-            // branch-free apart from rare paths behind wavefront-wide checks
-            const uint32_t n_valid = uni(c_end - c_cur < kWave ? (uint32_t)(c_end - c_cur) : kWave);
+        for (uint64_t c_cur = wlo; c_cur < whi; c_cur += kWave) {
+            const uint32_t n_valid = uni(whi - c_cur < kWave ? (uint32_t)(whi - c_cur) : kWave);
             const uint64_t c = c_cur + lane;
             const bool valid = lane < n_valid;
-            uint32_t row, info;
-            if (lds_rows) {
-                row = step_row(r_start, a.n_rows, row_cur, c_cur, n_valid, lane, flags);
-                info = r_info[valid ? row : row_cur];
-            } else {
-                row = step_row(a.row_start, a.n_rows, row_cur, c_cur, n_valid, lane, flags);
-                info = a.row_info[valid ? row : row_cur];
-            }
+            const uint32_t row = step_row(rs, a.n_rows, row_cur, c_cur, n_valid, lane, flags);
+            const uint32_t info = ri[valid ? row : row_cur];
             const uint32_t kc = info & 0xFFu;
             const bool ge_phi = (info >> 8) & 1u;
             const Words w = dev::draw_block(a.seed, c, 0, 0);
@@ -417,53 +319,186 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             const bool rec_ok = record && ge_phi;
             const bool divides = valid && !record && kc != 0;
             n.cells += valid ? 1u : 0u;
-            n.rec += rec_ok ? 1u : 0u;
             n.lost += record && !ge_phi ? 1u : 0u;
-            n.drop += valid && !record && kc == 0 ? 1u : 0u;
             n.pool += divides ? 1u : 0u;
+            // generation-0 tallies: the LDS window, else one device atomic per
+            // distinct (row, type) of the step
             const uint32_t tf = t_file[jt];
             const uint32_t woff = row - row_lo;
-            const bool rec_win = rec_ok && woff < a.win_rows;
-            atomicAdd(&win[rec_win ? woff * kdim * T + tf : win_len], rec_win ? 1u : 0u);
-            if (__builtin_expect(__ballot(rec_ok && !rec_win) != 0, 0)) {
-                if (rec_ok && !rec_win)
-                    atomicAdd(&a.tally[(uint64_t)row * kdim * T + tf], 1ull);
+            const bool in_win = woff < rwin_rows;
+            atomicAdd(&rwin[rec_ok && in_win ? woff * T + tf : rwin_len], rec_ok && in_win ? 1u : 0u);
+            uint64_t pend = __ballot(rec_ok && !in_win);
+            while (pend) {
+                const uint32_t leader = (uint32_t)__builtin_ctzll(pend);
+                const uint32_t row_l = uni(__shfl(row, (int)leader));
+                const uint32_t tf_l = uni(__shfl(tf, (int)leader));
+                const uint64_t same = __ballot(rec_ok && !in_win && row == row_l && tf == tf_l);
+                if (lane == leader)
+                    atomicAdd(&a.tally[(uint64_t)row_l * kdim * T + tf_l], (unsigned long long)__popcll(same));
+                pend &= ~same;
             }
-            // dividing roots go to the ring; the others write the spare slot
+            // dividing cells: staged in LDS, appended to the queue at the end
             const uint64_t dmask = __ballot(divides);
-            RootSlot &slot = ring[divides ? (rb_head + rb_count + lane_rank(dmask)) & (kRing - 1) : kRing];
-            slot.s0 = s0;
-            slot.cell = (uint32_t)c;
-            slot.row_type = row | (jt << 24);
-            rb_count = uni(rb_count + (uint32_t)__popcll(dmask));
+            if (dmask) {
+                uint32_t base = 0;
+                if (lane == 0)
+                    base = atomicAdd(n_entries, (uint32_t)__popcll(dmask));
+                base = uni(__shfl(base, 0));
+                if (divides) {
+                    QueueEntry &e = stage[base + lane_rank(dmask)];
+                    e.s0 = s0;
+                    e.cell = (uint32_t)c;
+                    e.row_type = row | (jt << 24);
+                }
+            }
             row_cur = uni(__shfl(row, (int)(n_valid - 1)));
-            c_cur = uni64(c_cur + n_valid);
         }
-        if (rb_count && n_idle) {
-            // ---- hand buffered roots to idle lanes (branch-free) ----
+        __syncthreads();
+        const uint32_t cnt = *n_entries;
+        if (tid == 0 && cnt)
+            *q_base = (uint32_t)atomicAdd(&a.seg_total[g], (unsigned long long)cnt);
+        for (uint32_t i = tid; i < rwin_len; i += bdim) {
+            const uint32_t r = rwin[i];
+            if (r)
+                atomicAdd(&a.tally[(uint64_t)(row_lo + i / T) * kdim * T + i % T], (unsigned long long)r);
+        }
+        __syncthreads();
+        // the segment's entries are contiguous from its first cell's position
+        const uint64_t qb = sb + *q_base;
+        for (uint32_t i = tid; i < cnt; i += bdim)
+            a.queue[qb + i] = stage[i];
+        __syncthreads();  // stage, counters and window are reused by the next chunk
+    }
+    flush_counts(n, a.counters, lane, red);
+}
+
+// Lane state: the dividing cell a lane currently holds.
+struct Lane {
+    uint64_t cell, node, pend;
+    uint64_t tb_glob;   // device tally index of (row, generation 0, type)
+    double s, mean, sd;
+    double tl, tr;      // daughters' timers of the division in progress
+    uint32_t k, kcap;
+    uint32_t tb_lds;    // window index of (row, generation 0, type), or kNoWindow
+    uint32_t redo;      // bit 0 / bit 1: left / right daughter's timer still to redraw
+    uint32_t att;       // attempt number of that redraw
+    bool active;
+};
+
+// ======================================================================
+// tree_kernel: workgroup g works through segment g's queue entries.
+// LDS: stream coefficients | type tables (mean, sd: double [T]; file: u32
+// [T]) | row info when it fits (u32 [n_rows]) | counter sums u32 [8] | window
+// tally u32 [win_rows * kdim * T + 1 spare] | stack double [stack_levels *
+// blockDim]
+// ======================================================================
+__global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t bdim = blockDim.x;
+    const uint32_t lane = tid & (kWave - 1);
+    const uint32_t wave = tid / kWave;
+    const uint32_t T = a.n_types;
+    const uint32_t kdim = a.kdim;
+    const uint32_t win_len = a.win_rows * kdim * T;
+    const uint32_t lds_rows = a.rows_in_lds ? a.n_rows : 0u;
+    dev::Coeffs *k_lds = (dev::Coeffs *)lds;
+    double *t_mean = (double *)(lds + (sizeof(dev::Coeffs) + 15) / 16 * 16);
+    double *t_sd = t_mean + T;
+    uint32_t *t_file = (uint32_t *)(t_sd + T);
+    uint32_t *r_info = (uint32_t *)((unsigned char *)t_mean + ((size_t)T * 20 + 15) / 16 * 16);
+    uint32_t *red = (uint32_t *)((unsigned char *)r_info + ((size_t)lds_rows * 4 + 15) / 16 * 16);
+    unsigned int *win = red + kNumCounters;
+    double *stack = (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
+
+    const uint32_t seg0 = blockIdx.x;
+    const uint32_t n_seg = gridDim.x;
+    const uint32_t row_lo = a.seg_row[seg0];  // first row of the tally window
+
+    // first claim of the wavefront's own segment, issued before the set-up
+    // so its latency overlaps it (lane 0 holds the result)
+    unsigned long long pf_raw = 0;
+    uint32_t pf_size = a.claim;   // size of the claim issued ahead of time
+    uint32_t pf_pending = 1;      // a claim of pf_size is in flight
+    if (lane == 0)
+        pf_raw = atomicAdd(&a.seg_next[seg0], (unsigned long long)pf_size);
+    const uint64_t own_base = uni64(a.seg_begin[seg0]);
+    const uint32_t own_total = uni((uint32_t)a.seg_total[seg0]);
+    if (tid == 0)
+        *k_lds = dev::coeffs();
+    for (uint32_t i = tid; i < T; i += bdim) {
+        t_mean[i] = a.type_mean[i];
+        t_sd[i] = a.type_sd[i];
+        t_file[i] = a.type_file[i];
+    }
+    for (uint32_t i = tid; i < lds_rows; i += bdim)
+        r_info[i] = a.row_info[i];
+    for (uint32_t i = tid; i <= win_len; i += bdim)
+        win[i] = 0u;  // win[win_len] absorbs the adds of lanes outside the window
+    if (tid < kNumCounters)
+        red[tid] = 0u;
+    __syncthreads();
+
+    // the stream's FP64 coefficients, held in vector registers for the whole
+    // kernel (loaded from LDS, so they are not re-materialised as scalar literals)
+    const dev::Coeffs K = *k_lds;
+    const uint32_t *ri = lds_rows ? r_info : a.row_info;
+
+    // wavefront-uniform claim state (scalar registers): queue entries
+    // [q_cur, q_end) are claimed and not yet loaded
+    uint64_t q_cur = 0, q_end = 0;
+    uint32_t victim = 0;      // 0: own segment; 1: own segment exhausted, steal
+    uint32_t n_probe = 0;     // steal attempts so far (varies the probed segments)
+    uint32_t roots_done = 0;
+    // prefetched entries: buffer positions q_head .. q_head + q_have - 1
+    // (mod 64), position p held by lane p
+    uint32_t q_head = 0, q_have = 0;
+    double pf_s0 = 0.0;
+    uint32_t pf_cell = 0, pf_rt = 0;
+
+    Lane L;
+    L.cell = L.node = L.pend = L.tb_glob = 0;
+    L.s = L.mean = L.sd = L.tl = L.tr = 0.0;
+    L.k = L.kcap = 0;
+    L.redo = 0;
+    L.att = 1;
+    L.tb_lds = kNoWindow;
+    L.active = false;
+    Counts n = {0, 0, 0, 0, 0};
+    const uint32_t idle_limit = a.idle_limit;
+
+    for (;;) {
+        // ======== hand buffered entries to idle lanes (branch-free) ========
+        uint64_t idle = __ballot(!L.active);
+        uint32_t n_idle = uni((uint32_t)__popcll(idle));
+        if (q_have && n_idle) {
             const uint32_t rank = lane_rank(idle);
-            const bool take = !L.active && rank < rb_count;
-            const RootSlot e = ring[(rb_head + (take ? rank : 0u)) & (kRing - 1)];
-            const uint32_t row = e.row_type & 0xFFFFFFu;
-            const uint32_t jt = min(e.row_type >> 24, T - 1);
+            const bool take = !L.active && rank < q_have;
+            const uint32_t src = (q_head + rank) & (kWave - 1);
+            const double e_s0 = __shfl(pf_s0, (int)src);
+            const uint32_t e_cell = __shfl(pf_cell, (int)src);
+            const uint32_t e_rt = __shfl(pf_rt, (int)src);
+            const uint32_t row = e_rt & 0xFFFFFFu;
+            const uint32_t jt = min(e_rt >> 24, T - 1);
+            const uint32_t rr = min(row, a.n_rows - 1);
             const uint32_t tf = t_file[jt];
             const uint32_t off = row - row_lo;
-            L.cell = take ? e.cell : L.cell;
-            L.s = take ? e.s0 : L.s;
+            L.cell = take ? e_cell : L.cell;
+            L.s = take ? e_s0 : L.s;
             L.node = take ? 1ull : L.node;
             L.k = take ? 0u : L.k;
             L.pend = take ? 0ull : L.pend;
             L.redo = take ? 0u : L.redo;
-            const uint32_t kcap_e = (lds_rows ? r_info[min(row, a.n_rows - 1)] : a.row_info[min(row, a.n_rows - 1)]) & 0xFFu;
-            L.kcap = take ? kcap_e : L.kcap;
+            L.kcap = take ? (ri[rr] & 0xFFu) : L.kcap;
             L.mean = take ? t_mean[jt] : L.mean;
             L.sd = take ? t_sd[jt] : L.sd;
             L.tb_lds = take ? (off < a.win_rows ? off * kdim * T + tf : kNoWindow) : L.tb_lds;
             L.tb_glob = take ? (uint64_t)row * kdim * T + tf : L.tb_glob;
             L.active = L.active || take;
-            const uint32_t taken = n_idle < rb_count ? n_idle : rb_count;
-            rb_head = uni((rb_head + taken) & (kRing - 1));
-            rb_count = uni(rb_count - taken);
+            const uint32_t taken = n_idle < q_have ? n_idle : q_have;
+            q_head = uni((q_head + taken) & (kWave - 1));
+            q_have = uni(q_have - taken);
         } else if (roots_done && n_idle) {
             // ---- steal pending daughters inside the wavefront ----
             const uint64_t donors = __ballot(L.active && L.pend != 0);
@@ -500,8 +535,98 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
                 }
             }
         }
+        // ======== top up the prefetch buffer from claimed entries ========
+        // (after the hand-out, so the loads complete while the division
+        // steps run and the next hand-out finds its entries in registers)
+        while (!roots_done && q_have < kWave) {
+            // own segment: issue the next claim when the current one is
+            // nearly used up, so its latency overlaps the last entries but
+            // no wavefront holds two claims' worth of unstarted entries;
+            // sized by what the segment has left (guided self-scheduling:
+            // claims shrink towards the end so the wavefronts finish
+            // together)
+            if (!victim && !pf_pending && q_end - q_cur < kClaimAhead) {
+                const uint64_t used = min(uni64(q_end - own_base), (uint64_t)own_total);
+                const uint64_t rest = ((uint64_t)own_total - used) / (2u * (bdim / kWave));
+                pf_size = uni((uint32_t)max<uint64_t>(a.steal_claim, min<uint64_t>(a.claim, rest)));
+                pf_pending = 1;
+                if (lane == 0)
+                    pf_raw = atomicAdd(&a.seg_next[seg0], (unsigned long long)pf_size);
+            }
+            if (q_cur >= q_end) {
+                // ---- claim more entries ----
+                if (!victim) {
+                    // take the claim in flight (waits for it if needed)
+                    const uint64_t base = min(uni64(__shfl(pf_raw, 0)), (uint64_t)own_total);
+                    pf_pending = 0;
+                    if (base < own_total) {
+                        q_cur = uni64(own_base + base);
+                        q_end = uni64(own_base + min(base + pf_size, (uint64_t)own_total));
+                    } else {
+                        victim = 1;  // own segment exhausted: steal
+                    }
+                    continue;
+                }
+                // a victim segment: each lane probes one segment (the 64
+                // probes of a wavefront spread over all segments, shifted
+                // by a hash of the wavefront and the attempt, so concurrent
+                // thieves look at different segments); the probe with the
+                // most entries left wins.  No entries left in any probe ends
+                // the roots: a segment's own workgroup always drains it.
+                const uint32_t h = hash32((seg0 * (bdim / kWave) + wave) * 0x9E3779B9u + n_probe * 0x85EBCA6Bu);
+                n_probe = uni(n_probe + 1);
+                const uint32_t stride = n_seg > kWave ? n_seg / kWave : 1u;
+                const uint32_t cand = (h + lane * stride + (h >> 16) % stride) % n_seg;
+                const uint64_t tot = a.seg_total[cand];
+                const uint64_t used = __hip_atomic_load(&a.seg_next[cand], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t left = used < tot ? tot - used : 0;
+                uint32_t best = cand;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const uint64_t ol = __shfl_xor(left, off);
+                    const uint32_t ob = __shfl_xor(best, off);
+                    if (ol > left || (ol == left && ob < best)) {
+                        left = ol;
+                        best = ob;
+                    }
+                }
+                if (uni64(left) == 0) {
+                    roots_done = 1;
+                    break;
+                }
+                const uint32_t sg = uni(best);
+                // guided: a share of what the victim has left, so steals stay
+                // few while there is much to take and small at the end
+                const uint32_t ssz = uni((uint32_t)max<uint64_t>(
+                    a.steal_claim, min<uint64_t>(a.claim, uni64(left) / (2u * (bdim / kWave)))));
+                unsigned long long sbase = ~0ull;
+                if (lane == 0)
+                    sbase = atomicAdd(&a.seg_next[sg], (unsigned long long)ssz);
+                const uint64_t base = uni64(__shfl(sbase, 0));
+                const uint64_t total = uni64(a.seg_total[sg]);
+                if (base < total) {
+                    const uint64_t sbeg = uni64(a.seg_begin[sg]);
+                    q_cur = uni64(sbeg + base);
+                    q_end = uni64(sbeg + min(base + ssz, total));
+                }
+                continue;
+            }
+            // ---- load claimed entries into the free buffer positions ----
+            const uint32_t nf = uni((uint32_t)min((uint64_t)(kWave - q_have), q_end - q_cur));
+            const uint32_t first = (q_head + q_have) & (kWave - 1);
+            const uint32_t i = (lane - first) & (kWave - 1);
+            if (i < nf) {
+                const QueueEntry &e = a.queue[q_cur + i];
+                pf_s0 = e.s0;
+                pf_cell = e.cell;
+                pf_rt = e.row_type;
+            }
+            q_cur = uni64(q_cur + nf);
+            q_have = uni(q_have + nf);
+        }
+
         if (__ballot(L.active) == 0) {
-            if (roots_done && rb_count == 0)
+            if (roots_done && q_have == 0)
                 break;
             continue;
         }
@@ -556,8 +681,6 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             const uint32_t div_l = (rec_l ^ 1u) & can;
             const uint32_t div_r = (rec_r ^ 1u) & can;
             const uint32_t nrec = cmp * (rec_l + rec_r);
-            n.rec += nrec;
-            n.drop += cmp * ((rec_l | div_l) ^ 1u) + cmp * ((rec_r | div_r) ^ 1u);
             const bool in_win = L.tb_lds != kNoWindow;
             // lanes without a completed division add 0 to the spare slot win[win_len]
             atomicAdd(&win[(cmp && in_win) ? L.tb_lds + k1 * T : win_len], nrec);
@@ -593,23 +716,7 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
         } while (uni((uint32_t)__popcll(__ballot(!L.active))) <= idle_limit);
     }
 
-    const uint32_t n_div = wave_sum(n.div);
-    const uint32_t n_rec = wave_sum(n.rec);
-    const uint32_t n_drop = wave_sum(n.drop);
-    const uint32_t n_rej = wave_sum(n.rej);
-    const uint32_t n_lost = wave_sum(n.lost);
-    const uint32_t n_cells = wave_sum(n.cells);
-    const uint32_t n_pool = wave_sum(n.pool);
-    if (lane == 0) {
-        if (n_div) atomicAdd(&a.counters[kDivisions], (unsigned long long)n_div);
-        if (n_rec) atomicAdd(&a.counters[kRecorded], (unsigned long long)n_rec);
-        if (n_drop) atomicAdd(&a.counters[kDropped], (unsigned long long)n_drop);
-        if (n_rej) atomicAdd(&a.counters[kRejected], (unsigned long long)n_rej);
-        if (n_lost) atomicAdd(&a.counters[kLost], (unsigned long long)n_lost);
-        if (n_cells) atomicAdd(&a.counters[kCells], (unsigned long long)n_cells);
-        if (n_pool) atomicAdd(&a.counters[kPoolCount], (unsigned long long)n_pool);
-    }
-    __syncthreads();
+    flush_counts(n, a.counters, lane, red);
     const uint64_t gbase = (uint64_t)row_lo * kdim * T;
     for (uint32_t i = tid; i < win_len; i += bdim) {
         const uint32_t c = win[i];
@@ -637,17 +744,21 @@ __global__ void debug_draws_kernel(uint64_t seed, const uint64_t *cell, const ui
 
 }  // namespace
 
+size_t root_lds_bytes(uint32_t n_types, uint32_t block_size, uint32_t lds_rows)
+{
+    return ((size_t)n_types * 28 + 15) / 16 * 16 + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16 +
+           (size_t)(block_size / kWave) * kWave * 4 + 48 +
+           ((size_t)root_win_rows(n_types) * n_types * 4 + 4 + 15) / 16 * 16 + (size_t)kChunk * sizeof(QueueEntry);
+}
+
 size_t tree_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32_t stack_levels,
                       uint32_t block_size, uint32_t lds_rows)
 {
-    const size_t tables = (sizeof(dev::Coeffs) + 15) / 16 * 16 + ((size_t)n_types * 28 + 15) / 16 * 16 +
-                          ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16;
+    const size_t tables = (sizeof(dev::Coeffs) + 15) / 16 * 16 + ((size_t)n_types * 20 + 15) / 16 * 16 +
+                          ((size_t)lds_rows * 4 + 15) / 16 * 16 + kNumCounters * 4;
     const size_t win = ((size_t)win_rows * kdim * n_types * 4 + 4 + 15) / 16 * 16;
     const size_t stk = (size_t)stack_levels * block_size * sizeof(double);
-    const size_t waves = block_size / kWave;
-    const size_t ring = waves * (kRing + 1) * sizeof(RootSlot);
-    const size_t flg = waves * kWave * sizeof(uint32_t);
-    return tables + win + stk + ring + flg;
+    return tables + win + stk;
 }
 
 hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t st)
@@ -658,6 +769,15 @@ hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t st)
     if (blocks > 4096)
         blocks = 4096;
     hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_root(const RunArgs &a, const LaunchShape &ls, hipStream_t st)
+{
+    if (a.n_cells == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(root_kernel, dim3((unsigned)ls.root_blocks), dim3(ls.root_block_size),
+                       ls.root_lds_bytes, st, a);
     return hipGetLastError();
 }
 
@@ -683,6 +803,11 @@ hipError_t launch_debug_draws(uint64_t seed, const uint64_t *cell, const uint64_
 hipError_t tree_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu)
 {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tree_kernel, block_size, lds_bytes);
+}
+
+hipError_t root_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, root_kernel, block_size, lds_bytes);
 }
 
 }  // namespace procell

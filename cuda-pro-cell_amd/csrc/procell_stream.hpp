@@ -44,8 +44,9 @@ __device__ __forceinline__ Words philox4x32_10(uint32_t c0, uint32_t c1, uint32_
     for (int round = 0; round < 10; ++round) {
         const uint64_t prod_a = (uint64_t)c0 * 0xD2511F53u;
         const uint64_t prod_b = (uint64_t)c2 * 0xCD9E8D57u;
-        const uint32_t n0 = (uint32_t)(prod_b >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(prod_a >> 32) ^ c3 ^ k1;
+        // three-input XORs in one instruction each (truth table 0x96)
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(prod_b >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(prod_a >> 32), c3, k1, 0x96);
         c1 = (uint32_t)prod_b;
         c3 = (uint32_t)prod_a;
         c0 = n0;
