@@ -1,8 +1,8 @@
 """Benchmark: cell divisions per second of the plain proliferation method on
 one MI355X.
 
-A step is one complete simulation (procell_plan_run: clear the tallies, run
-the persistent division-tree kernel) of the 1024-bin, ~1e7-cell histogram
+A step is one complete simulation (procell_plan_run: clear the tallies, the
+root kernel, the persistent division-tree kernel) of the 1024-bin, ~1e7-cell histogram
 with five cell types (four proliferating, one quiescent), t_max 240, phi 50
 and per-type tracking.  Inputs are uploaded once before timing (resident in
 HBM); every step uses a different seed.  value = divisions summed over the
@@ -31,8 +31,8 @@ METRIC = "cell divisions/s on 1 GPU (plain) + calib grad-steps/s batched vs plai
 LANE_OPS_PER_DIVISION = 180
 # MI355X VALU issue peak: 256 CUs x 64 lanes/clk x 2.4 GHz (lane-ops/s).
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "tree_kernel_hbm_traffic.txt")
-COUNTER_FILE = os.path.join(REPO, "profiles", "tree_kernel_valu_counters.txt")
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "simulation_kernels_hbm_traffic.txt")
+COUNTER_FILE = os.path.join(REPO, "profiles", "simulation_kernels_valu_counters.txt")
 
 
 def log(*a):
@@ -40,7 +40,8 @@ def log(*a):
 
 
 def traffic_from_profile():
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    """HBM bytes per run (root + tree kernel) from the committed rocprofv3
+    --pmc summary."""
     try:
         with open(TRAFFIC_FILE) as f:
             m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
@@ -217,7 +218,7 @@ def main():
             "unit": "Tlane-op/s",
             "frac": achieved / VALU_PEAK_LANE_OPS,
             "traffic": traffic,
-            "kernel": "tree_kernel",
+            "kernel": "root_kernel + tree_kernel (back to back, timed together)",
             "kernel_ms": kernel_ms,
             "work_per_unit": f"{LANE_OPS_PER_DIVISION} lane-ops per division",
             "peak_source": "MI355X FP64 vector peak 78.6 TFLOP/s / 2 = one wave64 VALU instruction "

@@ -362,7 +362,7 @@ int create_plan(const procell_params *p, procell_plan **out)
     for (;;) {
         const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
         int occ_v = 0;
-        if (tree_occupancy(bs, base, &occ_v) != hipSuccess)
+        if (tree_occupancy(bs, base, lds_rows != 0, kdim, &occ_v) != hipSuccess)
             return fail(PROCELL_E_HIP);
         if (occ_v >= 1) {
             occ = occ_v;
@@ -406,7 +406,7 @@ int create_plan(const procell_params *p, procell_plan **out)
     for (;;) {
         ls.tree_lds_bytes = tree_lds_bytes(win_rows, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
         int occ_w = 0;
-        if (tree_occupancy(bs, ls.tree_lds_bytes, &occ_w) != hipSuccess)
+        if (tree_occupancy(bs, ls.tree_lds_bytes, lds_rows != 0, kdim, &occ_w) != hipSuccess)
             return fail(PROCELL_E_HIP);
         // keep a margin for the per-workgroup LDS allocation granularity
         const bool fits = (size_t)occ * align_up(ls.tree_lds_bytes, 2048) <= lds_per_cu;
@@ -455,7 +455,7 @@ int create_plan(const procell_params *p, procell_plan **out)
     a.n_chunks = (uint32_t)chunk_seg.size();
     {
         int occ_r = 0;
-        if (root_occupancy(256, ls.root_lds_bytes, &occ_r) != hipSuccess)
+        if (root_occupancy(256, ls.root_lds_bytes, lds_rows != 0, &occ_r) != hipSuccess)
             return fail(PROCELL_E_HIP);
         const uint64_t resident = (uint64_t)n_cu * (uint64_t)std::max(occ_r, 1);
         ls.root_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>(a.n_chunks, resident));

@@ -117,7 +117,7 @@ hipError_t launch_tree(const RunArgs &a, const LaunchShape &ls, hipStream_t st);
 hipError_t launch_debug_draws(uint64_t seed, const uint64_t *cell, const uint64_t *node,
                               const uint32_t *attempt, uint64_t n, uint32_t *words, double *z,
                               hipStream_t st);
-hipError_t tree_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu);
-hipError_t root_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu);
+hipError_t tree_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, uint32_t kdim, int *blocks_per_cu);
+hipError_t root_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, int *blocks_per_cu);
 
 }  // namespace procell

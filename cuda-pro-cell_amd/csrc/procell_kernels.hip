@@ -17,8 +17,9 @@
 //    with one atomic, so each segment's entries are contiguous.
 //  * tree_kernel (proliferation.cu:317-381).  Persistent workgroups, one per
 //    segment.  A wavefront claims entries of its workgroup's segment with one
-//    atomic per claim (the next claim issued ahead of time) and keeps the
-//    next 64 entries in registers, one per lane, loaded ahead of use; when
+//    atomic per claim (the next claim issued ahead of time) and keeps up to
+//    128 entries in two register sets, one entry per lane and set, loaded
+//    ahead of use; when
 //    the segment is exhausted it claims from the segment with the most
 //    unclaimed entries (work stealing across workgroups), found by one
 //    parallel scan of all cursors.  Each busy lane divides its current cell
@@ -153,6 +154,21 @@ __device__ __forceinline__ uint32_t step_row(const uint32_t *row_start, uint32_t
 // device: the host takes recorded = the sum of the tally and dropped = cells
 // + divisions - recorded - lost (every division adds one cell, and every
 // cell ends recorded, dropped or lost).
+// LDS copies of the stream's lookup tables (procell_math_tables.inc).
+constexpr size_t kMathTableBytes = (128 + 256) * 16;
+
+__device__ __forceinline__ void copy_tables(double (*m_log)[2], double (*m_sc)[2], uint32_t tid, uint32_t bdim)
+{
+    for (uint32_t i = tid; i < 128; i += bdim) {
+        m_log[i][0] = dev::procell_log_table[i][0];
+        m_log[i][1] = dev::procell_log_table[i][1];
+    }
+    for (uint32_t i = tid; i < 256; i += bdim) {
+        m_sc[i][0] = dev::procell_sincos_table[i][0];
+        m_sc[i][1] = dev::procell_sincos_table[i][1];
+    }
+}
+
 struct Counts {
     uint32_t div, rej, lost, cells, pool;
 };
@@ -194,13 +210,15 @@ __device__ __forceinline__ void flush_counts(const Counts &n, unsigned long long
 // chunk_seg[c], kChunk cells (fewer at the segment's end); wavefront w takes
 // the w-th quarter of a chunk.  Chunks have equal cost, so the static
 // round-robin balances.
-// LDS: type tables (cum, mean, sd: double [T]; file: u32 [T]) | row tables
-// when they fit (first cell: u32 [n_rows + 1]; info: u32 [n_rows]) |
-// row-start flags u32 [waves * 64] | entry counter u32 | queue base u32 |
+// LDS: log and sin/cos tables | type tables (cum, mean, sd: double [T]; file:
+// u32 [T]) | row tables when they fit (first cell: u32 [n_rows + 1]; info:
+// u32 [n_rows]) | row-start flags u32 [waves * 64] | entry counter u32 |
+// queue base u32 |
 // 2 spare u32 | counter sums u32 [8] | generation-0 tally window u32
 // [root_win_rows(T) * T + 1 spare] over the chunk's first rows | staged
 // entries QueueEntry [kChunk]
 // ======================================================================
+template <bool RowsLds>
 __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -211,14 +229,16 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
     const uint32_t waves = bdim / kWave;
     const uint32_t T = a.n_types;
     const uint32_t kdim = a.kdim;
-    const uint32_t lds_rows = a.rows_in_lds ? a.n_rows : 0u;
+    const uint32_t lds_rows = RowsLds ? a.n_rows : 0u;
     const uint32_t rwin_rows = root_win_rows(T);
     const uint32_t rwin_len = rwin_rows * T;
-    double *t_cum = (double *)lds;
+    double(*m_log)[2] = (double(*)[2])lds;            // [128] log table
+    double(*m_sc)[2] = (double(*)[2])(lds + 128 * 16);  // [256] sin/cos table
+    double *t_cum = (double *)(lds + kMathTableBytes);
     double *t_mean = t_cum + T;
     double *t_sd = t_mean + T;
     uint32_t *t_file = (uint32_t *)(t_sd + T);
-    uint32_t *r_start = (uint32_t *)(lds + ((size_t)T * 28 + 15) / 16 * 16);
+    uint32_t *r_start = (uint32_t *)(lds + kMathTableBytes + ((size_t)T * 28 + 15) / 16 * 16);
     uint32_t *r_info = r_start + lds_rows + 1;
     uint32_t *flags_all = (uint32_t *)((unsigned char *)r_start + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16);
     uint32_t *flags = flags_all + wave * kWave;
@@ -228,6 +248,7 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
     uint32_t *rwin = n_entries + 12;
     QueueEntry *stage = (QueueEntry *)((unsigned char *)rwin + ((size_t)rwin_len * 4 + 4 + 15) / 16 * 16);
 
+    copy_tables(m_log, m_sc, tid, bdim);
     for (uint32_t i = tid; i < T; i += bdim) {
         t_cum[i] = a.type_cum[i];
         t_mean[i] = a.type_mean[i];
@@ -240,9 +261,11 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
         r_info[i] = a.row_info[i];
     if (tid < kNumCounters)
         red[tid] = 0u;
-    const uint32_t *rs = lds_rows ? r_start : a.row_start;
-    const uint32_t *ri = lds_rows ? r_info : a.row_info;
+    // the row tables' address space is known at compile time (no flat loads)
+    const uint32_t *rs = RowsLds ? r_start : a.row_start;
+    const uint32_t *ri = RowsLds ? r_info : a.row_info;
     const dev::Coeffs K = dev::coeffs();
+    const dev::Tables tb{m_log, m_sc};
     Counts n = {0, 0, 0, 0, 0};
 
     for (uint32_t ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
@@ -273,22 +296,30 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             const Words w = dev::draw_block(a.seed, c, 0, 0);
             const double ut = dev::uniform_co(w.w0, w.w1);
             const double U = dev::uniform_oc(w.w2, w.w3);
-            // first type with ut < cumulative proportion (cell.cu:90-101)
-            uint32_t jt = 0;
+            // first type with ut < cumulative proportion (cell.cu:90-101);
+            // the cumulative proportions are uniform, read eight at a time
+            // through the scalar cache so their latencies overlap
+            uint32_t jt = T - 1;
             bool found = false;
-            for (uint32_t i = 0; i + 1 < T; ++i) {
-                const bool hit = !found && ut < t_cum[i];
-                jt = hit ? i : jt;
-                found = found || hit;
+            for (uint32_t i0 = 0; i0 + 1 < T; i0 += 8) {
+                double cv[8];
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k)
+                    cv[k] = i0 + k + 1 < T ? a.type_cum[i0 + k] : 0.0;
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k) {
+                    const bool hit = !found && i0 + k + 1 < T && ut < cv[k];
+                    jt = hit ? i0 + k : jt;
+                    found = found || hit;
+                }
             }
-            jt = found ? jt : T - 1;
             const double m = t_mean[jt];
             const double sdv = t_sd[jt];
             const bool quiescent = m < 0.0;
             // first timer: stream of node 1 from attempt 1 (cell.cu:106-122)
             const Words w1 = dev::draw_block(a.seed, c, 1, 1);
             double z0, z1;
-            dev::normal_pair(w1, z0, z1, K);
+            dev::normal_pair(w1, z0, z1, K, tb);
             const double c0 = fma(z0, sdv, m);
             const double c1 = fma(z1, sdv, m);
             const bool live = valid && !quiescent;
@@ -301,7 +332,7 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
                     for (uint32_t att = 2; att < 65536u; ++att) {
                         const Words wr = dev::draw_block(a.seed, c, 1, att);
                         double y0, y1;
-                        dev::normal_pair(wr, y0, y1, K);
+                        dev::normal_pair(wr, y0, y1, K, tb);
                         const double d0 = fma(y0, sdv, m);
                         if (d0 > 0.0) { tau0 = d0; break; }
                         ++n.rej;
@@ -372,9 +403,19 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
     flush_counts(n, a.counters, lane, red);
 }
 
-// Lane state: the dividing cell a lane currently holds.
+// Highest / lowest set bit of a nonzero 32- or 64-bit mask.
+__device__ __forceinline__ uint32_t top_bit(uint32_t m) { return 31u - (uint32_t)__builtin_clz(m); }
+__device__ __forceinline__ uint32_t top_bit(uint64_t m) { return 63u - (uint32_t)__builtin_clzll(m); }
+__device__ __forceinline__ uint32_t low_bit(uint32_t m) { return (uint32_t)__builtin_ctz(m); }
+__device__ __forceinline__ uint32_t low_bit(uint64_t m) { return (uint32_t)__builtin_ctzll(m); }
+
+// Lane state: the dividing cell a lane currently holds.  N holds a node's
+// heap index and the pending-level mask: 32 bits when no tree is deeper than
+// 31 generations, else 64.
+template <typename N>
 struct Lane {
-    uint64_t cell, node, pend;
+    uint64_t cell;
+    N node, pend;
     uint64_t tb_glob;   // device tally index of (row, generation 0, type)
     double s, mean, sd;
     double tl, tr;      // daughters' timers of the division in progress
@@ -387,12 +428,14 @@ struct Lane {
 
 // ======================================================================
 // tree_kernel: workgroup g works through segment g's queue entries.
-// LDS: stream coefficients | type tables (mean, sd: double [T]; file: u32
+// LDS: log and sin/cos tables | stream coefficients | type tables (mean, sd:
+// double [T]; file: u32
 // [T]) | row info when it fits (u32 [n_rows]) | counter sums u32 [8] | window
 // tally u32 [win_rows * kdim * T + 1 spare] | stack double [stack_levels *
 // blockDim]
 // ======================================================================
-__global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
+template <bool RowsLds, typename N>
+__global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t tid = threadIdx.x;
@@ -402,9 +445,11 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
     const uint32_t T = a.n_types;
     const uint32_t kdim = a.kdim;
     const uint32_t win_len = a.win_rows * kdim * T;
-    const uint32_t lds_rows = a.rows_in_lds ? a.n_rows : 0u;
-    dev::Coeffs *k_lds = (dev::Coeffs *)lds;
-    double *t_mean = (double *)(lds + (sizeof(dev::Coeffs) + 15) / 16 * 16);
+    const uint32_t lds_rows = RowsLds ? a.n_rows : 0u;
+    double(*m_log)[2] = (double(*)[2])lds;            // [128] log table
+    double(*m_sc)[2] = (double(*)[2])(lds + 128 * 16);  // [256] sin/cos table
+    dev::Coeffs *k_lds = (dev::Coeffs *)(lds + kMathTableBytes);
+    double *t_mean = (double *)(lds + kMathTableBytes + (sizeof(dev::Coeffs) + 15) / 16 * 16);
     double *t_sd = t_mean + T;
     uint32_t *t_file = (uint32_t *)(t_sd + T);
     uint32_t *r_info = (uint32_t *)((unsigned char *)t_mean + ((size_t)T * 20 + 15) / 16 * 16);
@@ -427,6 +472,7 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
     const uint32_t own_total = uni((uint32_t)a.seg_total[seg0]);
     if (tid == 0)
         *k_lds = dev::coeffs();
+    copy_tables(m_log, m_sc, tid, bdim);
     for (uint32_t i = tid; i < T; i += bdim) {
         t_mean[i] = a.type_mean[i];
         t_sd[i] = a.type_sd[i];
@@ -443,7 +489,8 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
     // the stream's FP64 coefficients, held in vector registers for the whole
     // kernel (loaded from LDS, so they are not re-materialised as scalar literals)
     const dev::Coeffs K = *k_lds;
-    const uint32_t *ri = lds_rows ? r_info : a.row_info;
+    const dev::Tables tb{m_log, m_sc};
+    const uint32_t *ri = RowsLds ? r_info : a.row_info;
 
     // wavefront-uniform claim state (scalar registers): queue entries
     // [q_cur, q_end) are claimed and not yet loaded
@@ -451,14 +498,18 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
     uint32_t victim = 0;      // 0: own segment; 1: own segment exhausted, steal
     uint32_t n_probe = 0;     // steal attempts so far (varies the probed segments)
     uint32_t roots_done = 0;
-    // prefetched entries: buffer positions q_head .. q_head + q_have - 1
-    // (mod 64), position p held by lane p
-    uint32_t q_head = 0, q_have = 0;
-    double pf_s0 = 0.0;
-    uint32_t pf_cell = 0, pf_rt = 0;
+    // two prefetch sets of up to 64 entries, entry i of a set held by lane
+    // i: hand-outs read the current set (entries pos .. pos + have - 1);
+    // the incoming set (in_have entries) is loaded in one unconditional load
+    // per lane and copied into the current set only when that runs empty,
+    // long after its loads were issued
+    uint32_t pos = 0, have = 0, in_have = 0;
+    double c_s0 = 0.0, i_s0 = 0.0;
+    uint32_t c_cell = 0, c_rt = 0, i_cell = 0, i_rt = 0;
 
-    Lane L;
-    L.cell = L.node = L.pend = L.tb_glob = 0;
+    Lane<N> L;
+    L.cell = L.tb_glob = 0;
+    L.node = L.pend = 0;
     L.s = L.mean = L.sd = L.tl = L.tr = 0.0;
     L.k = L.kcap = 0;
     L.redo = 0;
@@ -470,15 +521,23 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
 
     for (;;) {
         // ======== hand buffered entries to idle lanes (branch-free) ========
+        if (have == 0 && in_have) {
+            c_s0 = i_s0;
+            c_cell = i_cell;
+            c_rt = i_rt;
+            pos = 0;
+            have = in_have;
+            in_have = 0;
+        }
         uint64_t idle = __ballot(!L.active);
         uint32_t n_idle = uni((uint32_t)__popcll(idle));
-        if (q_have && n_idle) {
+        if (have && n_idle) {
             const uint32_t rank = lane_rank(idle);
-            const bool take = !L.active && rank < q_have;
-            const uint32_t src = (q_head + rank) & (kWave - 1);
-            const double e_s0 = __shfl(pf_s0, (int)src);
-            const uint32_t e_cell = __shfl(pf_cell, (int)src);
-            const uint32_t e_rt = __shfl(pf_rt, (int)src);
+            const bool take = !L.active && rank < have;
+            const uint32_t src = min(pos + rank, kWave - 1);
+            const double e_s0 = __shfl(c_s0, (int)src);
+            const uint32_t e_cell = __shfl(c_cell, (int)src);
+            const uint32_t e_rt = __shfl(c_rt, (int)src);
             const uint32_t row = e_rt & 0xFFFFFFu;
             const uint32_t jt = min(e_rt >> 24, T - 1);
             const uint32_t rr = min(row, a.n_rows - 1);
@@ -486,9 +545,9 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             const uint32_t off = row - row_lo;
             L.cell = take ? e_cell : L.cell;
             L.s = take ? e_s0 : L.s;
-            L.node = take ? 1ull : L.node;
+            L.node = take ? (N)1 : L.node;
             L.k = take ? 0u : L.k;
-            L.pend = take ? 0ull : L.pend;
+            L.pend = take ? (N)0 : L.pend;
             L.redo = take ? 0u : L.redo;
             L.kcap = take ? (ri[rr] & 0xFFu) : L.kcap;
             L.mean = take ? t_mean[jt] : L.mean;
@@ -496,20 +555,20 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             L.tb_lds = take ? (off < a.win_rows ? off * kdim * T + tf : kNoWindow) : L.tb_lds;
             L.tb_glob = take ? (uint64_t)row * kdim * T + tf : L.tb_glob;
             L.active = L.active || take;
-            const uint32_t taken = n_idle < q_have ? n_idle : q_have;
-            q_head = uni((q_head + taken) & (kWave - 1));
-            q_have = uni(q_have - taken);
+            const uint32_t taken = n_idle < have ? n_idle : have;
+            pos = uni(pos + taken);
+            have = uni(have - taken);
         } else if (roots_done && n_idle) {
             // ---- steal pending daughters inside the wavefront ----
             const uint64_t donors = __ballot(L.active && L.pend != 0);
             const uint32_t nd = uni(min((uint32_t)__popcll(donors), n_idle));
             if (nd) {
-                const uint32_t my_lvl = L.pend ? (uint32_t)__builtin_ctzll(L.pend) : 0u;
+                const uint32_t my_lvl = L.pend ? low_bit(L.pend) : 0u;
                 const uint32_t r = lane_rank(idle);
                 const bool thief = !L.active && r < nd;
                 const uint32_t src = thief ? nth_set_bit(donors, r) : lane;
                 const uint32_t lvl = __shfl(my_lvl, (int)src);
-                const uint64_t d_node = __shfl(L.node, (int)src);
+                const N d_node = __shfl(L.node, (int)src);
                 const uint32_t d_k = __shfl(L.k, (int)src);
                 const uint64_t d_cell = __shfl(L.cell, (int)src);
                 const uint32_t d_kcap = __shfl(L.kcap, (int)src);
@@ -518,10 +577,10 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
                 const double d_mean = __shfl(L.mean, (int)src);
                 const double d_sd = __shfl(L.sd, (int)src);
                 if (L.active && L.pend && lane_rank(donors) < nd)
-                    L.pend &= ~(1ull << my_lvl);
+                    L.pend &= ~((N)1 << my_lvl);
                 if (thief) {
                     L.s = stack[(size_t)(lvl - 1) * bdim + wave * kWave + src];
-                    L.node = (d_node >> (d_k - lvl)) | 1ull;
+                    L.node = (d_node >> (d_k - lvl)) | (N)1;
                     L.k = lvl;
                     L.cell = d_cell;
                     L.kcap = d_kcap;
@@ -535,10 +594,8 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
                 }
             }
         }
-        // ======== top up the prefetch buffer from claimed entries ========
-        // (after the hand-out, so the loads complete while the division
-        // steps run and the next hand-out finds its entries in registers)
-        while (!roots_done && q_have < kWave) {
+        // ======== fill the incoming prefetch set once it is empty ========
+        while (!roots_done && in_have == 0) {
             // own segment: issue the next claim when the current one is
             // nearly used up, so its latency overlaps the last entries but
             // no wavefront holds two claims' worth of unstarted entries;
@@ -611,22 +668,19 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
                 }
                 continue;
             }
-            // ---- load claimed entries into the free buffer positions ----
-            const uint32_t nf = uni((uint32_t)min((uint64_t)(kWave - q_have), q_end - q_cur));
-            const uint32_t first = (q_head + q_have) & (kWave - 1);
-            const uint32_t i = (lane - first) & (kWave - 1);
-            if (i < nf) {
-                const QueueEntry &e = a.queue[q_cur + i];
-                pf_s0 = e.s0;
-                pf_cell = e.cell;
-                pf_rt = e.row_type;
-            }
+            // ---- load up to 64 claimed entries (lanes past the last one
+            // load it again; their copies are never handed out) ----
+            const uint32_t nf = uni((uint32_t)min((uint64_t)kWave, q_end - q_cur));
+            const QueueEntry e = a.queue[q_cur + min(lane, nf - 1)];
+            i_s0 = e.s0;
+            i_cell = e.cell;
+            i_rt = e.row_type;
             q_cur = uni64(q_cur + nf);
-            q_have = uni(q_have + nf);
+            in_have = nf;
         }
 
         if (__ballot(L.active) == 0) {
-            if (roots_done && q_have == 0)
+            if (roots_done && have == 0 && in_have == 0)
                 break;
             continue;
         }
@@ -645,10 +699,10 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             const uint32_t act = L.active ? 1u : 0u;
             const uint32_t fresh = L.redo == 0 ? 1u : 0u;
             const uint32_t rright = (L.redo & 1u) ? 0u : 1u;   // redraw the right daughter?
-            const uint64_t bnode = fresh ? L.node : (L.node << 1) | (uint64_t)rright;
+            const N bnode = fresh ? L.node : (L.node << 1) | (N)rright;
             const Words w = dev::draw_block(a.seed, L.cell, bnode, fresh ? 0u : L.att);
             double z0, z1;
-            dev::normal_pair(w, z0, z1, K);
+            dev::normal_pair(w, z0, z1, K, tb);
             const double c0 = fma(z0, L.sd, L.mean);
             const double c1 = fma(z1, L.sd, L.mean);
             const uint32_t r0 = c0 > 0.0 ? 0u : 1u;
@@ -695,19 +749,21 @@ __global__ void __launch_bounds__(256) tree_kernel(RunArgs a)
             const uint32_t go_l = cmp & div_l;
             const uint32_t go_r = cmp & (div_l ^ 1u) & div_r;
             const uint32_t pop = cmp & (div_l ^ 1u) & (div_r ^ 1u);
-            const uint64_t pend2 = L.pend | ((uint64_t)(go_l & div_r) << k1);
+            // (k1 stays below the width of N for a lane that pushes; the
+            // mask keeps the shift defined for the others)
+            const N pend2 = L.pend | ((N)(go_l & div_r) << (k1 & (sizeof(N) * 8 - 1)));
             const uint32_t has = pend2 != 0 ? 1u : 0u;
-            const uint32_t lvl = has ? 63u - (uint32_t)__builtin_clzll(pend2 | 1ull) : 1u;
+            const uint32_t lvl = has ? top_bit(pend2 | (N)1) : 1u;
             const double sp = stack[(size_t)(lvl - 1) * bdim + tid];
             // single-level selects only (nested ones turn into branches)
             const uint32_t down = go_l | go_r;
-            const uint64_t child = (L.node << 1) | (uint64_t)go_r;
-            const uint32_t sh = pop ? ((L.k - lvl) & 63u) : 0u;
-            const uint64_t upn = (L.node >> sh) | (uint64_t)pop;  // pending right daughter
+            const N child = (L.node << 1) | (N)go_r;
+            const uint32_t sh = pop ? ((L.k - lvl) & (sizeof(N) * 8 - 1)) : 0u;
+            const N upn = (L.node >> sh) | (N)pop;  // pending right daughter
             const double s_child = go_l ? sl : sr;
             const double s_up = pop ? sp : L.s;
             const uint32_t k_up = pop ? lvl : L.k;
-            const uint64_t clear = (uint64_t)pop << lvl;
+            const N clear = (N)pop << lvl;
             L.node = down ? child : upn;
             L.s = down ? s_child : s_up;
             L.k = down ? k1 : k_up;
@@ -737,7 +793,7 @@ __global__ void debug_draws_kernel(uint64_t seed, const uint64_t *cell, const ui
     words[4 * i + 2] = w.w2;
     words[4 * i + 3] = w.w3;
     double z0, z1;
-    dev::normal_pair(w, z0, z1);
+    dev::normal_pair(w, z0, z1, dev::coeffs(), dev::global_tables());
     z[2 * i] = z0;
     z[2 * i + 1] = z1;
 }
@@ -746,7 +802,7 @@ __global__ void debug_draws_kernel(uint64_t seed, const uint64_t *cell, const ui
 
 size_t root_lds_bytes(uint32_t n_types, uint32_t block_size, uint32_t lds_rows)
 {
-    return ((size_t)n_types * 28 + 15) / 16 * 16 + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16 +
+    return kMathTableBytes + ((size_t)n_types * 28 + 15) / 16 * 16 + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16 +
            (size_t)(block_size / kWave) * kWave * 4 + 48 +
            ((size_t)root_win_rows(n_types) * n_types * 4 + 4 + 15) / 16 * 16 + (size_t)kChunk * sizeof(QueueEntry);
 }
@@ -754,7 +810,7 @@ size_t root_lds_bytes(uint32_t n_types, uint32_t block_size, uint32_t lds_rows)
 size_t tree_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32_t stack_levels,
                       uint32_t block_size, uint32_t lds_rows)
 {
-    const size_t tables = (sizeof(dev::Coeffs) + 15) / 16 * 16 + ((size_t)n_types * 20 + 15) / 16 * 16 +
+    const size_t tables = kMathTableBytes + (sizeof(dev::Coeffs) + 15) / 16 * 16 + ((size_t)n_types * 20 + 15) / 16 * 16 +
                           ((size_t)lds_rows * 4 + 15) / 16 * 16 + kNumCounters * 4;
     const size_t win = ((size_t)win_rows * kdim * n_types * 4 + 4 + 15) / 16 * 16;
     const size_t stk = (size_t)stack_levels * block_size * sizeof(double);
@@ -776,8 +832,12 @@ hipError_t launch_root(const RunArgs &a, const LaunchShape &ls, hipStream_t st)
 {
     if (a.n_cells == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(root_kernel, dim3((unsigned)ls.root_blocks), dim3(ls.root_block_size),
-                       ls.root_lds_bytes, st, a);
+    if (a.rows_in_lds)
+        hipLaunchKernelGGL(root_kernel<true>, dim3((unsigned)ls.root_blocks), dim3(ls.root_block_size),
+                           ls.root_lds_bytes, st, a);
+    else
+        hipLaunchKernelGGL(root_kernel<false>, dim3((unsigned)ls.root_blocks), dim3(ls.root_block_size),
+                           ls.root_lds_bytes, st, a);
     return hipGetLastError();
 }
 
@@ -785,7 +845,16 @@ hipError_t launch_tree(const RunArgs &a, const LaunchShape &ls, hipStream_t st)
 {
     if (a.n_cells == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(tree_kernel, dim3(ls.tree_blocks), dim3(ls.tree_block_size), ls.tree_lds_bytes, st, a);
+    const dim3 g(ls.tree_blocks), b(ls.tree_block_size);
+    const bool narrow = a.kdim <= 32;  // heap indices and level masks fit 32 bits
+    if (a.rows_in_lds && narrow)
+        hipLaunchKernelGGL((tree_kernel<true, uint32_t>), g, b, ls.tree_lds_bytes, st, a);
+    else if (a.rows_in_lds)
+        hipLaunchKernelGGL((tree_kernel<true, uint64_t>), g, b, ls.tree_lds_bytes, st, a);
+    else if (narrow)
+        hipLaunchKernelGGL((tree_kernel<false, uint32_t>), g, b, ls.tree_lds_bytes, st, a);
+    else
+        hipLaunchKernelGGL((tree_kernel<false, uint64_t>), g, b, ls.tree_lds_bytes, st, a);
     return hipGetLastError();
 }
 
@@ -800,14 +869,28 @@ hipError_t launch_debug_draws(uint64_t seed, const uint64_t *cell, const uint64_
     return hipGetLastError();
 }
 
-hipError_t tree_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu)
+hipError_t tree_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, uint32_t kdim, int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tree_kernel, block_size, lds_bytes);
+    const bool narrow = kdim <= 32;
+    if (rows_in_lds && narrow)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tree_kernel<true, uint32_t>, block_size,
+                                                            lds_bytes);
+    if (rows_in_lds)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tree_kernel<true, uint64_t>, block_size,
+                                                            lds_bytes);
+    if (narrow)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tree_kernel<false, uint32_t>, block_size,
+                                                            lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, tree_kernel<false, uint64_t>, block_size,
+                                                        lds_bytes);
 }
 
-hipError_t root_occupancy(int block_size, size_t lds_bytes, int *blocks_per_cu)
+hipError_t root_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, root_kernel, block_size, lds_bytes);
+    return rows_in_lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, root_kernel<true>, block_size,
+                                                                      lds_bytes)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, root_kernel<false>, block_size,
+                                                                      lds_bytes);
 }
 
 }  // namespace procell

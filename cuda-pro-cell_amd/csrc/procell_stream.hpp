@@ -15,16 +15,18 @@
 // (src/simulation/proliferation.cu:330-331, cells_population.cu:108).
 //
 // The FP64 functions below use only +, -, *, fma and integer/bit operations,
-// in a fixed order, and the library is compiled with -ffp-contract=off, so a
-// host build of the same operation sequence reproduces every bit.  Methods:
-//   log   : mantissa folded to [sqrt(1/2), sqrt(2)), log(1+f) = 2 atanh(s),
-//           s = f/(2+f), fdlibm-style rearrangement, degree-10 Taylor
-//           polynomial of atanh in s^2, 1/(2+f) by a quadratic seed and three
-//           Newton steps (no division instruction).
+// in a fixed order, plus two lookup tables (procell_math_tables.inc), and the
+// library is compiled with -ffp-contract=off, so a host build of the same
+// operation sequence reproduces every bit.  Methods:
+//   log   : table-driven (Tang, ACM TOMS 16(4), 1990): x = 2^e m, table
+//           entry {1/c, -log(1/c)} by the top 7 fraction bits (intervals
+//           next to 1 use c = 1), r = fma(m, 1/c, -1), Taylor polynomial of
+//           log(1+r) of degree 7, e ln 2 in two parts.
 //   sqrt  : Lomont's 64-bit inverse-square-root seed, three Newton steps,
 //           one residual correction.
-//   sin/cos of 2 pi u : exact reduction to the nearest quarter turn, Taylor
-//           polynomials of sin(2 pi r) and cos(2 pi r) for |r| <= 1/8.
+//   sin/cos of 2 pi u : nearest 1/256 turn j (exact reduction), table entry
+//           {sin, cos}(2 pi j/256), Taylor polynomials of the remainder,
+//           addition theorems.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -80,64 +82,68 @@ __device__ __forceinline__ double uniform_oc(uint32_t hi, uint32_t lo)  // (0, 1
     return (mantissa53(hi, lo) + 1.0) * 0x1p-53;
 }
 
+#define PROCELL_TABLE_QUAL __constant__
+#include "procell_math_tables.inc"
+#undef PROCELL_TABLE_QUAL
+
 // FP64 coefficients of the stream's log and sin/cos, gathered in one struct so
 // that a kernel can load them once into vector registers instead of
 // re-materialising every 64-bit literal in scalar registers inside its loop
 // (the CU's single scalar unit is shared by all its wavefronts).
 struct Coeffs {
-    double sqrt2;            // mantissa fold threshold of log
-    double rcp[3];           // quadratic seed of 1/(2+f): rcp[0] + rcp[1] d + rcp[2] d^2
-    double at[10];           // 2/(2n+1), n = 1..10
+    double lp[6];            // log: -1/2, 1/3, -1/4, 1/5, -1/6, 1/7
     double ln2_hi, ln2_lo;   // ln 2 split: hi has 32 significant bits
-    double sn[9];            // (-1)^n (2 pi)^(2n+1) / (2n+1)!, n = 0..8
-    double cs[9];            // (-1)^n (2 pi)^(2n) / (2n)!, n = 1..9
+    double two_pi_hi, two_pi_lo;
+    double s1, s2, s3;       // sin t = t + t^3 (s1 + t^2 (s2 + t^2 s3))
+    double c1, c2, c3;       // cos t - 1 = t^2 (c1 + t^2 (c2 + t^2 c3))
 };
 
 __host__ __device__ constexpr Coeffs coeffs()
 {
-    return Coeffs{0x1.6a09e667f3bcdp+0,
-                  {0x1.7aa45399395aap+0, -0x1.72711bb58452fp-1, 0x1.df894e1f2af38p-4},
-                  {0x1.5555555555555p-1, 0x1.999999999999ap-2, 0x1.2492492492492p-2, 0x1.c71c71c71c71cp-3,
-                   0x1.745d1745d1746p-3, 0x1.3b13b13b13b14p-3, 0x1.1111111111111p-3, 0x1.e1e1e1e1e1e1ep-4,
-                   0x1.af286bca1af28p-4, 0x1.8618618618618p-4},
+    return Coeffs{{-0.5, 0x1.5555555555555p-2, -0.25, 0x1.999999999999ap-3, -0x1.5555555555555p-3,
+                   0x1.2492492492492p-3},
                   0x1.62e42fee00000p-1,
                   0x1.a39ef35793c76p-33,
-                  {0x1.921fb54442d18p+2, -0x1.4abbce625be53p+5, 0x1.466bc6775aae2p+6, -0x1.32d2cce62bd86p+6,
-                   0x1.50783487ee782p+5, -0x1.e3074fde8871fp+3, 0x1.e8f434d018d63p+1, -0x1.6fadb9f155744p-1,
-                   0x1.aaec32af93359p-4},
-                  {-0x1.3bd3cc9be45dep+4, 0x1.03c1f081b5ac4p+6, -0x1.55d3c7e3cbffap+6, 0x1.e1f506891babbp+5,
-                   -0x1.a6d1f2a204a8cp+4, 0x1.f9d38a3763cc3p+2, -0x1.b6e24f44b128fp+0, 0x1.20c62c2f2d7f5p-2,
-                   -0x1.2a0c591af8314p-5}};
+                  0x1.921fb54442d18p+2,
+                  0x1.1a62633145c07p-52,
+                  -0x1.5555555555555p-3,
+                  0x1.1111111111111p-7,
+                  -0x1.a01a01a01a01ap-13,
+                  -0.5,
+                  0x1.5555555555555p-5,
+                  -0x1.6c16c16c16c17p-10};
 }
 
+// Lookup tables of the stream's math; LDS copies in the hot kernels.
+struct Tables {
+    const double (*log)[2];     // [128] {1/c, -log(1/c)}
+    const double (*sincos)[2];  // [256] {sin, cos}(2 pi j/256)
+};
+
 // Natural log for positive normal x.
-__device__ __forceinline__ double log_pos(double x, const Coeffs &K)
+__device__ __forceinline__ double log_pos(double x, const Coeffs &K, const Tables &tb)
 {
     const uint64_t bits = __builtin_bit_cast(uint64_t, x);
     int expo = (int)(bits >> 52) - 1023;
+    const uint32_t j = (uint32_t)(bits >> 45) & 127u;
     double m = __builtin_bit_cast(double, (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
-    if (m > K.sqrt2) {
+    if (j >= 64u) {
         m = m * 0.5;
         expo = expo + 1;
     }
-    const double f = m - 1.0;
-    const double d = 2.0 + f;
-    double r = fma(fma(K.rcp[2], d, K.rcp[1]), d, K.rcp[0]);
-    double e;
-    e = fma(-d, r, 1.0); r = fma(r, e, r);
-    e = fma(-d, r, 1.0); r = fma(r, e, r);
-    e = fma(-d, r, 1.0); r = fma(r, e, r);
-    const double s = f * r;
-    const double z = s * s;
-    double p = K.at[9];
+    const double invc = tb.log[j][0];
+    const double logc = tb.log[j][1];
+    const double r = fma(m, invc, -1.0);
+    const double r2 = r * r;
+    double p = K.lp[5];
 #pragma unroll
-    for (int i = 8; i >= 0; --i)
-        p = fma(p, z, K.at[i]);
-    const double R = p * z;
-    const double hfsq = (0.5 * f) * f;
-    const double dk = (double)expo;
-    const double lo_part = fma(s, hfsq + R, dk * K.ln2_lo);
-    return dk * K.ln2_hi - ((hfsq - lo_part) - f);
+    for (int i = 4; i >= 0; --i)
+        p = fma(p, r, K.lp[i]);
+    const double de = (double)expo;
+    const double hi = fma(de, K.ln2_hi, logc);
+    const double lo = de * K.ln2_lo;
+    const double t = fma(r2, p, lo);
+    return hi + (r + t);
 }
 
 // Square root for finite a >= 0.
@@ -158,43 +164,39 @@ __device__ __forceinline__ double sqrt_nonneg(double a)
 
 // sin and cos of 2 pi u with u = uniform_co(hi, lo).
 __device__ __forceinline__ void sincos_turns(uint32_t hi, uint32_t lo, double &sn_out, double &cs_out,
-                                             const Coeffs &K)
+                                             const Coeffs &K, const Tables &tb)
 {
     const double u = uniform_co(hi, lo);
-    const uint32_t q = ((hi >> 29) + 1u) >> 1;   // nearest quarter turn, 0..4
-    const double r = u - (double)q * 0.25;       // exact, |r| <= 1/8
-    const double z = r * r;
-    double ps = K.sn[8];
-#pragma unroll
-    for (int i = 7; i >= 1; --i)
-        ps = fma(ps, z, K.sn[i]);
-    const double sn = fma(r * z, ps, r * K.sn[0]);
-    double pc = K.cs[8];
-#pragma unroll
-    for (int i = 7; i >= 0; --i)
-        pc = fma(pc, z, K.cs[i]);
-    const double cs = fma(z, pc, 1.0);
-    const uint32_t quad = q & 3u;
-    const double a = (quad & 1u) ? cs : sn;   // |sin| candidate
-    const double b = (quad & 1u) ? sn : cs;   // |cos| candidate
-    sn_out = (quad & 2u) ? -a : a;
-    cs_out = ((quad + 1u) & 2u) ? -b : b;
+    const uint32_t j = ((hi >> 23) + 1u) >> 1;     // nearest 1/256 turn, 0..256
+    const double d = u - (double)j * 0x1p-8;       // exact, |d| <= 1/512
+    const double t = fma(d, K.two_pi_hi, d * K.two_pi_lo);
+    const double t2 = t * t;
+    double ps = fma(t2, K.s3, K.s2);
+    ps = fma(t2, ps, K.s1);
+    const double st = fma(t * t2, ps, t);
+    const double cm = t2 * fma(t2, fma(t2, K.c3, K.c2), K.c1);
+    const double sa = tb.sincos[j & 255u][0];
+    const double ca = tb.sincos[j & 255u][1];
+    sn_out = sa + fma(sa, cm, ca * st);
+    cs_out = ca + fma(ca, cm, -(sa * st));
 }
 
 // Box-Muller (Box & Muller 1958) pair from one block.
-__device__ __forceinline__ void normal_pair(const Words &w, double &z0, double &z1, const Coeffs &K)
+__device__ __forceinline__ void normal_pair(const Words &w, double &z0, double &z1, const Coeffs &K,
+                                            const Tables &tb)
 {
     const double u1 = uniform_oc(w.w0, w.w1);
-    const double rho = sqrt_nonneg(-2.0 * log_pos(u1, K));
+    const double rho = sqrt_nonneg(-2.0 * log_pos(u1, K, tb));
     double sn, cs;
-    sincos_turns(w.w2, w.w3, sn, cs, K);
+    sincos_turns(w.w2, w.w3, sn, cs, K, tb);
     z0 = rho * cs;
     z1 = rho * sn;
 }
 
-__device__ __forceinline__ void normal_pair(const Words &w, double &z0, double &z1)
+// Tables read from constant memory (for kernels off the hot path).
+__device__ __forceinline__ Tables global_tables()
 {
-    normal_pair(w, z0, z1, coeffs());
+    return Tables{procell_log_table, procell_sincos_table};
 }
 
 }  // namespace dev

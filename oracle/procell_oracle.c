@@ -93,69 +93,58 @@ double oracle_u01_open_closed(uint32_t hi, uint32_t lo)
     return (m + 1.0) * 0x1p-53;
 }
 
+#define PROCELL_TABLE_QUAL static
+#include "procell_math_tables.inc"
+
 /* ------------------------------------------------------------------ */
-/* FP64 log, division-free.  Method: exponent/mantissa split with the   */
-/* mantissa folded into [sqrt(1/2), sqrt(2)), then                       */
-/* log(1+f) = 2 atanh(s), s = f/(2+f), with the fdlibm-style             */
-/* rearrangement log(1+f) = f - (hfsq - s*(hfsq + R)), hfsq = f^2/2,     */
-/* R = sum_{n=1..10} 2/(2n+1) s^(2n) (Taylor series of 2 atanh(s)/s - 2).*/
-/* 1/(2+f) comes from a quadratic minimax seed (max rel. error 1.4e-3)   */
-/* and three Newton-Raphson steps.  Valid for positive normal x.         */
+/* FP64 log, table driven (the method of Tang, "Table-driven           */
+/* implementation of the logarithm function in IEEE floating-point      */
+/* arithmetic", ACM TOMS 16(4), 1990, in the fma form used by modern     */
+/* libms).  x = 2^e m, m in [1, 2); j = top 7 fraction bits; for j >= 64 */
+/* m is halved and e incremented, so the two table intervals next to 1   */
+/* have c = 1 (exact results near x = 1).  With {invc, logc} from the    */
+/* table (procell_math_tables.inc): r = fma(m, invc, -1), |r| < 2^-7,     */
+/* log x = (e ln2_hi + logc) + (r + (r^2 P(r) + e ln2_lo)) with           */
+/* P(r) = -1/2 + r/3 - r^2/4 + r^3/5 - r^4/6 + r^5/7 (Taylor series of    */
+/* (log(1+r) - r)/r^2).  Valid for positive normal x.                    */
 /* ------------------------------------------------------------------ */
 static const double LN2_HI = 0x1.62e42fee00000p-1;
 static const double LN2_LO = 0x1.a39ef35793c76p-33;
-static const double RCP_P0 = 0x1.7aa45399395aap+0;
-static const double RCP_P1 = -0x1.72711bb58452fp-1;
-static const double RCP_P2 = 0x1.df894e1f2af38p-4;
-static const double AT1 = 0x1.5555555555555p-1;  /* 2/3  */
-static const double AT2 = 0x1.999999999999ap-2;  /* 2/5  */
-static const double AT3 = 0x1.2492492492492p-2;  /* 2/7  */
-static const double AT4 = 0x1.c71c71c71c71cp-3;  /* 2/9  */
-static const double AT5 = 0x1.745d1745d1746p-3;  /* 2/11 */
-static const double AT6 = 0x1.3b13b13b13b14p-3;  /* 2/13 */
-static const double AT7 = 0x1.1111111111111p-3;  /* 2/15 */
-static const double AT8 = 0x1.e1e1e1e1e1e1ep-4;  /* 2/17 */
-static const double AT9 = 0x1.af286bca1af28p-4;  /* 2/19 */
-static const double AT10 = 0x1.8618618618618p-4; /* 2/21 */
+static const double LP1 = -0.5;
+static const double LP2 = 0x1.5555555555555p-2;   /*  1/3 */
+static const double LP3 = -0.25;
+static const double LP4 = 0x1.999999999999ap-3;   /*  1/5 */
+static const double LP5 = -0x1.5555555555555p-3;  /* -1/6 */
+static const double LP6 = 0x1.2492492492492p-3;   /*  1/7 */
 
 double oracle_log(double x)
 {
     uint64_t bits;
     memcpy(&bits, &x, sizeof bits);
     int e = (int)(bits >> 52) - 1023;
+    uint32_t j = (uint32_t)(bits >> 45) & 127u;
     uint64_t mbits = (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
     double m;
     memcpy(&m, &mbits, sizeof m);
-    if (m > 0x1.6a09e667f3bcdp+0) {
+    if (j >= 64u) {
         m = m * 0.5;
         e = e + 1;
     }
-    double f = m - 1.0;
-    double d = 2.0 + f;
-    double r = fma(fma(RCP_P2, d, RCP_P1), d, RCP_P0);
-    double t;
-    t = fma(-d, r, 1.0); r = fma(r, t, r);
-    t = fma(-d, r, 1.0); r = fma(r, t, r);
-    t = fma(-d, r, 1.0); r = fma(r, t, r);
-    double s = f * r;
-    double z = s * s;
-    double p = AT10;
-    p = fma(p, z, AT9);
-    p = fma(p, z, AT8);
-    p = fma(p, z, AT7);
-    p = fma(p, z, AT6);
-    p = fma(p, z, AT5);
-    p = fma(p, z, AT4);
-    p = fma(p, z, AT3);
-    p = fma(p, z, AT2);
-    p = fma(p, z, AT1);
-    double R = p * z;
-    double hfsq = (0.5 * f) * f;
-    double dk = (double)e;
-    double t1 = dk * LN2_LO;
-    double t2 = fma(s, hfsq + R, t1);
-    double t3 = (hfsq - t2) - f;
-    return dk * LN2_HI - t3;
+    double invc = procell_log_table[j][0];
+    double logc = procell_log_table[j][1];
+    double r = fma(m, invc, -1.0);
+    double r2 = r * r;
+    double p = LP6;
+    p = fma(p, r, LP5);
+    p = fma(p, r, LP4);
+    p = fma(p, r, LP3);
+    p = fma(p, r, LP2);
+    p = fma(p, r, LP1);
+    double de = (double)e;
+    double hi = fma(de, LN2_HI, logc);
+    double lo = de * LN2_LO;
+    double t = fma(r2, p, lo);
+    return hi + (r + t);
 }
 
 /* ------------------------------------------------------------------ */
@@ -186,62 +175,38 @@ double oracle_sqrt(double a)
 }
 
 /* ------------------------------------------------------------------ */
-/* sin and cos of 2*pi*u, u = u01_closed_open(hi, lo).  The argument is  */
-/* reduced exactly in turns: q = nearest quarter turn (from the top bits */
-/* of hi), r = u - q/4 in [-1/8, 1/8] (exact), then Taylor polynomials   */
-/* of sin(2 pi r) (9 terms) and cos(2 pi r) (10 terms) in r^2, Horner     */
-/* with fma, and a quadrant swap.                                        */
+/* sin and cos of 2*pi*u, u = u01_closed_open(hi, lo), table driven.    */
+/* j = nearest multiple of 1/256 turn (from the top bits of hi, 0..256), */
+/* d = u - j/256 exactly (|d| <= 1/512), t = 2 pi d with 2 pi split in    */
+/* hi and lo parts; sin t = t + t^3 (S1 + t^2 (S2 + t^2 S3)) and          */
+/* cos t - 1 = t^2 (C1 + t^2 (C2 + t^2 C3)) (Taylor series); then the     */
+/* addition theorems with {sin, cos}(2 pi j/256) from the table           */
+/* (procell_math_tables.inc).                                             */
 /* ------------------------------------------------------------------ */
-static const double SN0 = 0x1.921fb54442d18p+2;
-static const double SN1 = -0x1.4abbce625be53p+5;
-static const double SN2 = 0x1.466bc6775aae2p+6;
-static const double SN3 = -0x1.32d2cce62bd86p+6;
-static const double SN4 = 0x1.50783487ee782p+5;
-static const double SN5 = -0x1.e3074fde8871fp+3;
-static const double SN6 = 0x1.e8f434d018d63p+1;
-static const double SN7 = -0x1.6fadb9f155744p-1;
-static const double SN8 = 0x1.aaec32af93359p-4;
-static const double CS1 = -0x1.3bd3cc9be45dep+4;
-static const double CS2 = 0x1.03c1f081b5ac4p+6;
-static const double CS3 = -0x1.55d3c7e3cbffap+6;
-static const double CS4 = 0x1.e1f506891babbp+5;
-static const double CS5 = -0x1.a6d1f2a204a8cp+4;
-static const double CS6 = 0x1.f9d38a3763cc3p+2;
-static const double CS7 = -0x1.b6e24f44b128fp+0;
-static const double CS8 = 0x1.20c62c2f2d7f5p-2;
-static const double CS9 = -0x1.2a0c591af8314p-5;
+static const double TWO_PI_HI = 0x1.921fb54442d18p+2;
+static const double TWO_PI_LO = 0x1.1a62633145c07p-52;
+static const double SC_S1 = -0x1.5555555555555p-3;  /* -1/6    */
+static const double SC_S2 = 0x1.1111111111111p-7;   /*  1/120  */
+static const double SC_S3 = -0x1.a01a01a01a01ap-13; /* -1/5040 */
+static const double SC_C1 = -0.5;
+static const double SC_C2 = 0x1.5555555555555p-5;   /*  1/24   */
+static const double SC_C3 = -0x1.6c16c16c16c17p-10; /* -1/720  */
 
 void oracle_sincos_turns(uint32_t hi, uint32_t lo, double *s_out, double *c_out)
 {
     double u = oracle_u01_closed_open(hi, lo);
-    uint32_t q = ((hi >> 29) + 1u) >> 1;          /* 0..4 */
-    double r = u - (double)q * 0.25;              /* exact */
-    double z = r * r;
-    double ps = SN8;
-    ps = fma(ps, z, SN7);
-    ps = fma(ps, z, SN6);
-    ps = fma(ps, z, SN5);
-    ps = fma(ps, z, SN4);
-    ps = fma(ps, z, SN3);
-    ps = fma(ps, z, SN2);
-    ps = fma(ps, z, SN1);
-    double sn = fma(r * z, ps, r * SN0);
-    double pc = CS9;
-    pc = fma(pc, z, CS8);
-    pc = fma(pc, z, CS7);
-    pc = fma(pc, z, CS6);
-    pc = fma(pc, z, CS5);
-    pc = fma(pc, z, CS4);
-    pc = fma(pc, z, CS3);
-    pc = fma(pc, z, CS2);
-    pc = fma(pc, z, CS1);
-    double cs = fma(z, pc, 1.0);
-    switch (q & 3u) {
-    case 0: *s_out = sn; *c_out = cs; break;
-    case 1: *s_out = cs; *c_out = -sn; break;
-    case 2: *s_out = -sn; *c_out = -cs; break;
-    default: *s_out = -cs; *c_out = sn; break;
-    }
+    uint32_t j = ((hi >> 23) + 1u) >> 1;          /* 0..256 */
+    double d = u - (double)j * 0x1p-8;            /* exact */
+    double t = fma(d, TWO_PI_HI, d * TWO_PI_LO);
+    double t2 = t * t;
+    double ps = fma(t2, SC_S3, SC_S2);
+    ps = fma(t2, ps, SC_S1);
+    double st = fma(t * t2, ps, t);
+    double cm = t2 * fma(t2, fma(t2, SC_C3, SC_C2), SC_C1);
+    double sa = procell_sincos_table[j & 255u][0];
+    double ca = procell_sincos_table[j & 255u][1];
+    *s_out = sa + fma(sa, cm, ca * st);
+    *c_out = ca + fma(ca, cm, -(sa * st));
 }
 
 /* Box-Muller (Box & Muller 1958): z0 = rho cos(2 pi u2), z1 = rho sin(2 pi u2),

@@ -6,7 +6,7 @@ these tests pin it against what does not depend on it:
   * the counter layout of a draw against a direct Philox call,
   * the 53-bit uniforms against exact rational arithmetic,
   * log, sqrt and sin/cos of 2 pi u against mpmath at 300 bits (error bounds
-    in units in the last place),
+    in units in the last place: 1.5, 0.5 and 2),
   * Box-Muller pairs against the composition of those primitives and, over
     many draws, against the standard normal distribution (moments and a
     Kolmogorov-Smirnov test).
@@ -79,7 +79,7 @@ def _ulps(got, exact):
     return float(abs(mpmath.mpf(got) - exact) / math.ulp(float(exact)))
 
 
-def test_log_within_one_ulp(oracle):
+def test_log_within_one_and_a_half_ulp(oracle):
     mpmath = pytest.importorskip("mpmath")
     mpmath.mp.prec = 300
     worst = 0.0
@@ -90,10 +90,16 @@ def test_log_within_one_ulp(oracle):
             assert got == 0.0
             continue
         worst = max(worst, _ulps(got, mpmath.log(mpmath.mpf(u))))
-    # powers of two are exact multiples of ln 2 split; values near the fold
-    for x in [2.0**-53, 0.5, 0.25, 0.7071067811865475, 0.7071067811865476, 1.0 - 2.0**-53, 0.999999, 1e-10]:
+    # powers of two, the fold at 1.5, the table intervals next to 1
+    for x in [2.0**-53, 0.5, 0.25, 0.75, 0.7499999999999999, 1.0 - 2.0**-53, 0.999999, 1.0 - 2.0**-8,
+              1.0 - 2.0**-8 - 2.0**-40, 1e-10, 0.5 + 2.0**-52]:
         worst = max(worst, _ulps(oracle.log(x), mpmath.log(mpmath.mpf(x))))
-    assert worst <= 1.0, worst
+    # near 1 the result is exact to about an ulp of the result itself
+    for k in range(1, 2000):
+        x = 1.0 - k * 977 * 2.0**-60
+        worst = max(worst, _ulps(oracle.log(x), mpmath.log(mpmath.mpf(x))))
+    assert worst <= 1.5, worst
+    assert oracle.log(1.0) == 0.0
 
 
 def test_sqrt_correctly_rounded(oracle):
@@ -162,3 +168,19 @@ def test_normal_pairs_are_standard_normal(oracle):
     assert scipy_stats.kstest(z, "norm").pvalue > 1e-4
     # the two members of a pair are uncorrelated
     assert abs(np.corrcoef(z[0::2], z[1::2])[0, 1]) < 5 / math.sqrt(n / 2)
+
+
+def test_math_tables_shared_and_regenerable():
+    """The oracle and the device code include byte-identical copies of the
+    log and sin/cos tables, and the generator reproduces them."""
+    import importlib.util
+
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    a = open(os.path.join(repo, "oracle", "procell_math_tables.inc")).read()
+    b = open(os.path.join(repo, "cuda-pro-cell_amd", "csrc", "procell_math_tables.inc")).read()
+    assert a == b
+    pytest.importorskip("mpmath")
+    spec = importlib.util.spec_from_file_location("gen_math_tables", os.path.join(repo, "oracle", "gen_math_tables.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    assert gen.render() == a
