@@ -258,27 +258,6 @@ double row_cost(const Setup &s, uint32_t row)
     return w;
 }
 
-// Expected share of a row's initial cells that divide at least once (first
-// division completing by t_max, same normal approximation as row_cost).
-double row_dividing(const Setup &s, uint32_t row)
-{
-    if ((s.row_info[row] & 0xFFu) == 0)
-        return 0.0;
-    double w = 0.0, p_prev = 0.0;
-    for (uint32_t j = 0; j < s.n_types; ++j) {
-        const double m = s.type_mean[j];
-        const double sd = fabs(s.type_sd[j]);
-        const double p = s.type_cum[j] - p_prev;
-        p_prev = s.type_cum[j];
-        if (m < 0.0)
-            continue;
-        const double var = (7.0 / 3.0) * sd * sd + m * m / 12.0;
-        const double z = (s.t_max - 1.5 * m) / sqrt(var > 0.0 ? var : 1e-300);
-        w += p * 0.5 * erfc(-z / sqrt(2.0));
-    }
-    return w;
-}
-
 // Split the initial cells into `g` contiguous segments of about equal cost.
 void split_segments(const Setup &s, uint32_t g, std::vector<uint64_t> &begin, std::vector<uint32_t> &row_of)
 {
@@ -372,8 +351,10 @@ int create_plan(const procell_params *p, procell_plan **out)
             return fail(PROCELL_E_DEPTH);
         bs -= 64;  // deep trees: a smaller workgroup keeps the stack within LDS
     }
-    if (tu.blocks_per_cu)
-        occ = std::min<int>(occ, (int)tu.blocks_per_cu);
+    // by default at most 12 resident wavefronts per CU (3 per SIMD): more
+    // shrink every workgroup's tally window and add claim traffic without
+    // adding issue rate (measured on the 1e7-cell histogram workload)
+    occ = std::min<int>(occ, tu.blocks_per_cu ? (int)tu.blocks_per_cu : std::max(1, 12 * 64 / bs));
     ls.tree_block_size = bs;
     const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
     const size_t budget = std::min(lds_per_cu / (size_t)occ, lds_per_block);
@@ -418,28 +399,18 @@ int create_plan(const procell_params *p, procell_plan **out)
     ls.root_block_size = 256;
     ls.root_lds_bytes = root_lds_bytes(T, 256, lds_rows);
     a.win_rows = win_rows;
-    // queue entries per claim from the own segment: at least two claims per
-    // wavefront of the workgroup (a power of two in [64, 4096]) balances the
-    // atomics against the imbalance between the workgroup's wavefronts
-    uint32_t auto_claim = 64;
-    {
-        double entries = 0.0;
-        for (uint32_t r = 0; r < n_rows; ++r)
-            entries += row_dividing(s, r) * (double)(s.start[r + 1] - s.start[r]);
-        const double per_wave = entries / (double)g / (double)(bs / 64);
-        while (auto_claim < 4096 && 2.0 * (double)(auto_claim * 2) <= per_wave)
-            auto_claim *= 2;
-    }
-    a.claim = tu.chunk ? tu.chunk : std::max<uint32_t>(std::min<uint32_t>(auto_claim * 2, 1024), 64);
-    a.steal_claim = std::max<uint32_t>(1, std::min<uint32_t>(a.claim, 64));
+    // queue entries per claim: a share of what the segment has left, at
+    // most `claim` (a few prefetch sets) and at least half a set
+    a.claim = tu.chunk ? std::min<uint32_t>(tu.chunk, 1u << 20) : 1024;
+    a.claim_min = std::min<uint32_t>(a.claim, 32);
     a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 16;
 
-    // ---- one device allocation: counters | tally | seg_next | tables ----
+    // ---- one device allocation: counters | tally | segment cursors | tables ----
     size_t off = 0;
     const size_t o_cnt = off; off += (size_t)kCounterCopies * kNumCounters * 8;
     const size_t o_tally = off; off += pl->tally_len * 8;
-    const size_t o_next = off; off += (size_t)g * 8;
-    const size_t o_total = off; off += (size_t)g * 8;
+    off = align_up(off, 256);
+    const size_t o_cur = off; off += (size_t)g * kCursorWords * 8;
     pl->zero_words = off / 8;
     off = align_up(off, 256);
     // root-kernel chunks: kChunk cells each, none spanning two segments
@@ -487,8 +458,7 @@ int create_plan(const procell_params *p, procell_plan **out)
     a.kdim = kdim;
     a.counters = (unsigned long long *)(dbase + o_cnt);
     a.tally = (unsigned long long *)(dbase + o_tally);
-    a.seg_next = (unsigned long long *)(dbase + o_next);
-    a.seg_total = (unsigned long long *)(dbase + o_total);
+    a.seg_cur = (unsigned long long *)(dbase + o_cur);
     a.chunk_seg = (const uint32_t *)(dbase + o_cseg);
     a.chunk_first = (const uint32_t *)(dbase + o_cfirst);
     a.queue = (QueueEntry *)pl->d_queue;

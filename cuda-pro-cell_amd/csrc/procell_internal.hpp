@@ -46,6 +46,11 @@ enum Counter : int {
 };
 // Copies of the counters block: workgroup b adds into copy b % kCounterCopies,
 // so the device atomics on any one counter stay few; the host sums the copies.
+// Words per segment cursor block.  Claims are device-scope atomics on these
+// cursors; a block of its own per segment spreads them over many cache lines
+// and memory channels instead of queueing them on a few.
+constexpr uint32_t kCursorWords = 32;
+
 constexpr uint32_t kCounterCopies = 64;
 
 // Problem after validation: nonzero rows in file order, types sorted by
@@ -76,8 +81,8 @@ struct RunArgs {
     uint32_t kdim;          // generations + 1
     uint32_t stack_levels;  // per-lane LDS stack depth (levels 1 .. kdim-1)
     uint32_t win_rows;      // rows of the LDS tally window (0: none)
-    uint32_t claim;         // queue entries per claim from the workgroup's own segment
-    uint32_t steal_claim;   // queue entries per claim from another segment
+    uint32_t claim;         // largest claim of queue entries
+    uint32_t claim_min;     // smallest claim (guided claims shrink down to it)
     uint32_t idle_limit;    // idle lanes tolerated before a wavefront refills
     uint32_t rows_in_lds;   // nonzero: the row tables are copied to LDS
     uint32_t n_chunks;      // root-kernel chunks
@@ -89,8 +94,8 @@ struct RunArgs {
     const uint32_t *type_file;   // sorted position -> file index
     const uint64_t *seg_begin;   // [segments + 1] first initial cell of each workgroup's segment
     const uint32_t *seg_row;     // [segments] row of each segment's first cell (window base)
-    unsigned long long *seg_next;  // [segments] queue entries claimed so far per segment
-    unsigned long long *seg_total; // [segments] queue entries per segment
+    unsigned long long *seg_cur;   // [segments][kCursorWords]: word 0 queue entries claimed so
+                                   // far, word 1 queue entries of the segment
     const uint32_t *chunk_seg;   // [chunks] segment of each root-kernel chunk
     const uint32_t *chunk_first; // [segments + 1] first chunk of each segment
     QueueEntry *queue;           // [n_cells] dividing initial cells; segment g's entries

@@ -16,13 +16,16 @@
 //    entries in LDS and appends them to its segment's part of the queue
 //    with one atomic, so each segment's entries are contiguous.
 //  * tree_kernel (proliferation.cu:317-381).  Persistent workgroups, one per
-//    segment.  A wavefront claims entries of its workgroup's segment with one
-//    atomic per claim (the next claim issued ahead of time) and keeps up to
-//    128 entries in two register sets, one entry per lane and set, loaded
-//    ahead of use; when
-//    the segment is exhausted it claims from the segment with the most
-//    unclaimed entries (work stealing across workgroups), found by one
-//    parallel scan of all cursors.  Each busy lane divides its current cell
+//    segment.  A wavefront claims entries with one atomic per claim, first
+//    from its workgroup's segment, then, once that is drained, from the
+//    segment with the most unclaimed entries among 64 probed ones (work
+//    stealing across workgroups), which it keeps draining before it probes
+//    again.  Claims are a share of what the segment has left (guided
+//    self-scheduling), so they shrink towards its end and the wavefronts
+//    finish together; the next claim is issued when the last entries of the
+//    current one are loaded, so its latency overlaps the division steps.
+//    Entries are loaded 64 at a time into two register sets (one entry per
+//    lane and set).  Each busy lane divides its current cell
 //    per step: one Philox block and one Box-Muller pair give both daughters'
 //    timers; daughters that will not divide are tallied or dropped on the
 //    spot, so the lane's next cell is always a dividing one (the left
@@ -46,7 +49,6 @@ namespace {
 using dev::Words;
 
 constexpr uint32_t kWave = 64;
-constexpr uint32_t kClaimAhead = 128;  // claimed entries left when the next own claim is issued
 constexpr uint32_t kNoWindow = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
@@ -387,7 +389,7 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
         __syncthreads();
         const uint32_t cnt = *n_entries;
         if (tid == 0 && cnt)
-            *q_base = (uint32_t)atomicAdd(&a.seg_total[g], (unsigned long long)cnt);
+            *q_base = (uint32_t)atomicAdd(&a.seg_cur[(size_t)g * kCursorWords + 1], (unsigned long long)cnt);
         for (uint32_t i = tid; i < rwin_len; i += bdim) {
             const uint32_t r = rwin[i];
             if (r)
@@ -464,12 +466,14 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
     // first claim of the wavefront's own segment, issued before the set-up
     // so its latency overlaps it (lane 0 holds the result)
     unsigned long long pf_raw = 0;
-    uint32_t pf_size = a.claim;   // size of the claim issued ahead of time
-    uint32_t pf_pending = 1;      // a claim of pf_size is in flight
+    uint32_t pf_size = a.claim;   // size of the claim in flight
+    uint32_t pf_pending = 1;      // a claim of pf_size on segment cur_seg is in flight
     if (lane == 0)
-        pf_raw = atomicAdd(&a.seg_next[seg0], (unsigned long long)pf_size);
-    const uint64_t own_base = uni64(a.seg_begin[seg0]);
-    const uint32_t own_total = uni((uint32_t)a.seg_total[seg0]);
+        pf_raw = atomicAdd(&a.seg_cur[(size_t)seg0 * kCursorWords], (unsigned long long)pf_size);
+    // the segment being drained: first the own one, then victims
+    uint32_t cur_seg = seg0;
+    uint64_t cur_begin = uni64(a.seg_begin[seg0]);
+    uint64_t cur_total = uni64(a.seg_cur[(size_t)seg0 * kCursorWords + 1]);
     if (tid == 0)
         *k_lds = dev::coeffs();
     copy_tables(m_log, m_sc, tid, bdim);
@@ -492,11 +496,17 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
     const dev::Tables tb{m_log, m_sc};
     const uint32_t *ri = RowsLds ? r_info : a.row_info;
 
-    // wavefront-uniform claim state (scalar registers): queue entries
-    // [q_cur, q_end) are claimed and not yet loaded
+    const uint32_t n_waves = bdim / kWave;
+    // claim size for a segment with `rest` unclaimed entries: a share of
+    // them (guided self-scheduling, so claims shrink towards the end and the
+    // wavefronts finish together), between claim_min and claim
+    auto guided = [&](uint64_t rest) {
+        const uint32_t share = (rest >> 32 ? 0xFFFFFFFFu : (uint32_t)rest) / (2u * n_waves);
+        return uni(max(a.claim_min, min(a.claim, share)));
+    };
+    uint32_t n_probe = 0;     // probes so far (varies the probed segments)
+    // claimed entries of segment cur_seg not yet loaded: [q_cur, q_end)
     uint64_t q_cur = 0, q_end = 0;
-    uint32_t victim = 0;      // 0: own segment; 1: own segment exhausted, steal
-    uint32_t n_probe = 0;     // steal attempts so far (varies the probed segments)
     uint32_t roots_done = 0;
     // two prefetch sets of up to 64 entries, entry i of a set held by lane
     // i: hand-outs read the current set (entries pos .. pos + have - 1);
@@ -595,88 +605,84 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
             }
         }
         // ======== fill the incoming prefetch set once it is empty ========
-        while (!roots_done && in_have == 0) {
-            // own segment: issue the next claim when the current one is
-            // nearly used up, so its latency overlaps the last entries but
-            // no wavefront holds two claims' worth of unstarted entries;
-            // sized by what the segment has left (guided self-scheduling:
-            // claims shrink towards the end so the wavefronts finish
-            // together)
-            if (!victim && !pf_pending && q_end - q_cur < kClaimAhead) {
-                const uint64_t used = min(uni64(q_end - own_base), (uint64_t)own_total);
-                const uint64_t rest = ((uint64_t)own_total - used) / (2u * (bdim / kWave));
-                pf_size = uni((uint32_t)max<uint64_t>(a.steal_claim, min<uint64_t>(a.claim, rest)));
-                pf_pending = 1;
-                if (lane == 0)
-                    pf_raw = atomicAdd(&a.seg_next[seg0], (unsigned long long)pf_size);
-            }
+        // (straight-line in the common case, so a claim issued here is
+        // defined once per iteration and not waited for before it is used)
+        if (!roots_done && in_have == 0) {
             if (q_cur >= q_end) {
-                // ---- claim more entries ----
-                if (!victim) {
-                    // take the claim in flight (waits for it if needed)
-                    const uint64_t base = min(uni64(__shfl(pf_raw, 0)), (uint64_t)own_total);
+                // take the claim in flight (issued when the previous claim's
+                // last entries were loaded)
+                uint64_t base = ~0ull;
+                if (pf_pending) {
+                    base = uni64(__shfl(pf_raw, 0));
                     pf_pending = 0;
-                    if (base < own_total) {
-                        q_cur = uni64(own_base + base);
-                        q_end = uni64(own_base + min(base + pf_size, (uint64_t)own_total));
-                    } else {
-                        victim = 1;  // own segment exhausted: steal
-                    }
-                    continue;
                 }
-                // a victim segment: each lane probes one segment (the 64
-                // probes of a wavefront spread over all segments, shifted
+                // the segment is drained: each lane probes one segment (the
+                // 64 probes of a wavefront spread over all segments, shifted
                 // by a hash of the wavefront and the attempt, so concurrent
                 // thieves look at different segments); the probe with the
-                // most entries left wins.  No entries left in any probe ends
-                // the roots: a segment's own workgroup always drains it.
-                const uint32_t h = hash32((seg0 * (bdim / kWave) + wave) * 0x9E3779B9u + n_probe * 0x85EBCA6Bu);
-                n_probe = uni(n_probe + 1);
-                const uint32_t stride = n_seg > kWave ? n_seg / kWave : 1u;
-                const uint32_t cand = (h + lane * stride + (h >> 16) % stride) % n_seg;
-                const uint64_t tot = a.seg_total[cand];
-                const uint64_t used = __hip_atomic_load(&a.seg_next[cand], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                uint64_t left = used < tot ? tot - used : 0;
-                uint32_t best = cand;
+                // most entries left becomes the segment to drain.  No
+                // entries left in any probe ends the roots: a segment's own
+                // workgroup always drains it.
+                while (base >= cur_total) {
+                    const uint32_t h = hash32((seg0 * n_waves + wave) * 0x9E3779B9u + n_probe * 0x85EBCA6Bu);
+                    n_probe = uni(n_probe + 1);
+                    const uint32_t stride = n_seg > kWave ? n_seg / kWave : 1u;
+                    const uint32_t cand = (h + lane * stride + (h >> 16) % stride) % n_seg;
+                    const uint64_t tot = a.seg_cur[(size_t)cand * kCursorWords + 1];
+                    const uint64_t used = __hip_atomic_load(&a.seg_cur[(size_t)cand * kCursorWords], __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+                    uint64_t left = used < tot ? tot - used : 0;
+                    uint32_t best = cand;
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    const uint64_t ol = __shfl_xor(left, off);
-                    const uint32_t ob = __shfl_xor(best, off);
-                    if (ol > left || (ol == left && ob < best)) {
-                        left = ol;
-                        best = ob;
+                    for (int off = 32; off > 0; off >>= 1) {
+                        const uint64_t ol = __shfl_xor(left, off);
+                        const uint32_t ob = __shfl_xor(best, off);
+                        if (ol > left || (ol == left && ob < best)) {
+                            left = ol;
+                            best = ob;
+                        }
                     }
+                    if (uni64(left) == 0) {
+                        roots_done = 1;
+                        break;
+                    }
+                    cur_seg = uni(best);
+                    cur_begin = uni64(a.seg_begin[cur_seg]);
+                    cur_total = uni64(a.seg_cur[(size_t)cur_seg * kCursorWords + 1]);
+                    pf_size = guided(uni64(left));
+                    unsigned long long r = 0;
+                    if (lane == 0)
+                        r = atomicAdd(&a.seg_cur[(size_t)cur_seg * kCursorWords], (unsigned long long)pf_size);
+                    base = uni64(__shfl(r, 0));
                 }
-                if (uni64(left) == 0) {
-                    roots_done = 1;
-                    break;
+                if (!roots_done) {
+                    q_cur = uni64(cur_begin + base);
+                    q_end = uni64(cur_begin + min(base + pf_size, cur_total));
                 }
-                const uint32_t sg = uni(best);
-                // guided: a share of what the victim has left, so steals stay
-                // few while there is much to take and small at the end
-                const uint32_t ssz = uni((uint32_t)max<uint64_t>(
-                    a.steal_claim, min<uint64_t>(a.claim, uni64(left) / (2u * (bdim / kWave)))));
-                unsigned long long sbase = ~0ull;
-                if (lane == 0)
-                    sbase = atomicAdd(&a.seg_next[sg], (unsigned long long)ssz);
-                const uint64_t base = uni64(__shfl(sbase, 0));
-                const uint64_t total = uni64(a.seg_total[sg]);
-                if (base < total) {
-                    const uint64_t sbeg = uni64(a.seg_begin[sg]);
-                    q_cur = uni64(sbeg + base);
-                    q_end = uni64(sbeg + min(base + ssz, total));
-                }
-                continue;
             }
-            // ---- load up to 64 claimed entries (lanes past the last one
-            // load it again; their copies are never handed out) ----
-            const uint32_t nf = uni((uint32_t)min((uint64_t)kWave, q_end - q_cur));
-            const QueueEntry e = a.queue[q_cur + min(lane, nf - 1)];
-            i_s0 = e.s0;
-            i_cell = e.cell;
-            i_rt = e.row_type;
-            q_cur = uni64(q_cur + nf);
-            in_have = nf;
+            if (!roots_done) {
+                // load up to 64 claimed entries (lanes past the last one load
+                // it again; their copies are never handed out)
+                const uint32_t nf = uni((uint32_t)min<uint64_t>(kWave, q_end - q_cur));
+                const QueueEntry e = a.queue[q_cur + min(lane, nf - 1)];
+                i_s0 = e.s0;
+                i_cell = e.cell;
+                i_rt = e.row_type;
+                in_have = nf;
+                q_cur = uni64(q_cur + nf);
+                // the claim's last entries are loaded: issue the next claim
+                // on the same segment now, so its latency overlaps the steps,
+                // sized by what the segment has left (guided self-scheduling:
+                // claims shrink towards the end, so the wavefronts finish
+                // together)
+                const uint64_t rest = cur_begin + cur_total - q_end;
+                if (q_cur >= q_end && rest) {
+                    pf_size = guided(rest);
+                    pf_pending = 1;
+                    if (lane == 0)
+                        pf_raw = atomicAdd(&a.seg_cur[(size_t)cur_seg * kCursorWords], (unsigned long long)pf_size);
+                }
+            }
         }
 
         if (__ballot(L.active) == 0) {

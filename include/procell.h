@@ -67,7 +67,7 @@ typedef struct {
 typedef struct {
     uint32_t block_size;    /* threads per workgroup of the tree kernel: 64..256, multiple of 64 */
     uint32_t blocks_per_cu; /* resident workgroups per CU of the persistent tree kernel */
-    uint32_t chunk;         /* initial cells a wavefront claims from the work queue at once */
+    uint32_t chunk;         /* most dividing initial cells a wavefront claims from the work queue at once */
     uint32_t idle_lanes;    /* idle lanes a wavefront tolerates before refilling them (0..63) */
     uint32_t flags;         /* PROCELL_FLAG_* */
 } procell_tuning;
