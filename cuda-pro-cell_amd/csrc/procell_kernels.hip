@@ -295,9 +295,11 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             const uint32_t info = ri[valid ? row : row_cur];
             const uint32_t kc = info & 0xFFu;
             const bool ge_phi = (info >> 8) & 1u;
+            // one block per initial cell: type uniform (word 0), phase (word
+            // 1) and the first timer's Box-Muller pair (words 2, 3)
             const Words w = dev::draw_block(a.seed, c, 0, 0);
-            const double ut = dev::uniform_co(w.w0, w.w1);
-            const double U = dev::uniform_oc(w.w2, w.w3);
+            const double ut = dev::uniform_co(w.w0, 0u);
+            const double U = dev::uniform_oc(w.w1, 0u);
             // first type with ut < cumulative proportion (cell.cu:90-101);
             // the cumulative proportions are uniform, read eight at a time
             // through the scalar cache so their latencies overlap
@@ -318,10 +320,10 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             const double m = t_mean[jt];
             const double sdv = t_sd[jt];
             const bool quiescent = m < 0.0;
-            // first timer: stream of node 1 from attempt 1 (cell.cu:106-122)
-            const Words w1 = dev::draw_block(a.seed, c, 1, 1);
+            // first timer (cell.cu:106-122): two candidates from the root
+            // block, then node 1's redraw stream
             double z0, z1;
-            dev::normal_pair(w1, z0, z1, K, tb);
+            dev::normal_pair(Words{w.w2, 0u, w.w3, 0u}, z0, z1, K, tb);
             const double c0 = fma(z0, sdv, m);
             const double c1 = fma(z1, sdv, m);
             const bool live = valid && !quiescent;
@@ -329,9 +331,9 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             double tau0 = c0 > 0.0 ? c0 : c1;
             if (__builtin_expect(__ballot(live && !(tau0 > 0.0)) != 0, 0)) {
                 if (live && !(tau0 > 0.0)) {
-                    // rare: keep drawing from attempt 2
+                    // rare: node 1's redraw stream from attempt 1
                     tau0 = m;
-                    for (uint32_t att = 2; att < 65536u; ++att) {
+                    for (uint32_t att = 1; att < 65536u; ++att) {
                         const Words wr = dev::draw_block(a.seed, c, 1, att);
                         double y0, y1;
                         dev::normal_pair(wr, y0, y1, K, tb);
@@ -703,30 +705,48 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
             // its tree in that step.  Flags are 0/1 integers so the step's
             // logic stays in vector instructions.
             const uint32_t act = L.active ? 1u : 0u;
-            const uint32_t fresh = L.redo == 0 ? 1u : 0u;
-            const uint32_t rright = (L.redo & 1u) ? 0u : 1u;   // redraw the right daughter?
-            const N bnode = fresh ? L.node : (L.node << 1) | (N)rright;
-            const Words w = dev::draw_block(a.seed, L.cell, bnode, fresh ? 0u : L.att);
-            double z0, z1;
-            dev::normal_pair(w, z0, z1, K, tb);
-            const double c0 = fma(z0, L.sd, L.mean);
-            const double c1 = fma(z1, L.sd, L.mean);
-            const uint32_t r0 = c0 > 0.0 ? 0u : 1u;
-            const uint32_t r1 = c1 > 0.0 ? 0u : 1u;
-            // redraw: first candidate > 0 of this block; after 65535 blocks the mean
-            const uint32_t got = (r0 & r1) ^ 1u;
-            const uint32_t give_up = (got ^ 1u) & (L.att >= 65535u ? 1u : 0u);
-            const double v = give_up ? L.mean : (r0 ? c1 : c0);
-            const uint32_t done = got | give_up;
-            const uint32_t rbit = rright ? 2u : 1u;
-            const double tl = fresh ? c0 : ((done & (rright ^ 1u)) ? v : L.tl);
-            const double tr = fresh ? c1 : ((done & rright) ? v : L.tr);
-            const uint32_t redo = fresh ? (r0 | (r1 << 1)) : (done ? L.redo & ~rbit : L.redo);
-            n.div += act & fresh;
-            n.rej += act * (fresh ? r0 + r1 : r0 + (r0 & r1));
+            double tl, tr;
+            uint32_t redo;
+            if (__builtin_expect(__ballot(L.redo != 0) == 0, 1)) {
+                // no redraw pending in the wavefront (the common case): every
+                // busy lane starts a new division with its pair block
+                const Words w = dev::draw_block(a.seed, L.cell, L.node, 0u);
+                double z0, z1;
+                dev::normal_pair(w, z0, z1, K, tb);
+                tl = fma(z0, L.sd, L.mean);
+                tr = fma(z1, L.sd, L.mean);
+                const uint32_t r0 = tl > 0.0 ? 0u : 1u;
+                const uint32_t r1 = tr > 0.0 ? 0u : 1u;
+                redo = r0 | (r1 << 1);
+                n.div += act;
+                n.rej += act * (r0 + r1);
+                L.att = 1u;
+            } else {
+                const uint32_t fresh = L.redo == 0 ? 1u : 0u;
+                const uint32_t rright = (L.redo & 1u) ? 0u : 1u;   // redraw the right daughter?
+                const N bnode = fresh ? L.node : (L.node << 1) | (N)rright;
+                const Words w = dev::draw_block(a.seed, L.cell, bnode, fresh ? 0u : L.att);
+                double z0, z1;
+                dev::normal_pair(w, z0, z1, K, tb);
+                const double c0 = fma(z0, L.sd, L.mean);
+                const double c1 = fma(z1, L.sd, L.mean);
+                const uint32_t r0 = c0 > 0.0 ? 0u : 1u;
+                const uint32_t r1 = c1 > 0.0 ? 0u : 1u;
+                // redraw: first candidate > 0 of this block; after 65535 blocks the mean
+                const uint32_t got = (r0 & r1) ^ 1u;
+                const uint32_t give_up = (got ^ 1u) & (L.att >= 65535u ? 1u : 0u);
+                const double v = give_up ? L.mean : (r0 ? c1 : c0);
+                const uint32_t done = got | give_up;
+                const uint32_t rbit = rright ? 2u : 1u;
+                tl = fresh ? c0 : ((done & (rright ^ 1u)) ? v : L.tl);
+                tr = fresh ? c1 : ((done & rright) ? v : L.tr);
+                redo = fresh ? (r0 | (r1 << 1)) : (done ? L.redo & ~rbit : L.redo);
+                n.div += act & fresh;
+                n.rej += act * (fresh ? r0 + r1 : r0 + (r0 & r1));
+                L.att = fresh | done ? 1u : L.att + 1u;
+            }
             L.tl = tl;
             L.tr = tr;
-            L.att = fresh | done ? 1u : L.att + 1u;
             L.redo = act ? redo : 0u;
             const uint32_t cmp = act & (redo == 0 ? 1u : 0u);  // division complete this step
 

@@ -6,10 +6,13 @@
 //   counter = (node[31:0], node[63:32], cell[31:0], cell[47:32] | attempt << 16)
 // node is the heap index of a cell in its division tree (root 1, daughters of
 // n are 2n and 2n+1); node 0 carries the root's set-up draws.
-//   (cell, 0, 0)    -> type uniform [0,1) and initial phase U (0,1]
+//   (cell, 0, 0)    -> the root's draws, one word each: type uniform [0,1)
+//                      (word 0), initial phase U (0,1] (word 1), Box-Muller
+//                      pair of its first timer's candidates (words 2, 3)
 //   (cell, n, 0)    -> Box-Muller pair: timers of daughters 2n and 2n+1
 //   (cell, n, a>=1) -> truncated-normal redraw stream of node n; the root's
-//                      first timer comes from this stream
+//                      first timer continues in node 1's stream when both
+//                      candidates are <= 0
 // This replaces the reference's per-draw cuRAND XORWOW re-initialisation
 // (src/utils/util.cu:143-169) and its colliding arithmetic seeds
 // (src/simulation/proliferation.cu:330-331, cells_population.cu:108).

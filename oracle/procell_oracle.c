@@ -17,10 +17,14 @@
  *   counter = (node lo, node hi, cell lo, cell hi[15:0] | attempt << 16).
  *   node is the heap index of a cell in its division tree (root = 1,
  *   daughters of n are 2n and 2n+1); node 0 holds the root's setup draws.
- *     block(cell, 0, 0)     -> type uniform in [0,1), phase uniform U in (0,1]
+ *     block(cell, 0, 0)     -> the root's draws, one word each: type uniform
+ *                              in [0,1) (word 0), phase U in (0,1] (word 1),
+ *                              and the Box-Muller pair of its first timer's
+ *                              two candidates (words 2, 3)
  *     block(cell, n, 0)     -> Box-Muller pair: timers of daughters 2n and 2n+1
  *     block(cell, n, a>=1)  -> truncated-normal redraw stream of node n (the
- *                              root's first timer also comes from this stream)
+ *                              root's first timer continues in node 1's
+ *                              stream when both of its candidates are <= 0)
  */
 #define _POSIX_C_SOURCE 200809L
 #include "procell_oracle.h"
@@ -486,8 +490,8 @@ static void root(cell_walk *w)
     const oracle_setup *s = w->s;
     uint32_t blk[4];
     oracle_block(s->seed, w->cell, 0, 0, blk);
-    double ut = oracle_u01_closed_open(blk[0], blk[1]);
-    double U = oracle_u01_open_closed(blk[2], blk[3]);
+    double ut = oracle_u01_closed_open(blk[0], 0u);
+    double U = oracle_u01_open_closed(blk[1], 0u);
     uint32_t j = 0;
     while (j + 1 < s->T && !(ut < s->cum[j]))
         ++j;
@@ -504,7 +508,20 @@ static void root(cell_walk *w)
             w->st[ORACLE_STAT_LOST] += 1;
         return;
     }
-    double tau0 = truncated_timer(w, 1);
+    /* first timer (cell.cu:106-122): two candidates from words 2 and 3 of
+     * the root block, then node 1's redraw stream */
+    const uint32_t pw[4] = {blk[2], 0u, blk[3], 0u};
+    double z0, z1;
+    oracle_normal_pair(pw, &z0, &z1);
+    double tau0 = fma(z0, w->sd, w->mean);
+    if (!(tau0 > 0.0)) {
+        w->st[ORACLE_STAT_REJECTED] += 1;
+        tau0 = fma(z1, w->sd, w->mean);
+        if (!(tau0 > 0.0)) {
+            w->st[ORACLE_STAT_REJECTED] += 1;
+            tau0 = truncated_timer(w, 1);
+        }
+    }
     double t0 = tau0 * U;
     double s0 = t0 + tau0;
     if (s0 > s->t_max) {

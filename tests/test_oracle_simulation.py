@@ -32,9 +32,12 @@ def _problem(oracle, values, counts, prop, mean, sd, t_max, phi=0.0, seed=0):
 
 class Brute:
     """Scalar restatement of S2-S6 on top of the stream primitives:
-    type/phase block (cell, 0, 0); pair block (cell, n, 0) for the timers of
+    root block (cell, 0, 0): type uniform from word 0, phase from word 1,
+    first-timer candidates from the Box-Muller pair of words 2, 3 (32 bits
+    each); pair block (cell, n, 0) for the timers of
     daughters 2n and 2n+1; redraw blocks (cell, n, a >= 1) for node n, two
-    candidates each; the root's first timer is node 1's redraw stream."""
+    candidates each; the root's first timer continues in node 1's redraw
+    stream when both of its candidates are <= 0."""
 
     def __init__(self, oracle, values, counts, prop, mean, sd, t_max, phi, seed):
         self.o = oracle
@@ -105,8 +108,8 @@ class Brute:
         for r, (v, n) in enumerate(self.rows):
             for _ in range(n):
                 w = self.o.block(self.seed, cell, 0, 0)
-                ut = self.o.u01_closed_open(w[0], w[1])
-                U = self.o.u01_open_closed(w[2], w[3])
+                ut = self.o.u01_closed_open(w[0], 0)
+                U = self.o.u01_open_closed(w[1], 0)
                 js = next((i for i, c in enumerate(self.cum) if ut < c), self.T - 1)
                 j = self.order[js]
                 mu, sd = self.mean[j], self.sd[j]
@@ -114,7 +117,15 @@ class Brute:
                 if mu < 0:
                     self._root_out(r, v, j)
                 else:
-                    tau = self.timer(cell, 1, mu, sd)
+                    tau = None
+                    for z in self.o.normal_pair([w[2], 0, w[3], 0]):
+                        t = _fma(z, sd, mu)
+                        if t > 0:
+                            tau = t
+                            break
+                        self.st["rejected"] += 1
+                    if tau is None:
+                        tau = self.timer(cell, 1, mu, sd)
                     s0 = tau * U + tau
                     if s0 > self.t_max:
                         self._root_out(r, v, j)
@@ -275,7 +286,7 @@ def test_sd_zero_closed_form(oracle):
     for r, (v, n) in enumerate(zip(values, counts)):
         for _ in range(n):
             w = oracle.block(seed, cell, 0, 0)
-            U = oracle.u01_open_closed(w[2], w[3])
+            U = oracle.u01_open_closed(w[1], 0)
             s = mean * U + mean
             nd = 0
             while s <= t_max:
