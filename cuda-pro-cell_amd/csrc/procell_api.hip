@@ -413,23 +413,23 @@ int create_plan(const procell_params *p, procell_plan **out)
     const size_t o_cur = off; off += (size_t)g * kCursorWords * 8;
     pl->zero_words = off / 8;
     off = align_up(off, 256);
-    // root-kernel chunks: kChunk cells each, none spanning two segments
-    std::vector<uint32_t> chunk_first(g + 1, 0), chunk_seg;
+    // root-kernel units: kUnit cells each, none spanning two segments
+    std::vector<uint32_t> unit_first(g + 1, 0), unit_seg;
     for (uint32_t i = 0; i < g; ++i) {
         const uint64_t len = seg_begin[i + 1] - seg_begin[i];
-        const uint64_t nc = (len + kChunk - 1) / kChunk;
-        chunk_first[i + 1] = chunk_first[i] + (uint32_t)nc;
-        chunk_seg.insert(chunk_seg.end(), (size_t)nc, i);
+        const uint64_t nc = (len + kUnit - 1) / kUnit;
+        unit_first[i + 1] = unit_first[i] + (uint32_t)nc;
+        unit_seg.insert(unit_seg.end(), (size_t)nc, i);
     }
-    const size_t o_cseg = off; off = align_up(off + chunk_seg.size() * 4 + 4, 256);
-    const size_t o_cfirst = off; off = align_up(off + (size_t)(g + 1) * 4, 256);
-    a.n_chunks = (uint32_t)chunk_seg.size();
+    const size_t o_useg = off; off = align_up(off + unit_seg.size() * 4 + 4, 256);
+    const size_t o_ufirst = off; off = align_up(off + (size_t)(g + 1) * 4, 256);
+    a.n_units = (uint32_t)unit_seg.size();
     {
         int occ_r = 0;
         if (root_occupancy(256, ls.root_lds_bytes, lds_rows != 0, &occ_r) != hipSuccess)
             return fail(PROCELL_E_HIP);
         const uint64_t resident = (uint64_t)n_cu * (uint64_t)std::max(occ_r, 1);
-        ls.root_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>(a.n_chunks, resident));
+        ls.root_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((a.n_units + 3) / 4, resident));
     }
     const size_t o_rs = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
     const size_t o_ri = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
@@ -459,8 +459,8 @@ int create_plan(const procell_params *p, procell_plan **out)
     a.counters = (unsigned long long *)(dbase + o_cnt);
     a.tally = (unsigned long long *)(dbase + o_tally);
     a.seg_cur = (unsigned long long *)(dbase + o_cur);
-    a.chunk_seg = (const uint32_t *)(dbase + o_cseg);
-    a.chunk_first = (const uint32_t *)(dbase + o_cfirst);
+    a.unit_seg = (const uint32_t *)(dbase + o_useg);
+    a.unit_first = (const uint32_t *)(dbase + o_ufirst);
     a.queue = (QueueEntry *)pl->d_queue;
     a.row_start = (const uint32_t *)(dbase + o_rs);
     a.row_info = (const uint32_t *)(dbase + o_ri);
@@ -478,7 +478,7 @@ int create_plan(const procell_params *p, procell_plan **out)
         !up(o_cum, s.type_cum.data(), s.type_cum.size() * 8) || !up(o_mean, s.type_mean.data(), s.type_mean.size() * 8) ||
         !up(o_sd, s.type_sd.data(), s.type_sd.size() * 8) || !up(o_tf, s.type_file.data(), s.type_file.size() * 4) ||
         !up(o_sb, seg_begin.data(), seg_begin.size() * 8) || !up(o_sr, seg_row.data(), seg_row.size() * 4) ||
-        !up(o_cseg, chunk_seg.data(), chunk_seg.size() * 4) || !up(o_cfirst, chunk_first.data(), chunk_first.size() * 4))
+        !up(o_useg, unit_seg.data(), unit_seg.size() * 4) || !up(o_ufirst, unit_first.data(), unit_first.size() * 4))
         return fail(PROCELL_E_HIP);
     if (hipStreamSynchronize(pl->stream) != hipSuccess)
         return fail(PROCELL_E_HIP);

@@ -17,13 +17,14 @@ constexpr uint64_t kMaxCells = 0xFFFFFFFFull;
 constexpr uint64_t kMaxRows = 1ull << 24;
 constexpr uint32_t kMaxTypes = 256;
 constexpr int kMaxGeneration = 62;  // heap index of a node must fit 64 bits
-constexpr uint32_t kChunk = 1024;   // initial cells per root-kernel workgroup
+constexpr uint32_t kUnit = 256;     // initial cells per root-kernel work unit (one wavefront's)
 constexpr uint32_t kMaxSegments = 2048;  // tree-kernel workgroups (= segments) at most
 
-// Rows of the root kernel's generation-0 tally window (LDS, at most 8 KB).
+// Rows of a root-kernel wavefront's generation-0 tally window (LDS, at most
+// 2 KB; a unit of kUnit cells mostly lies in one or two rows).
 __host__ __device__ constexpr uint32_t root_win_rows(uint32_t n_types)
 {
-    return n_types <= 64 ? 32u : 2048u / n_types;
+    return n_types <= 64 ? 8u : 512u / n_types;
 }
 
 // A dividing initial cell, written by the root kernel for the tree kernel.
@@ -85,7 +86,7 @@ struct RunArgs {
     uint32_t claim_min;     // smallest claim (guided claims shrink down to it)
     uint32_t idle_limit;    // idle lanes tolerated before a wavefront refills
     uint32_t rows_in_lds;   // nonzero: the row tables are copied to LDS
-    uint32_t n_chunks;      // root-kernel chunks
+    uint32_t n_units;       // root-kernel work units
     const uint32_t *row_start;   // [n_rows + 1] first cell of each nonzero row (cells < 2^32)
     const uint32_t *row_info;    // [n_rows]: generation cap | (value >= phi) << 8
     const double *type_cum;      // [n_types] sorted order
@@ -96,8 +97,8 @@ struct RunArgs {
     const uint32_t *seg_row;     // [segments] row of each segment's first cell (window base)
     unsigned long long *seg_cur;   // [segments][kCursorWords]: word 0 queue entries claimed so
                                    // far, word 1 queue entries of the segment
-    const uint32_t *chunk_seg;   // [chunks] segment of each root-kernel chunk
-    const uint32_t *chunk_first; // [segments + 1] first chunk of each segment
+    const uint32_t *unit_seg;    // [units] segment of each root-kernel unit
+    const uint32_t *unit_first;  // [segments + 1] first unit of each segment
     QueueEntry *queue;           // [n_cells] dividing initial cells; segment g's entries
                                  // start at seg_begin[g]
     unsigned long long *tally;   // [n_rows][kdim][n_types] file type order

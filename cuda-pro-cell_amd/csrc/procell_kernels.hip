@@ -11,10 +11,10 @@
 //    tallied (generation 0) or dropped at once; a dividing cell becomes a
 //    16-byte queue entry.  The initial cells are split on the host into one
 //    contiguous segment per tree-kernel workgroup, of roughly equal expected
-//    work, and each segment into chunks of kChunk cells, one root workgroup
-//    each, so every root workgroup has the same work.  A chunk stages its
-//    entries in LDS and appends them to its segment's part of the queue
-//    with one atomic, so each segment's entries are contiguous.
+//    work, and each segment into units of kUnit cells, one wavefront's work
+//    item each.  A unit stages its entries in LDS and appends them to its
+//    segment's part of the queue with one atomic, so each segment's entries
+//    are contiguous.
 //  * tree_kernel (proliferation.cu:317-381).  Persistent workgroups, one per
 //    segment.  A wavefront claims entries with one atomic per claim, first
 //    from its workgroup's segment, then, once that is drained, from the
@@ -207,19 +207,27 @@ __device__ __forceinline__ void flush_counts(const Counts &n, unsigned long long
 }
 
 // ======================================================================
-// root_kernel: persistent workgroups; workgroup b takes chunks b, b + G,
-// b + 2G, ... (G workgroups), chunk c = cells [lo, hi) of segment
-// chunk_seg[c], kChunk cells (fewer at the segment's end); wavefront w takes
-// the w-th quarter of a chunk.  Chunks have equal cost, so the static
-// round-robin balances.
-// LDS: log and sin/cos tables | type tables (cum, mean, sd: double [T]; file:
-// u32 [T]) | row tables when they fit (first cell: u32 [n_rows + 1]; info:
-// u32 [n_rows]) | row-start flags u32 [waves * 64] | entry counter u32 |
-// queue base u32 |
-// 2 spare u32 | counter sums u32 [8] | generation-0 tally window u32
-// [root_win_rows(T) * T + 1 spare] over the chunk's first rows | staged
-// entries QueueEntry [kChunk]
+// root_kernel: persistent wavefronts working independently on units of
+// kUnit initial cells (cells [lo, hi) of segment unit_seg[u], fewer at the
+// segment's end).  Wavefront i of the grid takes units i, i + W, i + 2W,
+// ... (W wavefronts; units have equal cost, and a counter shared by all
+// wavefronts would serialise its atomics).  A unit's dividing cells are staged in the wavefront's LDS area and
+// appended to the segment's part of the queue with one atomic; its
+// generation-0 tallies go to the wavefront's LDS window over the unit's
+// first rows.  No workgroup barrier inside the loop, so one wavefront's
+// atomic latencies are hidden by the others.
+// LDS: log and sin/cos tables | type tables (cum, mean, sd: double [T];
+// file: u32 [T]) | row tables when they fit (first cell: u32 [n_rows + 1];
+// info: u32 [n_rows]) | counter sums u32 [8] | per wavefront: row-start
+// flags u32 [64], generation-0 tally window u32 [root_win_rows(T) * T + 1
+// spare], staged entries QueueEntry [kUnit]
 // ======================================================================
+__host__ __device__ constexpr size_t root_wave_lds_bytes(uint32_t n_types)
+{
+    return kWave * 4 + ((size_t)root_win_rows(n_types) * n_types * 4 + 4 + 15) / 16 * 16 +
+           (size_t)kUnit * sizeof(QueueEntry);
+}
+
 template <bool RowsLds>
 __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
 {
@@ -242,13 +250,11 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
     uint32_t *t_file = (uint32_t *)(t_sd + T);
     uint32_t *r_start = (uint32_t *)(lds + kMathTableBytes + ((size_t)T * 28 + 15) / 16 * 16);
     uint32_t *r_info = r_start + lds_rows + 1;
-    uint32_t *flags_all = (uint32_t *)((unsigned char *)r_start + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16);
-    uint32_t *flags = flags_all + wave * kWave;
-    uint32_t *n_entries = flags_all + waves * kWave;
-    uint32_t *q_base = n_entries + 1;
-    uint32_t *red = n_entries + 4;    // [kNumCounters] workgroup counter sums
-    uint32_t *rwin = n_entries + 12;
-    QueueEntry *stage = (QueueEntry *)((unsigned char *)rwin + ((size_t)rwin_len * 4 + 4 + 15) / 16 * 16);
+    uint32_t *red = (uint32_t *)((unsigned char *)r_start + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16);
+    unsigned char *wlds = (unsigned char *)red + 32 + (size_t)wave * root_wave_lds_bytes(T);
+    uint32_t *flags = (uint32_t *)wlds;
+    uint32_t *rwin = flags + kWave;
+    QueueEntry *stage = (QueueEntry *)(wlds + kWave * 4 + ((size_t)rwin_len * 4 + 4 + 15) / 16 * 16);
 
     copy_tables(m_log, m_sc, tid, bdim);
     for (uint32_t i = tid; i < T; i += bdim) {
@@ -263,32 +269,30 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
         r_info[i] = a.row_info[i];
     if (tid < kNumCounters)
         red[tid] = 0u;
+    __syncthreads();
     // the row tables' address space is known at compile time (no flat loads)
     const uint32_t *rs = RowsLds ? r_start : a.row_start;
     const uint32_t *ri = RowsLds ? r_info : a.row_info;
     const dev::Coeffs K = dev::coeffs();
     const dev::Tables tb{m_log, m_sc};
     Counts n = {0, 0, 0, 0, 0};
+    const uint32_t n_w = gridDim.x * waves;
 
-    for (uint32_t ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
-        const uint32_t g = a.chunk_seg[ch];
-        const uint64_t sb = a.seg_begin[g];
-        const uint64_t lo = uni64(sb + (uint64_t)(ch - a.chunk_first[g]) * kChunk);
-        const uint64_t hi = uni64(min(lo + kChunk, a.seg_begin[g + 1]));
-        const uint32_t quarter = (kChunk + waves - 1) / waves;
-        const uint64_t wlo = uni64(min(lo + (uint64_t)wave * quarter, hi));
-        const uint64_t whi = uni64(min(wlo + quarter, hi));
-        if (tid == 0)
-            *n_entries = 0u;
-        for (uint32_t i = tid; i <= rwin_len; i += bdim)
+    uint32_t u = blockIdx.x * waves + wave;
+    while (u < a.n_units) {
+        const uint32_t g = uni(a.unit_seg[u]);
+        const uint64_t sb = uni64(a.seg_begin[g]);
+        const uint64_t lo = uni64(sb + (uint64_t)(u - a.unit_first[g]) * kUnit);
+        const uint64_t hi = uni64(min(lo + kUnit, a.seg_begin[g + 1]));
+        for (uint32_t i = lane; i <= rwin_len; i += kWave)
             rwin[i] = 0u;  // rwin[rwin_len] absorbs the adds of lanes outside the window
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
 
         const uint32_t row_lo = uni(find_row_wave(rs, 0, a.n_rows, lo, lane));  // first row of the window
-        uint32_t row_cur = wlo < whi ? uni(find_row_wave(rs, row_lo, a.n_rows, wlo, lane)) : 0u;
-
-        for (uint64_t c_cur = wlo; c_cur < whi; c_cur += kWave) {
-            const uint32_t n_valid = uni(whi - c_cur < kWave ? (uint32_t)(whi - c_cur) : kWave);
+        uint32_t row_cur = row_lo;
+        uint32_t cnt = 0;  // entries staged so far
+        for (uint64_t c_cur = lo; c_cur < hi; c_cur += kWave) {
+            const uint32_t n_valid = uni(hi - c_cur < kWave ? (uint32_t)(hi - c_cur) : kWave);
             const uint64_t c = c_cur + lane;
             const bool valid = lane < n_valid;
             const uint32_t row = step_row(rs, a.n_rows, row_cur, c_cur, n_valid, lane, flags);
@@ -374,35 +378,32 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             }
             // dividing cells: staged in LDS, appended to the queue at the end
             const uint64_t dmask = __ballot(divides);
-            if (dmask) {
-                uint32_t base = 0;
-                if (lane == 0)
-                    base = atomicAdd(n_entries, (uint32_t)__popcll(dmask));
-                base = uni(__shfl(base, 0));
-                if (divides) {
-                    QueueEntry &e = stage[base + lane_rank(dmask)];
-                    e.s0 = s0;
-                    e.cell = (uint32_t)c;
-                    e.row_type = row | (jt << 24);
-                }
+            if (divides) {
+                QueueEntry &e = stage[cnt + lane_rank(dmask)];
+                e.s0 = s0;
+                e.cell = (uint32_t)c;
+                e.row_type = row | (jt << 24);
             }
+            cnt = uni(cnt + (uint32_t)__popcll(dmask));
             row_cur = uni(__shfl(row, (int)(n_valid - 1)));
         }
-        __syncthreads();
-        const uint32_t cnt = *n_entries;
-        if (tid == 0 && cnt)
-            *q_base = (uint32_t)atomicAdd(&a.seg_cur[(size_t)g * kCursorWords + 1], (unsigned long long)cnt);
-        for (uint32_t i = tid; i < rwin_len; i += bdim) {
+        __builtin_amdgcn_wave_barrier();
+        // the segment's entries are contiguous from its first cell's position
+        if (cnt) {
+            unsigned long long qb = 0;
+            if (lane == 0)
+                qb = atomicAdd(&a.seg_cur[(size_t)g * kCursorWords + 1], (unsigned long long)cnt);
+            const uint64_t base = sb + uni64(__shfl(qb, 0));
+            for (uint32_t i = lane; i < cnt; i += kWave)
+                a.queue[base + i] = stage[i];
+        }
+        for (uint32_t i = lane; i < rwin_len; i += kWave) {
             const uint32_t r = rwin[i];
             if (r)
                 atomicAdd(&a.tally[(uint64_t)(row_lo + i / T) * kdim * T + i % T], (unsigned long long)r);
         }
-        __syncthreads();
-        // the segment's entries are contiguous from its first cell's position
-        const uint64_t qb = sb + *q_base;
-        for (uint32_t i = tid; i < cnt; i += bdim)
-            a.queue[qb + i] = stage[i];
-        __syncthreads();  // stage, counters and window are reused by the next chunk
+        __builtin_amdgcn_wave_barrier();  // stage and window are reused by the next unit
+        u = uni((uint32_t)min<uint64_t>((uint64_t)u + n_w, 0xFFFFFFFFull));
     }
     flush_counts(n, a.counters, lane, red);
 }
@@ -829,8 +830,7 @@ __global__ void debug_draws_kernel(uint64_t seed, const uint64_t *cell, const ui
 size_t root_lds_bytes(uint32_t n_types, uint32_t block_size, uint32_t lds_rows)
 {
     return kMathTableBytes + ((size_t)n_types * 28 + 15) / 16 * 16 + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16 +
-           (size_t)(block_size / kWave) * kWave * 4 + 48 +
-           ((size_t)root_win_rows(n_types) * n_types * 4 + 4 + 15) / 16 * 16 + (size_t)kChunk * sizeof(QueueEntry);
+           32 + (size_t)(block_size / kWave) * root_wave_lds_bytes(n_types);
 }
 
 size_t tree_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32_t stack_levels,
