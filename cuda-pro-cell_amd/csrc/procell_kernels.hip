@@ -277,6 +277,12 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
     const dev::Tables tb{m_log, m_sc};
     Counts n = {0, 0, 0, 0, 0};
     const uint32_t n_w = gridDim.x * waves;
+    // the first eight cumulative proportions of the type search (-inf for
+    // types that are not searched: the last one and those beyond T)
+    double cum8[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+        cum8[k] = k + 1 < T ? t_cum[k] : -__builtin_inf();
 
     uint32_t u = blockIdx.x * waves + wave;
     while (u < a.n_units) {
@@ -304,23 +310,17 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             const Words w = dev::draw_block(a.seed, c, 0, 0);
             const double ut = dev::uniform_co(w.w0, 0u);
             const double U = dev::uniform_oc(w.w1, 0u);
-            // first type with ut < cumulative proportion (cell.cu:90-101);
-            // the cumulative proportions are uniform, read eight at a time
-            // through the scalar cache so their latencies overlap
+            // first type with ut < cumulative proportion (cell.cu:90-101):
+            // scanning from the highest index down, the last hit is the
+            // first; the first eight proportions are held in registers
             uint32_t jt = T - 1;
-            bool found = false;
-            for (uint32_t i0 = 0; i0 + 1 < T; i0 += 8) {
-                double cv[8];
-#pragma unroll
-                for (uint32_t k = 0; k < 8; ++k)
-                    cv[k] = i0 + k + 1 < T ? a.type_cum[i0 + k] : 0.0;
-#pragma unroll
-                for (uint32_t k = 0; k < 8; ++k) {
-                    const bool hit = !found && i0 + k + 1 < T && ut < cv[k];
-                    jt = hit ? i0 + k : jt;
-                    found = found || hit;
-                }
+            if (T > 9) {
+                for (uint32_t k = T - 2; k >= 8; --k)
+                    jt = ut < t_cum[k] ? k : jt;
             }
+#pragma unroll
+            for (int k = 7; k >= 0; --k)
+                jt = ut < cum8[k] ? (uint32_t)k : jt;
             const double m = t_mean[jt];
             const double sdv = t_sd[jt];
             const bool quiescent = m < 0.0;
