@@ -403,6 +403,7 @@ int create_plan(const procell_params *p, procell_plan **out)
     // most `claim` (a few prefetch sets) and at least half a set
     a.claim = tu.chunk ? std::min<uint32_t>(tu.chunk, 1u << 20) : 1024;
     a.claim_min = std::min<uint32_t>(a.claim, 32);
+    a.claim_steal = std::min<uint32_t>(a.claim, 64);
     a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 16;
 
     // ---- one device allocation: counters | tally | segment cursors | tables ----

@@ -84,6 +84,7 @@ struct RunArgs {
     uint32_t win_rows;      // rows of the LDS tally window (0: none)
     uint32_t claim;         // largest claim of queue entries
     uint32_t claim_min;     // smallest claim (guided claims shrink down to it)
+    uint32_t claim_steal;   // largest claim from another workgroup's segment
     uint32_t idle_limit;    // idle lanes tolerated before a wavefront refills
     uint32_t rows_in_lds;   // nonzero: the row tables are copied to LDS
     uint32_t n_units;       // root-kernel work units

@@ -50,6 +50,7 @@ using dev::Words;
 
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kNoWindow = 0xFFFFFFFFu;
+constexpr uint32_t kProbes = 32;       // segments one steal scan probes
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
 {
@@ -434,10 +435,9 @@ struct Lane {
 // ======================================================================
 // tree_kernel: workgroup g works through segment g's queue entries.
 // LDS: log and sin/cos tables | stream coefficients | type tables (mean, sd:
-// double [T]; file: u32
-// [T]) | row info when it fits (u32 [n_rows]) | counter sums u32 [8] | window
-// tally u32 [win_rows * kdim * T + 1 spare] | stack double [stack_levels *
-// blockDim]
+// double [T]; file: u32 [T]) | row info when it fits (u32 [n_rows]) |
+// counter sums u32 [8] | victim hint u32 [4] | window tally u32 [win_rows *
+// kdim * T + 1 spare] | stack double [stack_levels * blockDim]
 // ======================================================================
 template <bool RowsLds, typename N>
 __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
@@ -459,7 +459,8 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
     uint32_t *t_file = (uint32_t *)(t_sd + T);
     uint32_t *r_info = (uint32_t *)((unsigned char *)t_mean + ((size_t)T * 20 + 15) / 16 * 16);
     uint32_t *red = (uint32_t *)((unsigned char *)r_info + ((size_t)lds_rows * 4 + 15) / 16 * 16);
-    unsigned int *win = red + kNumCounters;
+    uint32_t *hint = red + kNumCounters;   // last victim segment found in this workgroup, + 1 (0: none)
+    unsigned int *win = hint + 4;
     double *stack = (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
 
     const uint32_t seg0 = blockIdx.x;
@@ -489,8 +490,8 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
         r_info[i] = a.row_info[i];
     for (uint32_t i = tid; i <= win_len; i += bdim)
         win[i] = 0u;  // win[win_len] absorbs the adds of lanes outside the window
-    if (tid < kNumCounters)
-        red[tid] = 0u;
+    if (tid < kNumCounters + 4)
+        red[tid] = 0u;  // counter sums and the victim hint
     __syncthreads();
 
     // the stream's FP64 coefficients, held in vector registers for the whole
@@ -503,9 +504,13 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
     // claim size for a segment with `rest` unclaimed entries: a share of
     // them (guided self-scheduling, so claims shrink towards the end and the
     // wavefronts finish together), between claim_min and claim
+    // Claims from another workgroup's segment are capped at claim_steal:
+    // a lagging segment is drained by several thieves at the end, and a
+    // large share taken by one of them would leave it working alone.
     auto guided = [&](uint64_t rest) {
         const uint32_t share = (rest >> 32 ? 0xFFFFFFFFu : (uint32_t)rest) / (2u * n_waves);
-        return uni(max(a.claim_min, min(a.claim, share)));
+        const uint32_t cap = cur_seg == seg0 ? a.claim : a.claim_steal;
+        return uni(max(min(a.claim_min, cap), min(cap, share)));
     };
     uint32_t n_probe = 0;     // probes so far (varies the probed segments)
     // claimed entries of segment cur_seg not yet loaded: [q_cur, q_end)
@@ -619,22 +624,41 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
                     base = uni64(__shfl(pf_raw, 0));
                     pf_pending = 0;
                 }
-                // the segment is drained: each lane probes one segment (the
-                // 64 probes of a wavefront spread over all segments, shifted
-                // by a hash of the wavefront and the attempt, so concurrent
-                // thieves look at different segments); the probe with the
-                // most entries left becomes the segment to drain.  No
-                // entries left in any probe ends the roots: a segment's own
-                // workgroup always drains it.
+                // the segment is drained: first try the victim another
+                // wavefront of this workgroup found last (one atomic, no
+                // scan); else kProbes lanes probe one segment each (spread
+                // over all segments, shifted by a hash of the wavefront and
+                // the attempt, so concurrent thieves look at different
+                // segments) and the probe with the most entries left
+                // becomes the segment to drain, published as the
+                // workgroup's hint.  No entries left in any probe ends the
+                // roots: a segment's own workgroup always drains it.
+                uint32_t tried = 0;
                 while (base >= cur_total) {
+                    const uint32_t hv = uni(*hint);
+                    if (hv != 0 && hv - 1 != cur_seg && hv != tried) {
+                        tried = hv;
+                        cur_seg = hv - 1;
+                        cur_begin = uni64(a.seg_begin[cur_seg]);
+                        cur_total = uni64(a.seg_cur[(size_t)cur_seg * kCursorWords + 1]);
+                        pf_size = uni(min(a.claim_steal, a.claim));
+                        unsigned long long r = 0;
+                        if (lane == 0)
+                            r = atomicAdd(&a.seg_cur[(size_t)cur_seg * kCursorWords], (unsigned long long)pf_size);
+                        base = uni64(__shfl(r, 0));
+                        continue;
+                    }
                     const uint32_t h = hash32((seg0 * n_waves + wave) * 0x9E3779B9u + n_probe * 0x85EBCA6Bu);
                     n_probe = uni(n_probe + 1);
-                    const uint32_t stride = n_seg > kWave ? n_seg / kWave : 1u;
+                    const uint32_t stride = n_seg > kProbes ? n_seg / kProbes : 1u;
                     const uint32_t cand = (h + lane * stride + (h >> 16) % stride) % n_seg;
-                    const uint64_t tot = a.seg_cur[(size_t)cand * kCursorWords + 1];
-                    const uint64_t used = __hip_atomic_load(&a.seg_cur[(size_t)cand * kCursorWords], __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT);
-                    uint64_t left = used < tot ? tot - used : 0;
+                    uint64_t left = 0;
+                    if (lane < kProbes) {
+                        const uint64_t tot = a.seg_cur[(size_t)cand * kCursorWords + 1];
+                        const uint64_t used = __hip_atomic_load(&a.seg_cur[(size_t)cand * kCursorWords],
+                                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        left = used < tot ? tot - used : 0;
+                    }
                     uint32_t best = cand;
 #pragma unroll
                     for (int off = 32; off > 0; off >>= 1) {
@@ -650,6 +674,8 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
                         break;
                     }
                     cur_seg = uni(best);
+                    if (lane == 0)
+                        *hint = cur_seg + 1;
                     cur_begin = uni64(a.seg_begin[cur_seg]);
                     cur_total = uni64(a.seg_cur[(size_t)cur_seg * kCursorWords + 1]);
                     pf_size = guided(uni64(left));
@@ -837,7 +863,7 @@ size_t tree_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32
                       uint32_t block_size, uint32_t lds_rows)
 {
     const size_t tables = kMathTableBytes + (sizeof(dev::Coeffs) + 15) / 16 * 16 + ((size_t)n_types * 20 + 15) / 16 * 16 +
-                          ((size_t)lds_rows * 4 + 15) / 16 * 16 + kNumCounters * 4;
+                          ((size_t)lds_rows * 4 + 15) / 16 * 16 + (kNumCounters + 4) * 4;
     const size_t win = ((size_t)win_rows * kdim * n_types * 4 + 4 + 15) / 16 * 16;
     const size_t stk = (size_t)stack_levels * block_size * sizeof(double);
     return tables + win + stk;
