@@ -218,7 +218,7 @@ def main():
             "unit": "Tlane-op/s",
             "frac": achieved / VALU_PEAK_LANE_OPS,
             "traffic": traffic,
-            "kernel": "root_kernel + tree_kernel (back to back, timed together)",
+            "kernel": "zero_kernel + root_kernel + tree_kernel (one run, back to back, timed together)",
             "kernel_ms": kernel_ms,
             "work_per_unit": f"{LANE_OPS_PER_DIVISION} lane-ops per division",
             "peak_source": "MI355X FP64 vector peak 78.6 TFLOP/s / 2 = one wave64 VALU instruction "

@@ -163,8 +163,7 @@ struct procell_plan {
     void *d_queue = nullptr;  // queue of dividing initial cells (16 B per initial cell)
     size_t tally_len = 0;
     size_t zero_words = 0;    // counters, tally and segment cursors, cleared per run
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    std::vector<hipEvent_t> kev;   // PROCELL_FLAG_TIMING: start/stop of the tree kernel per run (ring)
+    std::vector<hipEvent_t> kev;   // PROCELL_FLAG_TIMING: start/stop of each run's kernels (ring)
     uint64_t runs = 0;             // runs since the last kernel-time read
     bool ran = false;
 };
@@ -209,9 +208,6 @@ void destroy_plan(procell_plan *pl)
     if (!pl)
         return;
     (void)hipSetDevice(pl->device);
-    for (hipEvent_t &e : pl->ev)
-        if (e)
-            (void)hipEventDestroy(e);
     for (hipEvent_t &e : pl->kev)
         if (e)
             (void)hipEventDestroy(e);
@@ -484,9 +480,6 @@ int create_plan(const procell_params *p, procell_plan **out)
     if (hipStreamSynchronize(pl->stream) != hipSuccess)
         return fail(PROCELL_E_HIP);
     if (pl->timing) {
-        for (hipEvent_t &e : pl->ev)
-            if (hipEventCreate(&e) != hipSuccess)
-                return fail(PROCELL_E_HIP);
         pl->kev.assign(2 * kTimedRuns, nullptr);
         for (hipEvent_t &e : pl->kev)
             if (hipEventCreate(&e) != hipSuccess)
@@ -500,20 +493,16 @@ int run_plan(procell_plan *pl, uint64_t seed)
 {
     HIP_TRY(hipSetDevice(pl->device));
     pl->args.seed = seed;
-    if (pl->timing)
-        HIP_TRY(hipEventRecord(pl->ev[0], pl->stream));
-    HIP_TRY(launch_zero(pl->args.counters, pl->zero_words, pl->stream));
+    // timing events bracket the whole run: an event recorded between two
+    // kernels of the run would add a gap of several microseconds there
     const size_t slot = (size_t)(pl->runs % kTimedRuns);
-    if (pl->timing) {
-        HIP_TRY(hipEventRecord(pl->ev[1], pl->stream));
+    if (pl->timing)
         HIP_TRY(hipEventRecord(pl->kev[2 * slot], pl->stream));
-    }
+    HIP_TRY(launch_zero(pl->args.counters, pl->zero_words, pl->stream));
     HIP_TRY(launch_root(pl->args, pl->shape, pl->stream));
     HIP_TRY(launch_tree(pl->args, pl->shape, pl->stream));
-    if (pl->timing) {
+    if (pl->timing)
         HIP_TRY(hipEventRecord(pl->kev[2 * slot + 1], pl->stream));
-        HIP_TRY(hipEventRecord(pl->ev[2], pl->stream));
-    }
     ++pl->runs;
     pl->ran = true;
     return PROCELL_OK;
@@ -560,9 +549,9 @@ int plan_result(procell_plan *pl, procell_result *r)
     st.window_rows = pl->args.win_rows;
     if (pl->timing) {
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, pl->ev[1], pl->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&ms, pl->kev[2 * ((pl->runs - 1) % kTimedRuns)],
+                                    pl->kev[2 * ((pl->runs - 1) % kTimedRuns) + 1]));
         st.ms_tree = ms;
-        HIP_TRY(hipEventElapsedTime(&ms, pl->ev[0], pl->ev[2]));
         st.ms_run = ms;
     }
     if (r->raw_tally) {

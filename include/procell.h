@@ -96,8 +96,8 @@ typedef struct {
     uint64_t rejected;    /* normal draws rejected because the timer was <= 0 */
     uint64_t cells;       /* initial cells */
     uint64_t pool;        /* initial cells that divided at least once */
-    double ms_tree;       /* device time of the simulation kernels, roots + trees (PROCELL_FLAG_TIMING) */
-    double ms_run;        /* device time of the whole run incl. clearing (PROCELL_FLAG_TIMING) */
+    double ms_tree;       /* device time of the last run's kernels: clearing, roots, trees (PROCELL_FLAG_TIMING) */
+    double ms_run;        /* device time of the whole run incl. clearing (PROCELL_FLAG_TIMING; = ms_tree) */
     uint32_t workgroups;  /* launch shape chosen for the run */
     uint32_t block_size;
     uint32_t lds_bytes;
@@ -146,10 +146,10 @@ int procell_plan_run(procell_plan *plan, uint64_t seed);            /* enqueue o
 int procell_plan_result(procell_plan *plan, procell_result *r);     /* wait, download, merge */
 int procell_plan_destroy(procell_plan *plan);
 void *procell_plan_stream(procell_plan *plan);                       /* hipStream_t */
-/* With PROCELL_FLAG_TIMING: device time (ms) of the simulation kernels (the
- * root kernel and the tree kernel, back to back) of each of the last
- * min(runs, 256, max_runs) runs since the previous call, oldest first, from
- * HIP events recorded on the plan's stream around the two kernels.
+/* With PROCELL_FLAG_TIMING: device time (ms) of the kernels of each of the
+ * last min(runs, 256, max_runs) runs since the previous call (the clearing
+ * kernel, the root kernel and the tree kernel, back to back), oldest first,
+ * from HIP events recorded on the plan's stream around the three kernels.
  * Waits for the stream; resets the run count. */
 int procell_plan_kernel_times(procell_plan *plan, double *ms, uint32_t max_runs, uint32_t *n_runs);
 
