@@ -148,23 +148,33 @@ int build_setup(const procell_params *p, Setup &s, bool want_bins)
 using namespace procell;
 
 // ---------------------------------------------------------------------------
-// Plan: device buffers for one problem, reused across runs.
+// Plan: device buffers for one problem (or a batch of parameter sets), reused
+// across runs.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kSeedSlots = 4;  // pinned host staging slots for batch seeds
+
 struct procell_plan {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     bool timing = false;
-    int32_t track_ratio = 0;
-    Setup setup;
+    std::vector<int32_t> track_ratio;  // per set
+    std::vector<Setup> setups;         // per set
+    std::vector<size_t> tally_base, tally_len;  // per set, in tally entries
     LaunchShape shape{};
     RunArgs args{};
     void *d_block = nullptr;  // one allocation holding every table
     void *d_queue = nullptr;  // queue of dividing initial cells (16 B per initial cell)
-    size_t tally_len = 0;
+    size_t tally_total = 0;
     size_t zero_words = 0;    // counters, tally and segment cursors, cleared per run
+    uint64_t *d_seeds = nullptr;       // [n_sets] seeds of the current batch run (in d_block)
+    uint64_t *h_seeds = nullptr;       // pinned [kSeedSlots][n_sets]
+    hipEvent_t seed_ev[kSeedSlots] = {};
+    hipGraph_t graph = nullptr;        // batch runs: clearing, root and tree kernels
+    hipGraphExec_t graph_exec = nullptr;
     std::vector<hipEvent_t> kev;   // PROCELL_FLAG_TIMING: start/stop of each run's kernels (ring)
     uint64_t runs = 0;             // runs since the last kernel-time read
+    uint64_t batch_runs = 0;
     bool ran = false;
 };
 
@@ -208,9 +218,18 @@ void destroy_plan(procell_plan *pl)
     if (!pl)
         return;
     (void)hipSetDevice(pl->device);
+    if (pl->graph_exec)
+        (void)hipGraphExecDestroy(pl->graph_exec);
+    if (pl->graph)
+        (void)hipGraphDestroy(pl->graph);
     for (hipEvent_t &e : pl->kev)
         if (e)
             (void)hipEventDestroy(e);
+    for (hipEvent_t &e : pl->seed_ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (pl->h_seeds)
+        (void)hipHostFree(pl->h_seeds);
     if (pl->d_block)
         (void)hipFree(pl->d_block);
     if (pl->d_queue)
@@ -289,41 +308,89 @@ void split_segments(const Setup &s, uint32_t g, std::vector<uint64_t> &begin, st
     }
 }
 
-int create_plan(const procell_params *p, procell_plan **out)
+// Expected tree-kernel work of a set (sum of row costs over its cells).
+double set_cost(const Setup &s)
 {
-    if (!p || !out)
+    double w = 0.0;
+    for (uint32_t r = 0; r < (uint32_t)s.v.size(); ++r)
+        w += row_cost(s, r) * (double)(s.start[r + 1] - s.start[r]);
+    return w;
+}
+
+// A plan over n_sets parameter sets (1: a plain run).  Rows, cells, type
+// entries, tally entries and segments of the sets are concatenated in set
+// order; the device code finds a set's values through its SetInfo.
+int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
+{
+    if (!ps || !out || n_sets == 0)
         return PROCELL_E_INVALID;
     *out = nullptr;
-    const procell_tuning &tu = p->tuning;
+    const procell_tuning &tu = ps[0].tuning;
     if (tu.block_size && (tu.block_size % 64 || tu.block_size > 256))
         return PROCELL_E_INVALID;
+    for (uint32_t i = 1; i < n_sets; ++i)
+        if (ps[i].device != ps[0].device)
+            return PROCELL_E_INVALID;
     procell_plan *pl = new (std::nothrow) procell_plan();
     if (!pl)
         return PROCELL_E_OUT_OF_MEMORY;
-    int rc = build_setup(p, pl->setup, true);
-    if (rc != PROCELL_OK) {
-        delete pl;
-        return rc;
+    pl->setups.resize(n_sets);
+    pl->track_ratio.resize(n_sets);
+    for (uint32_t i = 0; i < n_sets; ++i) {
+        const int rc = build_setup(&ps[i], pl->setups[i], true);
+        if (rc != PROCELL_OK) {
+            delete pl;
+            return rc;
+        }
+        pl->track_ratio[i] = ps[i].track_ratio;
     }
     int n_cu = 0;
     size_t lds_per_cu = 0, lds_per_block = 0;
-    rc = check_device(p->device, &n_cu, &lds_per_cu, &lds_per_block);
-    if (rc != PROCELL_OK) {
-        delete pl;
-        return rc;
+    {
+        const int rc = check_device(ps[0].device, &n_cu, &lds_per_cu, &lds_per_block);
+        if (rc != PROCELL_OK) {
+            delete pl;
+            return rc;
+        }
     }
-    pl->device = p->device;
+    pl->device = ps[0].device;
     pl->timing = (tu.flags & PROCELL_FLAG_TIMING) != 0;
-    pl->track_ratio = p->track_ratio;
-    const Setup &s = pl->setup;
-    const uint32_t n_rows = (uint32_t)s.v.size();
-    const uint32_t T = s.n_types ? s.n_types : 1;
-    const uint32_t kdim = (uint32_t)s.kmax + 1;
-    pl->tally_len = (size_t)n_rows * kdim * T;
     auto fail = [&](int code) {
         destroy_plan(pl);
         return code;
     };
+
+    // ---- concatenation of the sets ----
+    std::vector<SetInfo> info(n_sets);
+    uint64_t n_cells = 0, n_rows64 = 0, n_types_tot = 0;
+    uint32_t T = 1, kdim = 1;
+    pl->tally_base.resize(n_sets);
+    pl->tally_len.resize(n_sets);
+    for (uint32_t i = 0; i < n_sets; ++i) {
+        const Setup &s = pl->setups[i];
+        const uint32_t Ti = s.n_types ? s.n_types : 1;
+        const uint32_t ki = (uint32_t)s.kmax + 1;
+        SetInfo &si = info[i];
+        memset(&si, 0, sizeof si);
+        si.t_max = s.t_max;
+        si.cell_base = n_cells;
+        si.tally_base = pl->tally_total;
+        si.kdim = ki;
+        si.n_types = Ti;
+        si.type_base = (uint32_t)n_types_tot;
+        si.row_base = (uint32_t)n_rows64;
+        pl->tally_base[i] = pl->tally_total;
+        pl->tally_len[i] = (size_t)s.v.size() * ki * Ti;
+        pl->tally_total += pl->tally_len[i];
+        n_cells += s.n_cells;
+        n_rows64 += s.v.size();
+        n_types_tot += Ti;
+        T = std::max(T, Ti);
+        kdim = std::max(kdim, ki);
+    }
+    if (n_cells > kMaxCells || n_rows64 > kMaxRows)
+        return fail(PROCELL_E_TOO_MANY_CELLS);
+    const uint32_t n_rows = (uint32_t)n_rows64;
 
     // ---- launch shape: block size, resident workgroups, LDS split ----
     LaunchShape &ls = pl->shape;
@@ -356,25 +423,64 @@ int create_plan(const procell_params *p, procell_plan **out)
     const size_t budget = std::min(lds_per_cu / (size_t)occ, lds_per_block);
     const size_t row_bytes = (size_t)kdim * T * 4;
     uint32_t win_rows = budget > base + 16 ? (uint32_t)((budget - base - 16) / row_bytes) : 0;
-    // segments: one per resident workgroup, at most kMaxSegments (the tree
-    // kernel's per-wavefront victim masks hold 32 x 64 segments)
-    const uint32_t g = (uint32_t)std::min<int64_t>((int64_t)n_cu * occ, kMaxSegments);
-    std::vector<uint64_t> seg_begin;
-    std::vector<uint32_t> seg_row;
-    split_segments(s, g, seg_begin, seg_row);
+
+    // ---- segments: one per tree-kernel workgroup, each within one set ----
+    // A single set gets one segment per resident workgroup; in a batch each
+    // set gets a share of the resident workgroups proportional to its
+    // expected work (at least one), and workgroups beyond the resident ones
+    // start as others finish.
+    const uint32_t resident = (uint32_t)std::min<int64_t>((int64_t)n_cu * occ, kMaxSegments);
+    std::vector<uint32_t> g_set(n_sets, 1);
+    if (n_sets == 1) {
+        g_set[0] = resident;
+    } else {
+        std::vector<double> w(n_sets);
+        double wt = 0.0;
+        for (uint32_t i = 0; i < n_sets; ++i) {
+            w[i] = set_cost(pl->setups[i]);
+            wt += w[i];
+        }
+        for (uint32_t i = 0; i < n_sets; ++i)
+            g_set[i] = wt > 0.0 ? (uint32_t)std::max(1.0, std::floor(w[i] / wt * resident + 0.5)) : 1u;
+    }
+    std::vector<uint64_t> seg_begin(1, 0);
+    std::vector<uint32_t> seg_row, seg_set;
+    for (uint32_t i = 0; i < n_sets; ++i) {
+        const Setup &s = pl->setups[i];
+        std::vector<uint64_t> b;
+        std::vector<uint32_t> r;
+        split_segments(s, g_set[i], b, r);
+        info[i].seg_lo = (uint32_t)seg_row.size();
+        for (uint32_t k = 0; k < g_set[i]; ++k) {
+            seg_begin.push_back(info[i].cell_base + b[k + 1]);
+            seg_row.push_back(info[i].row_base + r[k]);
+            seg_set.push_back(i);
+        }
+        info[i].seg_hi = (uint32_t)seg_row.size();
+    }
+    const uint32_t g = (uint32_t)seg_row.size();
     // no more window rows than a segment spans
     uint32_t span = 1;
-    for (uint32_t i = 0; i < g; ++i) {
-        const uint64_t last = seg_begin[i + 1] > seg_begin[i] ? seg_begin[i + 1] - 1 : seg_begin[i];
-        const uint64_t lc = last < s.n_cells ? last : (s.n_cells ? s.n_cells - 1 : 0);
-        const uint32_t lr = (uint32_t)(std::upper_bound(s.start.begin(), s.start.end() - 1, lc) - s.start.begin()) - 1;
-        span = std::max(span, lr - seg_row[i] + 1);
+    for (uint32_t k = 0; k < g; ++k) {
+        const Setup &s = pl->setups[seg_set[k]];
+        const SetInfo &si = info[seg_set[k]];
+        if (!s.n_cells)
+            continue;
+        const uint64_t first = seg_begin[k] - si.cell_base;
+        const uint64_t end = seg_begin[k + 1] - si.cell_base;
+        const uint64_t lc = end > first ? end - 1 : first;
+        const uint64_t lcc = lc < s.n_cells ? lc : s.n_cells - 1;
+        const uint32_t lr = (uint32_t)(std::upper_bound(s.start.begin(), s.start.end() - 1, lcc) - s.start.begin()) - 1;
+        span = std::max(span, lr - (seg_row[k] - si.row_base) + 1);
     }
     win_rows = std::min(win_rows, span);
     // u32 window counters must not overflow: bound the leaves any window can see
-    for (uint32_t i = 0; i < g && win_rows; ++i) {
+    for (uint32_t k = 0; k < g && win_rows; ++k) {
+        const Setup &s = pl->setups[seg_set[k]];
+        const uint32_t r0 = seg_row[k] - info[seg_set[k]].row_base;
+        const uint32_t nr = (uint32_t)s.v.size();
         double bound = 0.0;
-        for (uint32_t r = seg_row[i]; r < std::min<uint32_t>(n_rows, seg_row[i] + win_rows); ++r)
+        for (uint32_t r = r0; r < std::min<uint32_t>(nr, r0 + win_rows); ++r)
             bound += (double)(s.start[r + 1] - s.start[r]) * ldexp(1.0, (int)(s.row_info[r] & 0xFFu));
         if (bound >= 4.0e9)
             win_rows = 0;
@@ -404,8 +510,8 @@ int create_plan(const procell_params *p, procell_plan **out)
 
     // ---- one device allocation: counters | tally | segment cursors | tables ----
     size_t off = 0;
-    const size_t o_cnt = off; off += (size_t)kCounterCopies * kNumCounters * 8;
-    const size_t o_tally = off; off += pl->tally_len * 8;
+    const size_t o_cnt = off; off += (size_t)n_sets * kCounterCopies * kNumCounters * 8;
+    const size_t o_tally = off; off += pl->tally_total * 8;
     off = align_up(off, 256);
     const size_t o_cur = off; off += (size_t)g * kCursorWords * 8;
     pl->zero_words = off / 8;
@@ -425,19 +531,22 @@ int create_plan(const procell_params *p, procell_plan **out)
         int occ_r = 0;
         if (root_occupancy(256, ls.root_lds_bytes, lds_rows != 0, &occ_r) != hipSuccess)
             return fail(PROCELL_E_HIP);
-        const uint64_t resident = (uint64_t)n_cu * (uint64_t)std::max(occ_r, 1);
-        ls.root_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((a.n_units + 3) / 4, resident));
+        const uint64_t res_r = (uint64_t)n_cu * (uint64_t)std::max(occ_r, 1);
+        ls.root_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((a.n_units + 3) / 4, res_r));
     }
     const size_t o_rs = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
     const size_t o_ri = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
-    const size_t o_cum = off; off = align_up(off + (size_t)T * 8, 256);
-    const size_t o_mean = off; off = align_up(off + (size_t)T * 8, 256);
-    const size_t o_sd = off; off = align_up(off + (size_t)T * 8, 256);
-    const size_t o_tf = off; off = align_up(off + (size_t)T * 4, 256);
+    const size_t o_cum = off; off = align_up(off + (size_t)n_types_tot * 8, 256);
+    const size_t o_mean = off; off = align_up(off + (size_t)n_types_tot * 8, 256);
+    const size_t o_sd = off; off = align_up(off + (size_t)n_types_tot * 8, 256);
+    const size_t o_tf = off; off = align_up(off + (size_t)n_types_tot * 4, 256);
     const size_t o_sb = off; off = align_up(off + (size_t)(g + 1) * 8, 256);
     const size_t o_sr = off; off = align_up(off + (size_t)g * 4, 256);
-    if (p->stream) {
-        pl->stream = (hipStream_t)p->stream;
+    const size_t o_ss = off; off = align_up(off + (size_t)g * 4, 256);
+    const size_t o_info = off; off = align_up(off + (size_t)n_sets * sizeof(SetInfo), 256);
+    const size_t o_seed = off; off = align_up(off + (size_t)n_sets * 8, 256);
+    if (ps[0].stream) {
+        pl->stream = (hipStream_t)ps[0].stream;
     } else {
         if (hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) != hipSuccess)
             return fail(PROCELL_E_HIP);
@@ -445,11 +554,11 @@ int create_plan(const procell_params *p, procell_plan **out)
     }
     if (hipMalloc(&pl->d_block, off) != hipSuccess)
         return fail(PROCELL_E_OUT_OF_MEMORY);
-    if (s.n_cells && hipMalloc(&pl->d_queue, (size_t)s.n_cells * sizeof(QueueEntry)) != hipSuccess)
+    if (n_cells && hipMalloc(&pl->d_queue, (size_t)n_cells * sizeof(QueueEntry)) != hipSuccess)
         return fail(PROCELL_E_OUT_OF_MEMORY);
     char *dbase = (char *)pl->d_block;
-    a.t_max = s.t_max;
-    a.n_cells = s.n_cells;
+    a.n_sets = n_sets;
+    a.n_cells = n_cells;
     a.n_rows = n_rows;
     a.n_types = T;
     a.kdim = kdim;
@@ -467,14 +576,39 @@ int create_plan(const procell_params *p, procell_plan **out)
     a.type_file = (const uint32_t *)(dbase + o_tf);
     a.seg_begin = (const uint64_t *)(dbase + o_sb);
     a.seg_row = (const uint32_t *)(dbase + o_sr);
+    a.seg_set = (const uint32_t *)(dbase + o_ss);
+    a.sets = (const SetInfo *)(dbase + o_info);
+    a.set_seed = (const uint64_t *)(dbase + o_seed);
+    pl->d_seeds = (uint64_t *)(dbase + o_seed);
+    // concatenated host tables (cells < 2^32)
+    std::vector<uint32_t> start32, info32;
+    std::vector<double> cum, mean, sd;
+    std::vector<uint32_t> tfile;
+    start32.reserve(n_rows + 1);
+    for (uint32_t i = 0; i < n_sets; ++i) {
+        const Setup &s = pl->setups[i];
+        for (size_t r = 0; r < s.v.size(); ++r) {
+            start32.push_back((uint32_t)(info[i].cell_base + s.start[r]));
+            info32.push_back(s.row_info[r]);
+        }
+        const uint32_t Ti = info[i].n_types;
+        for (uint32_t j = 0; j < Ti; ++j) {
+            cum.push_back(j < s.type_cum.size() ? s.type_cum[j] : 1.0);
+            mean.push_back(j < s.type_mean.size() ? s.type_mean[j] : -1.0);
+            sd.push_back(j < s.type_sd.size() ? s.type_sd[j] : -1.0);
+            tfile.push_back(j < s.type_file.size() ? s.type_file[j] : 0u);
+        }
+    }
+    start32.push_back((uint32_t)n_cells);
+    info32.push_back(0u);
     auto up = [&](size_t o, const void *src, size_t bytes) {
         return bytes == 0 || hipMemcpyAsync(dbase + o, src, bytes, hipMemcpyHostToDevice, pl->stream) == hipSuccess;
     };
-    std::vector<uint32_t> start32(s.start.begin(), s.start.end());  // n_cells < 2^32
-    if (!up(o_rs, start32.data(), start32.size() * 4) || !up(o_ri, s.row_info.data(), n_rows * 4) ||
-        !up(o_cum, s.type_cum.data(), s.type_cum.size() * 8) || !up(o_mean, s.type_mean.data(), s.type_mean.size() * 8) ||
-        !up(o_sd, s.type_sd.data(), s.type_sd.size() * 8) || !up(o_tf, s.type_file.data(), s.type_file.size() * 4) ||
+    if (!up(o_rs, start32.data(), start32.size() * 4) || !up(o_ri, info32.data(), info32.size() * 4) ||
+        !up(o_cum, cum.data(), cum.size() * 8) || !up(o_mean, mean.data(), mean.size() * 8) ||
+        !up(o_sd, sd.data(), sd.size() * 8) || !up(o_tf, tfile.data(), tfile.size() * 4) ||
         !up(o_sb, seg_begin.data(), seg_begin.size() * 8) || !up(o_sr, seg_row.data(), seg_row.size() * 4) ||
+        !up(o_ss, seg_set.data(), seg_set.size() * 4) || !up(o_info, info.data(), info.size() * sizeof(SetInfo)) ||
         !up(o_useg, unit_seg.data(), unit_seg.size() * 4) || !up(o_ufirst, unit_first.data(), unit_first.size() * 4))
         return fail(PROCELL_E_HIP);
     if (hipStreamSynchronize(pl->stream) != hipSuccess)
@@ -485,22 +619,96 @@ int create_plan(const procell_params *p, procell_plan **out)
             if (hipEventCreate(&e) != hipSuccess)
                 return fail(PROCELL_E_HIP);
     }
+    if (n_sets > 1) {
+        if (hipHostMalloc((void **)&pl->h_seeds, (size_t)kSeedSlots * n_sets * 8) != hipSuccess)
+            return fail(PROCELL_E_OUT_OF_MEMORY);
+        for (hipEvent_t &e : pl->seed_ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                return fail(PROCELL_E_HIP);
+    }
     *out = pl;
     return PROCELL_OK;
 }
 
-int run_plan(procell_plan *pl, uint64_t seed)
+// The run's kernels: clearing, roots, trees.
+hipError_t launch_run(procell_plan *pl)
+{
+    hipError_t e = launch_zero(pl->args.counters, pl->zero_words, pl->stream);
+    if (e == hipSuccess)
+        e = launch_root(pl->args, pl->shape, pl->stream);
+    if (e == hipSuccess)
+        e = launch_tree(pl->args, pl->shape, pl->stream);
+    return e;
+}
+
+// Batch runs replay one graph of the three kernels; it is captured on the
+// first run (the kernels read the seeds from device memory, so the graph
+// does not change between runs) and again after the launch knobs change.
+int capture_graph(procell_plan *pl)
+{
+    if (pl->graph_exec)
+        return PROCELL_OK;
+    HIP_TRY(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = launch_run(pl);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(pl->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        g_last_hip = e != hipSuccess ? e : e2;
+        return PROCELL_E_HIP;
+    }
+    pl->graph = graph;
+    HIP_TRY(hipGraphInstantiate(&pl->graph_exec, pl->graph, nullptr, nullptr, 0));
+    return PROCELL_OK;
+}
+
+void drop_graph(procell_plan *pl)
+{
+    if (pl->graph_exec)
+        (void)hipGraphExecDestroy(pl->graph_exec);
+    if (pl->graph)
+        (void)hipGraphDestroy(pl->graph);
+    pl->graph_exec = nullptr;
+    pl->graph = nullptr;
+}
+
+// One run: every set gets seeds[i] (or `seed` when seeds is null).
+int run_plan(procell_plan *pl, const uint64_t *seeds, uint64_t seed, bool use_graph = true)
 {
     HIP_TRY(hipSetDevice(pl->device));
-    pl->args.seed = seed;
+    const uint32_t n_sets = pl->args.n_sets;
     // timing events bracket the whole run: an event recorded between two
     // kernels of the run would add a gap of several microseconds there
     const size_t slot = (size_t)(pl->runs % kTimedRuns);
-    if (pl->timing)
-        HIP_TRY(hipEventRecord(pl->kev[2 * slot], pl->stream));
-    HIP_TRY(launch_zero(pl->args.counters, pl->zero_words, pl->stream));
-    HIP_TRY(launch_root(pl->args, pl->shape, pl->stream));
-    HIP_TRY(launch_tree(pl->args, pl->shape, pl->stream));
+    if (n_sets == 1) {
+        pl->args.seed = seeds ? seeds[0] : seed;
+        if (pl->timing)
+            HIP_TRY(hipEventRecord(pl->kev[2 * slot], pl->stream));
+        HIP_TRY(launch_run(pl));
+    } else {
+        // stage the seeds in a pinned slot whose previous copy has run, and
+        // copy them to the device ahead of the kernels
+        const uint32_t k = (uint32_t)(pl->batch_runs % kSeedSlots);
+        HIP_TRY(hipEventSynchronize(pl->seed_ev[k]));
+        uint64_t *h = pl->h_seeds + (size_t)k * n_sets;
+        for (uint32_t i = 0; i < n_sets; ++i)
+            h[i] = seeds ? seeds[i] : seed;
+        HIP_TRY(hipMemcpyAsync(pl->d_seeds, h, (size_t)n_sets * 8, hipMemcpyHostToDevice, pl->stream));
+        HIP_TRY(hipEventRecord(pl->seed_ev[k], pl->stream));
+        ++pl->batch_runs;
+        if (use_graph) {
+            const int rc = capture_graph(pl);
+            if (rc != PROCELL_OK)
+                return rc;
+        }
+        if (pl->timing)
+            HIP_TRY(hipEventRecord(pl->kev[2 * slot], pl->stream));
+        if (use_graph)
+            HIP_TRY(hipGraphLaunch(pl->graph_exec, pl->stream));
+        else
+            HIP_TRY(launch_run(pl));
+    }
     if (pl->timing)
         HIP_TRY(hipEventRecord(pl->kev[2 * slot + 1], pl->stream));
     ++pl->runs;
@@ -508,24 +716,26 @@ int run_plan(procell_plan *pl, uint64_t seed)
     return PROCELL_OK;
 }
 
-int plan_result(procell_plan *pl, procell_result *r)
+int plan_result(procell_plan *pl, uint32_t set, procell_result *r)
 {
-    if (!r || !pl->ran)
+    if (!r || !pl->ran || set >= pl->args.n_sets)
         return PROCELL_E_INVALID;
-    const Setup &s = pl->setup;
-    const uint32_t T = pl->args.n_types;
-    const uint32_t kdim = pl->args.kdim;
+    const Setup &s = pl->setups[set];
+    const uint32_t T = s.n_types ? s.n_types : 1;
+    const uint32_t kdim = (uint32_t)s.kmax + 1;
+    const size_t tally_len = pl->tally_len[set];
     const uint64_t n_rows = s.v.size();
     const uint64_t n_bins = s.bins.size();
     if (r->capacity && !(r->value && r->count))
         return PROCELL_E_INVALID;
     HIP_TRY(hipSetDevice(pl->device));
     std::vector<unsigned long long> cnt_all((size_t)kCounterCopies * kNumCounters), cnt(kNumCounters, 0);
-    std::vector<int64_t> tally(pl->tally_len);
-    HIP_TRY(hipMemcpyAsync(cnt_all.data(), pl->args.counters, cnt_all.size() * 8, hipMemcpyDeviceToHost,
-                           pl->stream));
-    if (pl->tally_len)
-        HIP_TRY(hipMemcpyAsync(tally.data(), pl->args.tally, pl->tally_len * 8, hipMemcpyDeviceToHost, pl->stream));
+    std::vector<int64_t> tally(tally_len);
+    HIP_TRY(hipMemcpyAsync(cnt_all.data(), pl->args.counters + (size_t)set * kCounterCopies * kNumCounters,
+                           cnt_all.size() * 8, hipMemcpyDeviceToHost, pl->stream));
+    if (tally_len)
+        HIP_TRY(hipMemcpyAsync(tally.data(), pl->args.tally + pl->tally_base[set], tally_len * 8,
+                               hipMemcpyDeviceToHost, pl->stream));
     HIP_TRY(hipStreamSynchronize(pl->stream));
     for (size_t i = 0; i < cnt_all.size(); ++i)
         cnt[i % kNumCounters] += cnt_all[i];
@@ -535,7 +745,7 @@ int plan_result(procell_plan *pl, procell_result *r)
     // recorded = tally sum; dropped from the event balance (every division
     // adds one cell; every cell ends recorded, dropped or lost)
     uint64_t recorded = 0;
-    for (size_t i = 0; i < pl->tally_len; ++i)
+    for (size_t i = 0; i < tally_len; ++i)
         recorded += (uint64_t)tally[i];
     st.recorded = recorded;
     st.dropped = cnt[kCells] + cnt[kDivisions] - recorded - cnt[kLost];
@@ -555,9 +765,9 @@ int plan_result(procell_plan *pl, procell_result *r)
         st.ms_run = ms;
     }
     if (r->raw_tally) {
-        if (r->raw_tally_len < pl->tally_len)
+        if (r->raw_tally_len < tally_len)
             return PROCELL_E_CAPACITY;
-        memcpy(r->raw_tally, tally.data(), pl->tally_len * 8);
+        memcpy(r->raw_tally, tally.data(), tally_len * 8);
     }
     // (row, generation) -> output value row_value * 2^-generation, exact
     // equality merge into the bin set (proliferation.cu:353-380 searches the
@@ -660,20 +870,107 @@ int procell_resolve_phi(const procell_params *p, double *phi)
     return PROCELL_OK;
 }
 
-int procell_plan_create(const procell_params *p, procell_plan **plan) { return create_plan(p, plan); }
+int procell_plan_create(const procell_params *p, procell_plan **plan) { return create_plan(p, 1, plan); }
 
 int procell_plan_run(procell_plan *plan, uint64_t seed)
 {
     if (!plan)
         return PROCELL_E_INVALID;
-    return run_plan(plan, seed);
+    return run_plan(plan, nullptr, seed);
 }
 
 int procell_plan_result(procell_plan *plan, procell_result *r)
 {
     if (!plan)
         return PROCELL_E_INVALID;
-    return plan_result(plan, r);
+    return plan_result(plan, 0, r);
+}
+
+int procell_batch_create(const procell_params *sets, uint32_t n_sets, procell_plan **plan)
+{
+    return create_plan(sets, n_sets, plan);
+}
+
+int procell_batch_run(procell_plan *plan, const uint64_t *seeds)
+{
+    if (!plan || !seeds)
+        return PROCELL_E_INVALID;
+    return run_plan(plan, seeds, 0);
+}
+
+int procell_batch_result(procell_plan *plan, uint32_t set, procell_result *r)
+{
+    if (!plan)
+        return PROCELL_E_INVALID;
+    return plan_result(plan, set, r);
+}
+
+int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t reps, procell_tuning *chosen)
+{
+    if (!plan || !seeds || !chosen || reps == 0)
+        return PROCELL_E_INVALID;
+    HIP_TRY(hipSetDevice(plan->device));
+    static const uint32_t kClaims[] = {256, 512, 1024, 2048};
+    static const uint32_t kIdle[] = {8, 16, 24};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return PROCELL_E_HIP;
+    }
+    RunArgs &a = plan->args;
+    const RunArgs keep = a;
+    float best = -1.f;
+    uint32_t best_claim = a.claim, best_idle = a.idle_limit;
+    int rc = PROCELL_OK;
+    const bool timing = plan->timing;
+    plan->timing = false;
+    drop_graph(plan);
+    for (uint32_t c : kClaims) {
+        for (uint32_t idle : kIdle) {
+            a.claim = c;
+            a.claim_min = std::min<uint32_t>(c, 32);
+            a.claim_steal = std::min<uint32_t>(c, 64);
+            a.idle_limit = idle;
+            rc = run_plan(plan, seeds, 0, false);  // warm-up
+            for (uint32_t i = 0; rc == PROCELL_OK && i < reps; ++i) {
+                if (i == 0 && hipEventRecord(e0, plan->stream) != hipSuccess)
+                    rc = PROCELL_E_HIP;
+                if (rc == PROCELL_OK)
+                    rc = run_plan(plan, seeds, 0, false);
+            }
+            float ms = 0.f;
+            if (rc == PROCELL_OK &&
+                (hipEventRecord(e1, plan->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = PROCELL_E_HIP;
+            if (rc != PROCELL_OK)
+                break;
+            if (best < 0.f || ms < best) {
+                best = ms;
+                best_claim = c;
+                best_idle = idle;
+            }
+        }
+        if (rc != PROCELL_OK)
+            break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    plan->timing = timing;
+    a = keep;
+    if (rc != PROCELL_OK)
+        return rc;
+    a.claim = best_claim;
+    a.claim_min = std::min<uint32_t>(best_claim, 32);
+    a.claim_steal = std::min<uint32_t>(best_claim, 64);
+    a.idle_limit = best_idle;
+    plan->runs = 0;
+    memset(chosen, 0, sizeof *chosen);
+    chosen->block_size = (uint32_t)plan->shape.tree_block_size;
+    chosen->chunk = best_claim;
+    chosen->idle_lanes = best_idle;
+    return PROCELL_OK;
 }
 
 int procell_plan_destroy(procell_plan *plan)
@@ -712,12 +1009,12 @@ int procell_simulate(const procell_params *p, procell_result *r)
     procell_params q = *p;
     q.stream = nullptr;
     procell_plan *pl = nullptr;
-    int rc = create_plan(&q, &pl);
+    int rc = create_plan(&q, 1, &pl);
     if (rc != PROCELL_OK)
         return rc;
-    rc = run_plan(pl, p->seed);
+    rc = run_plan(pl, nullptr, p->seed);
     if (rc == PROCELL_OK)
-        rc = plan_result(pl, r);
+        rc = plan_result(pl, 0, r);
     destroy_plan(pl);
     return rc;
 }

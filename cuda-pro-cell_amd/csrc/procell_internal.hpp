@@ -21,11 +21,31 @@ constexpr uint32_t kUnit = 256;     // initial cells per root-kernel work unit (
 constexpr uint32_t kMaxSegments = 2048;  // tree-kernel workgroups (= segments) at most
 
 // Rows of a root-kernel wavefront's generation-0 tally window (LDS, at most
-// 2 KB; a unit of kUnit cells mostly lies in one or two rows).
+// 2 KB; a unit of kUnit cells mostly lies in one or two rows), and the
+// window capacity that covers every type count up to n_types.
 __host__ __device__ constexpr uint32_t root_win_rows(uint32_t n_types)
 {
     return n_types <= 64 ? 8u : 512u / n_types;
 }
+__host__ __device__ constexpr uint32_t root_win_cap(uint32_t max_types)
+{
+    return max_types <= 64 ? 8u * max_types : 512u;
+}
+
+// Per-parameter-set values of a plan: one set for a plain run, many for a
+// batch.  Rows, initial cells, type-table entries, tally entries and
+// segments of all sets are concatenated in set order.
+struct SetInfo {
+    double t_max;
+    uint64_t cell_base;   // first initial cell of the set (cell ids of the stream are relative to it)
+    uint64_t tally_base;  // tally index of the set's first nonzero row
+    uint32_t kdim;        // generations + 1
+    uint32_t n_types;
+    uint32_t type_base;   // first entry of the set in the type tables
+    uint32_t row_base;    // first nonzero row of the set
+    uint32_t seg_lo, seg_hi;  // the set's segments (tree-kernel workgroups)
+    uint32_t pad[2];
+};
 
 // A dividing initial cell, written by the root kernel for the tree kernel.
 struct QueueEntry {
@@ -74,12 +94,12 @@ int build_setup(const procell_params *p, Setup &s, bool want_bins);
 
 // Arguments of one run, passed by value to the kernel.
 struct RunArgs {
-    uint64_t seed;
-    double t_max;
+    uint64_t seed;          // seed of a single-set run (n_sets == 1)
     uint64_t n_cells;
+    uint32_t n_sets;
     uint32_t n_rows;
-    uint32_t n_types;
-    uint32_t kdim;          // generations + 1
+    uint32_t n_types;       // most types of any set
+    uint32_t kdim;          // most generations + 1 of any set
     uint32_t stack_levels;  // per-lane LDS stack depth (levels 1 .. kdim-1)
     uint32_t win_rows;      // rows of the LDS tally window (0: none)
     uint32_t claim;         // largest claim of queue entries
@@ -88,6 +108,9 @@ struct RunArgs {
     uint32_t idle_limit;    // idle lanes tolerated before a wavefront refills
     uint32_t rows_in_lds;   // nonzero: the row tables are copied to LDS
     uint32_t n_units;       // root-kernel work units
+    const SetInfo *sets;         // [n_sets]
+    const uint64_t *set_seed;    // [n_sets] seeds of a batch run (n_sets > 1)
+    const uint32_t *seg_set;     // [segments] set of each segment
     const uint32_t *row_start;   // [n_rows + 1] first cell of each nonzero row (cells < 2^32)
     const uint32_t *row_info;    // [n_rows]: generation cap | (value >= phi) << 8
     const double *type_cum;      // [n_types] sorted order
@@ -102,8 +125,8 @@ struct RunArgs {
     const uint32_t *unit_first;  // [segments + 1] first unit of each segment
     QueueEntry *queue;           // [n_cells] dividing initial cells; segment g's entries
                                  // start at seg_begin[g]
-    unsigned long long *tally;   // [n_rows][kdim][n_types] file type order
-    unsigned long long *counters;  // [kCounterCopies][kNumCounters]
+    unsigned long long *tally;   // per set [rows][kdim][n_types] of the set, file type order
+    unsigned long long *counters;  // [n_sets][kCounterCopies][kNumCounters]
 };
 
 struct LaunchShape {

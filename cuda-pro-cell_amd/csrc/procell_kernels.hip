@@ -223,13 +223,38 @@ __device__ __forceinline__ void flush_counts(const Counts &n, unsigned long long
 // flags u32 [64], generation-0 tally window u32 [root_win_rows(T) * T + 1
 // spare], staged entries QueueEntry [kUnit]
 // ======================================================================
-__host__ __device__ constexpr size_t root_wave_lds_bytes(uint32_t n_types)
+__host__ __device__ constexpr size_t root_wave_lds_bytes(uint32_t max_types)
 {
-    return kWave * 4 + ((size_t)root_win_rows(n_types) * n_types * 4 + 4 + 15) / 16 * 16 +
-           (size_t)kUnit * sizeof(QueueEntry);
+    return kWave * 4 + ((size_t)root_win_cap(max_types) * 4 + 4 + 15) / 16 * 16 + (size_t)kUnit * sizeof(QueueEntry);
 }
 
-template <bool RowsLds>
+// Adds a wavefront's counts to counter copy `copy` of set `set` and clears
+// them (batched runs: a wavefront's units belong to different sets).
+__device__ __forceinline__ void flush_counts_wave(Counts &n, unsigned long long *counters, uint32_t set,
+                                                  uint32_t copy, uint32_t lane)
+{
+    constexpr uint32_t kN = 5;
+    const uint32_t v[kN] = {wave_sum(n.pool), wave_sum(n.div), wave_sum(n.lost), wave_sum(n.rej),
+                            wave_sum(n.cells)};
+    const uint32_t idx[kN] = {kPoolCount, kDivisions, kLost, kRejected, kCells};
+    unsigned long long *c = counters + ((size_t)set * kCounterCopies + copy) * kNumCounters;
+    if (lane < kN) {
+        uint32_t x = v[0], ci = idx[0];
+#pragma unroll
+        for (uint32_t i = 1; i < kN; ++i) {
+            x = lane == i ? v[i] : x;
+            ci = lane == i ? idx[i] : ci;
+        }
+        if (x)
+            atomicAdd(&c[ci], (unsigned long long)x);
+    }
+    n = Counts{0, 0, 0, 0, 0};
+}
+
+// Batched: the units of one wavefront belong to different parameter sets;
+// the set's values are loaded when it changes, the type tables are read
+// from device memory and counts are flushed per unit.
+template <bool RowsLds, bool Batched>
 __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -238,27 +263,32 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
     const uint32_t lane = tid & (kWave - 1);
     const uint32_t wave = tid / kWave;
     const uint32_t waves = bdim / kWave;
-    const uint32_t T = a.n_types;
-    const uint32_t kdim = a.kdim;
+    const uint32_t Tl = a.n_types;  // LDS layout: the most types of any set
     const uint32_t lds_rows = RowsLds ? a.n_rows : 0u;
-    const uint32_t rwin_rows = root_win_rows(T);
-    const uint32_t rwin_len = rwin_rows * T;
     double(*m_log)[2] = (double(*)[2])lds;            // [128] log table
     double(*m_sc)[2] = (double(*)[2])(lds + 128 * 16);  // [256] sin/cos table
     double *t_cum = (double *)(lds + kMathTableBytes);
-    double *t_mean = t_cum + T;
-    double *t_sd = t_mean + T;
-    uint32_t *t_file = (uint32_t *)(t_sd + T);
-    uint32_t *r_start = (uint32_t *)(lds + kMathTableBytes + ((size_t)T * 28 + 15) / 16 * 16);
+    double *t_mean = t_cum + Tl;
+    double *t_sd = t_mean + Tl;
+    uint32_t *t_file = (uint32_t *)(t_sd + Tl);
+    uint32_t *r_start = (uint32_t *)(lds + kMathTableBytes + ((size_t)Tl * 28 + 15) / 16 * 16);
     uint32_t *r_info = r_start + lds_rows + 1;
     uint32_t *red = (uint32_t *)((unsigned char *)r_start + ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16);
-    unsigned char *wlds = (unsigned char *)red + 32 + (size_t)wave * root_wave_lds_bytes(T);
+    unsigned char *wlds = (unsigned char *)red + 32 + (size_t)wave * root_wave_lds_bytes(Tl);
     uint32_t *flags = (uint32_t *)wlds;
     uint32_t *rwin = flags + kWave;
-    QueueEntry *stage = (QueueEntry *)(wlds + kWave * 4 + ((size_t)rwin_len * 4 + 4 + 15) / 16 * 16);
+    QueueEntry *stage = (QueueEntry *)(wlds + kWave * 4 + ((size_t)root_win_cap(Tl) * 4 + 4 + 15) / 16 * 16);
+
+    // the current parameter set (the only one unless Batched)
+    uint32_t set = 0;
+    SetInfo S = a.sets[0];
+    uint64_t seed = Batched ? a.set_seed[0] : a.seed;
+    uint32_t T = S.n_types;
+    uint32_t kdim = S.kdim;
+    double t_max = S.t_max;
 
     copy_tables(m_log, m_sc, tid, bdim);
-    for (uint32_t i = tid; i < T; i += bdim) {
+    for (uint32_t i = tid; i < (Batched ? 0u : T); i += bdim) {
         t_cum[i] = a.type_cum[i];
         t_mean[i] = a.type_mean[i];
         t_sd[i] = a.type_sd[i];
@@ -283,7 +313,7 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
     double cum8[8];
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k)
-        cum8[k] = k + 1 < T ? t_cum[k] : -__builtin_inf();
+        cum8[k] = k + 1 < T ? (Batched ? a.type_cum[k] : t_cum[k]) : -__builtin_inf();
 
     uint32_t u = blockIdx.x * waves + wave;
     while (u < a.n_units) {
@@ -291,6 +321,23 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
         const uint64_t sb = uni64(a.seg_begin[g]);
         const uint64_t lo = uni64(sb + (uint64_t)(u - a.unit_first[g]) * kUnit);
         const uint64_t hi = uni64(min(lo + kUnit, a.seg_begin[g + 1]));
+        if (Batched) {
+            const uint32_t su = uni(a.seg_set[g]);
+            if (su != set) {
+                flush_counts_wave(n, a.counters, set, (blockIdx.x * waves + wave) % kCounterCopies, lane);
+                set = su;
+                S = a.sets[set];
+                seed = a.set_seed[set];
+                T = S.n_types;
+                kdim = S.kdim;
+                t_max = S.t_max;
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k)
+                    cum8[k] = k + 1 < T ? a.type_cum[S.type_base + k] : -__builtin_inf();
+            }
+        }
+        const uint32_t rwin_rows = root_win_rows(T);
+        const uint32_t rwin_len = rwin_rows * T;
         for (uint32_t i = lane; i <= rwin_len; i += kWave)
             rwin[i] = 0u;  // rwin[rwin_len] absorbs the adds of lanes outside the window
         __builtin_amdgcn_wave_barrier();
@@ -308,7 +355,7 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             const bool ge_phi = (info >> 8) & 1u;
             // one block per initial cell: type uniform (word 0), phase (word
             // 1) and the first timer's Box-Muller pair (words 2, 3)
-            const Words w = dev::draw_block(a.seed, c, 0, 0);
+            const Words w = dev::draw_block(seed, c - S.cell_base, 0, 0);
             const double ut = dev::uniform_co(w.w0, 0u);
             const double U = dev::uniform_oc(w.w1, 0u);
             // first type with ut < cumulative proportion (cell.cu:90-101):
@@ -317,13 +364,13 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             uint32_t jt = T - 1;
             if (T > 9) {
                 for (uint32_t k = T - 2; k >= 8; --k)
-                    jt = ut < t_cum[k] ? k : jt;
+                    jt = ut < (Batched ? a.type_cum[S.type_base + k] : t_cum[k]) ? k : jt;
             }
 #pragma unroll
             for (int k = 7; k >= 0; --k)
                 jt = ut < cum8[k] ? (uint32_t)k : jt;
-            const double m = t_mean[jt];
-            const double sdv = t_sd[jt];
+            const double m = Batched ? a.type_mean[S.type_base + jt] : t_mean[jt];
+            const double sdv = Batched ? a.type_sd[S.type_base + jt] : t_sd[jt];
             const bool quiescent = m < 0.0;
             // first timer (cell.cu:106-122): two candidates from the root
             // block, then node 1's redraw stream
@@ -339,7 +386,7 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
                     // rare: node 1's redraw stream from attempt 1
                     tau0 = m;
                     for (uint32_t att = 1; att < 65536u; ++att) {
-                        const Words wr = dev::draw_block(a.seed, c, 1, att);
+                        const Words wr = dev::draw_block(seed, c - S.cell_base, 1, att);
                         double y0, y1;
                         dev::normal_pair(wr, y0, y1, K, tb);
                         const double d0 = fma(y0, sdv, m);
@@ -355,7 +402,7 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             const double s0 = t0 + tau0;
             // quiescent or out of time: tallied if its value is in the output
             // bin set; in time with f/2 <= phi: dropped; else it divides
-            const bool record = valid && (quiescent || s0 > a.t_max);
+            const bool record = valid && (quiescent || s0 > t_max);
             const bool rec_ok = record && ge_phi;
             const bool divides = valid && !record && kc != 0;
             n.cells += valid ? 1u : 0u;
@@ -363,7 +410,7 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             n.pool += divides ? 1u : 0u;
             // generation-0 tallies: the LDS window, else one device atomic per
             // distinct (row, type) of the step
-            const uint32_t tf = t_file[jt];
+            const uint32_t tf = Batched ? a.type_file[S.type_base + jt] : t_file[jt];
             const uint32_t woff = row - row_lo;
             const bool in_win = woff < rwin_rows;
             atomicAdd(&rwin[rec_ok && in_win ? woff * T + tf : rwin_len], rec_ok && in_win ? 1u : 0u);
@@ -374,7 +421,8 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
                 const uint32_t tf_l = uni(__shfl(tf, (int)leader));
                 const uint64_t same = __ballot(rec_ok && !in_win && row == row_l && tf == tf_l);
                 if (lane == leader)
-                    atomicAdd(&a.tally[(uint64_t)row_l * kdim * T + tf_l], (unsigned long long)__popcll(same));
+                    atomicAdd(&a.tally[S.tally_base + (uint64_t)(row_l - S.row_base) * kdim * T + tf_l],
+                              (unsigned long long)__popcll(same));
                 pend &= ~same;
             }
             // dividing cells: staged in LDS, appended to the queue at the end
@@ -382,7 +430,7 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
             if (divides) {
                 QueueEntry &e = stage[cnt + lane_rank(dmask)];
                 e.s0 = s0;
-                e.cell = (uint32_t)c;
+                e.cell = (uint32_t)(c - S.cell_base);
                 e.row_type = row | (jt << 24);
             }
             cnt = uni(cnt + (uint32_t)__popcll(dmask));
@@ -401,12 +449,16 @@ __global__ void __launch_bounds__(256) root_kernel(RunArgs a)
         for (uint32_t i = lane; i < rwin_len; i += kWave) {
             const uint32_t r = rwin[i];
             if (r)
-                atomicAdd(&a.tally[(uint64_t)(row_lo + i / T) * kdim * T + i % T], (unsigned long long)r);
+                atomicAdd(&a.tally[S.tally_base + (uint64_t)(row_lo - S.row_base + i / T) * kdim * T + i % T],
+                          (unsigned long long)r);
         }
         __builtin_amdgcn_wave_barrier();  // stage and window are reused by the next unit
         u = uni((uint32_t)min<uint64_t>((uint64_t)u + n_w, 0xFFFFFFFFull));
     }
-    flush_counts(n, a.counters, lane, red);
+    if (Batched)
+        flush_counts_wave(n, a.counters, set, (blockIdx.x * waves + wave) % kCounterCopies, lane);
+    else
+        flush_counts(n, a.counters, lane, red);
 }
 
 // Highest / lowest set bit of a nonzero 32- or 64-bit mask.
@@ -447,8 +499,14 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
     const uint32_t bdim = blockDim.x;
     const uint32_t lane = tid & (kWave - 1);
     const uint32_t wave = tid / kWave;
-    const uint32_t T = a.n_types;
-    const uint32_t kdim = a.kdim;
+    const uint32_t seg0 = blockIdx.x;
+    // the workgroup's parameter set (every segment it drains belongs to it)
+    const uint32_t set = a.n_sets > 1 ? a.seg_set[seg0] : 0u;
+    const SetInfo S = a.sets[set];
+    const uint64_t seed = a.n_sets > 1 ? a.set_seed[set] : a.seed;
+    const double t_max = S.t_max;
+    const uint32_t T = S.n_types;
+    const uint32_t kdim = S.kdim;
     const uint32_t win_len = a.win_rows * kdim * T;
     const uint32_t lds_rows = RowsLds ? a.n_rows : 0u;
     double(*m_log)[2] = (double(*)[2])lds;            // [128] log table
@@ -463,8 +521,8 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
     unsigned int *win = hint + 4;
     double *stack = (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
 
-    const uint32_t seg0 = blockIdx.x;
-    const uint32_t n_seg = gridDim.x;
+    const uint32_t seg_lo = S.seg_lo;            // the set's segments: [seg_lo, seg_lo + n_seg)
+    const uint32_t n_seg = S.seg_hi - S.seg_lo;
     const uint32_t row_lo = a.seg_row[seg0];  // first row of the tally window
 
     // first claim of the wavefront's own segment, issued before the set-up
@@ -482,9 +540,9 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
         *k_lds = dev::coeffs();
     copy_tables(m_log, m_sc, tid, bdim);
     for (uint32_t i = tid; i < T; i += bdim) {
-        t_mean[i] = a.type_mean[i];
-        t_sd[i] = a.type_sd[i];
-        t_file[i] = a.type_file[i];
+        t_mean[i] = a.type_mean[S.type_base + i];
+        t_sd[i] = a.type_sd[S.type_base + i];
+        t_file[i] = a.type_file[S.type_base + i];
     }
     for (uint32_t i = tid; i < lds_rows; i += bdim)
         r_info[i] = a.row_info[i];
@@ -571,7 +629,7 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
             L.mean = take ? t_mean[jt] : L.mean;
             L.sd = take ? t_sd[jt] : L.sd;
             L.tb_lds = take ? (off < a.win_rows ? off * kdim * T + tf : kNoWindow) : L.tb_lds;
-            L.tb_glob = take ? (uint64_t)row * kdim * T + tf : L.tb_glob;
+            L.tb_glob = take ? S.tally_base + (uint64_t)(row - S.row_base) * kdim * T + tf : L.tb_glob;
             L.active = L.active || take;
             const uint32_t taken = n_idle < have ? n_idle : have;
             pos = uni(pos + taken);
@@ -651,7 +709,7 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
                     const uint32_t h = hash32((seg0 * n_waves + wave) * 0x9E3779B9u + n_probe * 0x85EBCA6Bu);
                     n_probe = uni(n_probe + 1);
                     const uint32_t stride = n_seg > kProbes ? n_seg / kProbes : 1u;
-                    const uint32_t cand = (h + lane * stride + (h >> 16) % stride) % n_seg;
+                    const uint32_t cand = seg_lo + (h + lane * stride + (h >> 16) % stride) % n_seg;
                     uint64_t left = 0;
                     if (lane < kProbes) {
                         const uint64_t tot = a.seg_cur[(size_t)cand * kCursorWords + 1];
@@ -737,7 +795,7 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
             if (__builtin_expect(__ballot(L.redo != 0) == 0, 1)) {
                 // no redraw pending in the wavefront (the common case): every
                 // busy lane starts a new division with its pair block
-                const Words w = dev::draw_block(a.seed, L.cell, L.node, 0u);
+                const Words w = dev::draw_block(seed, L.cell, L.node, 0u);
                 double z0, z1;
                 dev::normal_pair(w, z0, z1, K, tb);
                 tl = fma(z0, L.sd, L.mean);
@@ -752,7 +810,7 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
                 const uint32_t fresh = L.redo == 0 ? 1u : 0u;
                 const uint32_t rright = (L.redo & 1u) ? 0u : 1u;   // redraw the right daughter?
                 const N bnode = fresh ? L.node : (L.node << 1) | (N)rright;
-                const Words w = dev::draw_block(a.seed, L.cell, bnode, fresh ? 0u : L.att);
+                const Words w = dev::draw_block(seed, L.cell, bnode, fresh ? 0u : L.att);
                 double z0, z1;
                 dev::normal_pair(w, z0, z1, K, tb);
                 const double c0 = fma(z0, L.sd, L.mean);
@@ -782,8 +840,8 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
             const double sl = L.s + tl;
             const double sr = L.s + tr;
             const uint32_t k1 = L.k + 1;
-            const uint32_t rec_l = sl > a.t_max ? 1u : 0u;
-            const uint32_t rec_r = sr > a.t_max ? 1u : 0u;
+            const uint32_t rec_l = sl > t_max ? 1u : 0u;
+            const uint32_t rec_r = sr > t_max ? 1u : 0u;
             const uint32_t can = k1 < L.kcap ? 1u : 0u;
             const uint32_t div_l = (rec_l ^ 1u) & can;
             const uint32_t div_r = (rec_r ^ 1u) & can;
@@ -825,8 +883,8 @@ __global__ void __launch_bounds__(256, 4) tree_kernel(RunArgs a)
         } while (uni((uint32_t)__popcll(__ballot(!L.active))) <= idle_limit);
     }
 
-    flush_counts(n, a.counters, lane, red);
-    const uint64_t gbase = (uint64_t)row_lo * kdim * T;
+    flush_counts(n, a.counters + (size_t)set * kCounterCopies * kNumCounters, lane, red);
+    const uint64_t gbase = S.tally_base + (uint64_t)(row_lo - S.row_base) * kdim * T;
     for (uint32_t i = tid; i < win_len; i += bdim) {
         const uint32_t c = win[i];
         if (c)
@@ -884,12 +942,16 @@ hipError_t launch_root(const RunArgs &a, const LaunchShape &ls, hipStream_t st)
 {
     if (a.n_cells == 0)
         return hipSuccess;
-    if (a.rows_in_lds)
-        hipLaunchKernelGGL(root_kernel<true>, dim3((unsigned)ls.root_blocks), dim3(ls.root_block_size),
-                           ls.root_lds_bytes, st, a);
+    const dim3 g((unsigned)ls.root_blocks), b(ls.root_block_size);
+    const bool batched = a.n_sets > 1;
+    if (a.rows_in_lds && !batched)
+        hipLaunchKernelGGL((root_kernel<true, false>), g, b, ls.root_lds_bytes, st, a);
+    else if (!batched)
+        hipLaunchKernelGGL((root_kernel<false, false>), g, b, ls.root_lds_bytes, st, a);
+    else if (a.rows_in_lds)
+        hipLaunchKernelGGL((root_kernel<true, true>), g, b, ls.root_lds_bytes, st, a);
     else
-        hipLaunchKernelGGL(root_kernel<false>, dim3((unsigned)ls.root_blocks), dim3(ls.root_block_size),
-                           ls.root_lds_bytes, st, a);
+        hipLaunchKernelGGL((root_kernel<false, true>), g, b, ls.root_lds_bytes, st, a);
     return hipGetLastError();
 }
 
@@ -939,9 +1001,9 @@ hipError_t tree_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, ui
 
 hipError_t root_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, int *blocks_per_cu)
 {
-    return rows_in_lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, root_kernel<true>, block_size,
+    return rows_in_lds ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, root_kernel<true, false>, block_size,
                                                                       lds_bytes)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, root_kernel<false>, block_size,
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, root_kernel<false, false>, block_size,
                                                                       lds_bytes);
 }
 

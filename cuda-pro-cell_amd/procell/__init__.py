@@ -95,6 +95,7 @@ EXPORTS = (
     "procell_plan_result", "procell_plan_destroy", "procell_plan_stream", "procell_plan_kernel_times",
     "procell_read_histogram",
     "procell_read_types", "procell_write_histogram", "procell_free", "procell_debug_draws",
+    "procell_batch_create", "procell_batch_run", "procell_batch_result", "procell_batch_autotune",
 )
 
 
@@ -127,6 +128,11 @@ def lib():
     L.procell_free.argtypes = [ctypes.c_void_p]
     L.procell_debug_draws.argtypes = [ctypes.c_int32, ctypes.c_uint64, _u64p, _u64p, _u32p,
                                       ctypes.c_uint64, _u32p, _dp]
+    if hasattr(L, "procell_batch_create"):  # (libraries built before the batch API lack it)
+        L.procell_batch_create.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
+        L.procell_batch_run.argtypes = [ctypes.c_void_p, _u64p]
+        L.procell_batch_result.argtypes = [ctypes.c_void_p, ctypes.c_uint32, R]
+        L.procell_batch_autotune.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint32, ctypes.POINTER(Tuning)]
     _lib = L
     return L
 
@@ -288,6 +294,46 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Batch(Plan):
+    """procell_batch_*: many parameter sets run by one launch of each kernel
+    (a replayed hipGraph); set i's result equals Plan(problems[i]) run with
+    seeds[i]."""
+
+    def __init__(self, problems, stream=None):
+        self.problems = list(problems)
+        self.prob = self.problems[0] if self.problems else None
+        arr = (Params * max(len(self.problems), 1))(*[p.params for p in self.problems])
+        if stream is not None and self.problems:
+            arr[0].stream = ctypes.c_void_p(int(stream))
+        self.handle = ctypes.c_void_p()
+        _check(lib().procell_batch_create(arr, len(self.problems), ctypes.byref(self.handle)), "batch_create")
+        self._buf = None
+
+    def _seeds(self, seeds):
+        s = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+        if s.shape != (len(self.problems),):
+            raise ValueError("one seed per parameter set")
+        return s
+
+    def run(self, seeds):
+        s = self._seeds(seeds)
+        _check(lib().procell_batch_run(self.handle, s.ctypes.data_as(_u64p)), "batch_run")
+
+    def result(self, i: int = 0, want_raw: bool = True) -> Result:
+        buf = _ResultBuffers(self.problems[i], want_raw)
+        _check(lib().procell_batch_result(self.handle, int(i), ctypes.byref(buf.r)), "batch_result")
+        return buf.result()
+
+    def autotune(self, seeds, reps: int = 2) -> dict:
+        """Times the candidate claim sizes and idle-lane thresholds on these
+        sets, keeps the fastest and returns it."""
+        s = self._seeds(seeds)
+        t = Tuning()
+        _check(lib().procell_batch_autotune(self.handle, s.ctypes.data_as(_u64p), int(reps), ctypes.byref(t)),
+               "batch_autotune")
+        return {"chunk": t.chunk, "idle_lanes": t.idle_lanes, "block_size": t.block_size}
 
 
 def debug_draws(seed, cells, nodes, attempts, device=0):

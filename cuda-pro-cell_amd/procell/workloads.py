@@ -87,6 +87,44 @@ def large_histogram(seed: int = 0) -> Workload:
     )
 
 
+def parameter_sets(seed: int = 0, n_sets: int = 512, scale: float = 1.0) -> list:
+    """Heterogeneous parameter sets of the batched plain method.  Per set:
+    128-1024 log-spaced bins from 1 to 1e5 with lognormal-shaped Poisson
+    counts totalling 10^x cells, x ~ N(4.5, 0.75) clipped to [3, 6] (times
+    `scale`); 1-5 types with means U(24,120), sd = U(0.15,0.5) x mean and,
+    for two or more types, a quiescent last type half of the time; ratios
+    Dirichlet(1,...,1); t_max U(72,336); phi U(1,100); per-type tracking."""
+    rng = np.random.default_rng([seed, 3])
+    sets = []
+    for i in range(n_sets):
+        bins = int(rng.integers(128, 1025))
+        cells = 10.0 ** float(np.clip(rng.normal(4.5, 0.75), 3.0, 6.0)) * scale
+        values = [10.0 ** (5.0 * j / (bins - 1)) for j in range(bins)]
+        counts = _lognormal_counts(values, cells, math.log(800.0), 1.0, rng)
+        n_types = int(rng.integers(1, 6))
+        quiescent = n_types >= 2 and bool(rng.integers(0, 2))
+        n_prolif = n_types - (1 if quiescent else 0)
+        means = rng.uniform(24.0, 120.0, size=n_prolif)
+        sds = rng.uniform(0.15, 0.5, size=n_prolif) * means
+        if quiescent:
+            means = np.concatenate([means, [-1.0]])
+            sds = np.concatenate([sds, [-1.0]])
+        ratios = rng.dirichlet(np.ones(n_types))
+        sets.append(Workload(
+            name=f"set{i}_bins{bins}",
+            values=np.array(values, dtype=np.float64),
+            counts=counts,
+            proportion=ratios.astype(np.float64),
+            mean=means.astype(np.float64),
+            sd=sds.astype(np.float64),
+            t_max=float(rng.uniform(72.0, 336.0)),
+            phi=float(rng.uniform(1.0, 100.0)),
+            track_ratio=True,
+            meta={"bins": bins, "target_cells": cells},
+        ))
+    return sets
+
+
 def scaled(w: Workload, factor: float, name: str | None = None) -> Workload:
     """Same shape with every count multiplied by `factor` (rounded down)."""
     counts = np.floor(w.counts.astype(np.float64) * factor).astype(np.uint64)

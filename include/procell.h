@@ -153,6 +153,25 @@ void *procell_plan_stream(procell_plan *plan);                       /* hipStrea
  * Waits for the stream; resets the run count. */
 int procell_plan_kernel_times(procell_plan *plan, double *ms, uint32_t max_runs, uint32_t *n_runs);
 
+/* ---- batched plain method: many parameter sets per launch --------------
+ * No reference counterpart (the reference simulates one parameter set per
+ * process).  The sets' initial cells, rows, types and tallies are
+ * concatenated and run by one launch of each kernel; every tree workgroup
+ * works on one set.  Set i's result equals procell_simulate of sets[i] with
+ * seed seeds[i], bit for bit.  Runs replay one hipGraph of the three
+ * kernels, captured on the first run.  All sets must name the same device;
+ * the launch knobs, flags and stream of sets[0] apply to the batch.  On a
+ * batch plan, procell_plan_run gives every set the same seed and
+ * procell_plan_result returns set 0. */
+int procell_batch_create(const procell_params *sets, uint32_t n_sets, procell_plan **plan);
+int procell_batch_run(procell_plan *plan, const uint64_t *seeds);              /* seeds[n_sets]; enqueue only */
+int procell_batch_result(procell_plan *plan, uint32_t set, procell_result *r); /* wait, download, merge */
+/* Autotuner: times `reps` runs (seeds[n_sets]) of every candidate claim
+ * size (chunk) and idle-lane threshold on this plan's own sets, keeps the
+ * fastest for later runs and returns it in *chosen.  Results do not depend
+ * on the knobs. */
+int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t reps, procell_tuning *chosen);
+
 /* ---- text formats ------------------------------------------------------
  * Replace load_fluorescences' file reading (src/io/parser.cu:68-124),
  * load_cell_types (src/io/parser.cu:156-185) and save_fluorescences

@@ -1,0 +1,105 @@
+"""GPU: the batched plain method equals single runs of its sets, bit for bit.
+
+A batch concatenates heterogeneous parameter sets (rows, cells, types,
+t_max, phi) and runs them with one launch of each kernel, replayed from a
+hipGraph.  Set i of a batch run with seeds[i] must equal a single-set plan of
+the same parameters run with seeds[i]: raw tally, merged histogram, per-type
+columns and event counters.  Graph replays are checked one by one, with
+eager runs and synchronisations between them.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STATS = ("divisions", "recorded", "dropped", "lost", "rejected", "cells")
+
+
+def _sets(procell, seed, n, scale):
+    from procell import workloads
+
+    return [procell.Problem.from_workload(w, 0) for w in workloads.parameter_sets(seed, n, scale)]
+
+
+def _same(got, ref, what=""):
+    assert got.raw_tally.shape == ref.raw_tally.shape, what
+    diff = np.argwhere(got.raw_tally != ref.raw_tally)
+    assert diff.size == 0, f"{what}: {len(diff)} tally cells differ, first {diff[:5].tolist()}"
+    assert np.array_equal(got.value, ref.value), what
+    assert np.array_equal(got.count, ref.count), what
+    assert np.array_equal(got.type_count, ref.type_count), what
+    for k in STATS:
+        assert got.stats[k] == ref.stats[k], (what, k, got.stats[k], ref.stats[k])
+
+
+def _single(procell, prob, seed):
+    with procell.Plan(prob) as pl:
+        pl.run(seed)
+        return pl.result()
+
+
+def test_batch_equals_single_runs_over_graph_replays(procell_lib):
+    probs = _sets(procell_lib, 1, 12, 0.02)
+    print("batch of", len(probs), "sets", flush=True)
+    with procell_lib.Batch(probs) as b:
+        for rep in range(3):
+            seeds = [1000 * rep + 7 * i + 1 for i in range(len(probs))]
+            b.run(seeds)
+            got = [b.result(i) for i in range(len(probs))]
+            assert any(g.stats["divisions"] > 0 for g in got)
+            for i, p in enumerate(probs):
+                _same(got[i], _single(procell_lib, p, seeds[i]), f"replay {rep} set {i}")
+
+
+def test_batch_edge_sets(procell_lib):
+    """All-quiescent, t_max 0, deep trees and a one-row set in one batch."""
+    P = procell_lib.Problem
+    probs = [
+        P([8.0, 3.0], [30, 20], [1.0], [-1.0], [-1.0], 40.0, 1.0),                   # all quiescent
+        P([100.0, 37.5], [40, 50], [0.5, 0.5], [10.0, 20.0], [3.0, 4.0], 0.0, 1.0),  # t_max 0
+        P([2.0**20], [3], [1.0], [2.0], [0.5], 60.0, 1.0),                           # 20 generations
+        P([64.0, 5.0, 2.5], [600, 300, 200], [0.5, 0.5], [12.0, 30.0], [11.0, 40.0], 90.0, 2.0),
+        P([1000.0], [500], [0.25, 0.75], [5.0, 7.0], [6.0, 2.0], 38.0, 10.0),       # heavy truncation
+    ]
+    seeds = [11, 12, 13, 14, 15]
+    with procell_lib.Batch(probs) as b:
+        b.run(seeds)
+        for i, p in enumerate(probs):
+            _same(b.result(i), _single(procell_lib, p, seeds[i]), f"set {i}")
+
+
+def test_batch_of_one_set_equals_plan(procell_lib):
+    from procell import workloads
+
+    prob = procell_lib.Problem.from_workload(workloads.small_histogram(2), 0)
+    with procell_lib.Batch([prob]) as b:
+        b.run([77])
+        _same(b.result(0), _single(procell_lib, prob, 77))
+
+
+def test_batch_matches_oracle(procell_lib, oracle):
+    from procell import workloads
+
+    ws = workloads.parameter_sets(5, 3, 0.02)
+    probs = [procell_lib.Problem.from_workload(w, 0) for w in ws]
+    seeds = [21, 22, 23]
+    with procell_lib.Batch(probs) as b:
+        b.run(seeds)
+        for i, w in enumerate(ws):
+            op = oracle.Problem.from_workload(w, seeds[i])
+            tally, _, st = oracle.simulate(op)
+            res = b.result(i)
+            assert np.array_equal(res.raw_tally, tally), i
+            for k in STATS:
+                assert res.stats[k] == st[k], (i, k)
+
+
+def test_autotune_keeps_results(procell_lib):
+    probs = _sets(procell_lib, 3, 8, 0.02)
+    seeds = [5 + i for i in range(len(probs))]
+    with procell_lib.Batch(probs) as b:
+        chosen = b.autotune(seeds, reps=1)
+        assert chosen["chunk"] in (256, 512, 1024, 2048) and chosen["idle_lanes"] in (8, 16, 24)
+        b.run(seeds)
+        for i, p in enumerate(probs):
+            _same(b.result(i), _single(procell_lib, p, seeds[i]), f"set {i}")

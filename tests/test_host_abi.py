@@ -36,7 +36,7 @@ def _has_gpu():
 
 def test_library_exports_every_declared_function(procell_lib):
     declared = _header_functions(procell_lib.HEADER_PATH)
-    assert len(declared) == 17
+    assert len(declared) == 21
     L = procell_lib.lib()
     for name in declared:
         assert hasattr(L, name), name
@@ -232,6 +232,30 @@ def test_simulate_fails_loudly_without_device(procell_lib):
     assert e.value.code == procell_lib.E_NO_DEVICE
     with pytest.raises(procell_lib.ProcellError):
         procell_lib.Plan(prob)
+
+
+def test_batch_validation(procell_lib):
+    """Batch creation validates every set before it touches a device."""
+    good = _readme_problem(procell_lib)
+    with pytest.raises(procell_lib.ProcellError) as e:
+        procell_lib.Batch([])
+    assert e.value.code == procell_lib.E_INVALID
+    bad = procell_lib.Problem([8.0], [3], [0.5, 0.4], [10.0, 20.0], [2.0, 2.0], 10.0, 1.0)
+    with pytest.raises(procell_lib.ProcellError) as e:
+        procell_lib.Batch([good, bad])
+    assert e.value.code == procell_lib.E_PROPORTIONS
+    zero_mean = procell_lib.Problem([8.0], [3], [1.0], [0.0], [2.0], 10.0, 1.0)
+    with pytest.raises(procell_lib.ProcellError) as e:
+        procell_lib.Batch([zero_mean, good])
+    assert e.value.code == procell_lib.E_ZERO_MEAN
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the behaviour without a device")
+def test_batch_fails_loudly_without_device(procell_lib):
+    prob = _readme_problem(procell_lib)
+    with pytest.raises(procell_lib.ProcellError) as e:
+        procell_lib.Batch([prob, prob])
+    assert e.value.code == procell_lib.E_NO_DEVICE
 
 
 def test_missing_library_raises(procell_lib, monkeypatch):
