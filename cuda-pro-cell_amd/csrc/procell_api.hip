@@ -152,6 +152,7 @@ using namespace procell;
 // across runs.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSeedSlots = 4;  // pinned host staging slots for batch seeds
+constexpr double kBatchSegments = 8.0;  // batch segments per resident tree workgroup (measured: 1x 6.5 ms, 2x 4.4, 4x 3.3, 8x 3.1, 16x 3.4 on 512 sets)
 
 struct procell_plan {
     int device = 0;
@@ -174,6 +175,8 @@ struct procell_plan {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<hipEvent_t> kev;   // PROCELL_FLAG_TIMING: start/stop of each run's kernels (ring)
     uint64_t runs = 0;             // runs since the last kernel-time read
+    size_t last_slot = 0;          // event slot of the last timed run
+    bool timed = false;            // a run has recorded its events
     uint64_t batch_runs = 0;
     bool ran = false;
 };
@@ -425,10 +428,11 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     uint32_t win_rows = budget > base + 16 ? (uint32_t)((budget - base - 16) / row_bytes) : 0;
 
     // ---- segments: one per tree-kernel workgroup, each within one set ----
-    // A single set gets one segment per resident workgroup; in a batch each
-    // set gets a share of the resident workgroups proportional to its
-    // expected work (at least one), and workgroups beyond the resident ones
-    // start as others finish.
+    // A single set gets one segment per resident workgroup.  In a batch each
+    // set gets segments in proportion to its expected work (at least one),
+    // kBatchSegments per resident workgroup in total: stealing stays within
+    // a set, so finer segments let the workgroup dispatcher balance the
+    // sets (workgroups beyond the resident ones start as others finish).
     const uint32_t resident = (uint32_t)std::min<int64_t>((int64_t)n_cu * occ, kMaxSegments);
     std::vector<uint32_t> g_set(n_sets, 1);
     if (n_sets == 1) {
@@ -440,8 +444,9 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
             w[i] = set_cost(pl->setups[i]);
             wt += w[i];
         }
+        const double target = kBatchSegments * (double)resident;
         for (uint32_t i = 0; i < n_sets; ++i)
-            g_set[i] = wt > 0.0 ? (uint32_t)std::max(1.0, std::floor(w[i] / wt * resident + 0.5)) : 1u;
+            g_set[i] = wt > 0.0 ? (uint32_t)std::max(1.0, std::floor(w[i] / wt * target + 0.5)) : 1u;
     }
     std::vector<uint64_t> seg_begin(1, 0);
     std::vector<uint32_t> seg_row, seg_set;
@@ -711,6 +716,10 @@ int run_plan(procell_plan *pl, const uint64_t *seeds, uint64_t seed, bool use_gr
     }
     if (pl->timing)
         HIP_TRY(hipEventRecord(pl->kev[2 * slot + 1], pl->stream));
+    if (pl->timing) {
+        pl->last_slot = slot;
+        pl->timed = true;
+    }
     ++pl->runs;
     pl->ran = true;
     return PROCELL_OK;
@@ -757,10 +766,9 @@ int plan_result(procell_plan *pl, uint32_t set, procell_result *r)
     st.block_size = (uint32_t)pl->shape.tree_block_size;
     st.lds_bytes = (uint32_t)pl->shape.tree_lds_bytes;
     st.window_rows = pl->args.win_rows;
-    if (pl->timing) {
+    if (pl->timing && pl->timed) {
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, pl->kev[2 * ((pl->runs - 1) % kTimedRuns)],
-                                    pl->kev[2 * ((pl->runs - 1) % kTimedRuns) + 1]));
+        HIP_TRY(hipEventElapsedTime(&ms, pl->kev[2 * pl->last_slot], pl->kev[2 * pl->last_slot + 1]));
         st.ms_tree = ms;
         st.ms_run = ms;
     }
