@@ -58,6 +58,8 @@ def lib():
         L.oracle_max_generation.argtypes = [ctypes.POINTER(_Problem)]
         L.oracle_simulate.restype = ctypes.c_int
         L.oracle_simulate.argtypes = [ctypes.POINTER(_Problem), _i64p, _i64p, _u64p, ctypes.c_int]
+        L.oracle_relaxed.restype = ctypes.c_int
+        L.oracle_relaxed.argtypes = [ctypes.POINTER(_Problem), ctypes.c_double, _dp, _dp, _dp, ctypes.c_int]
         L.oracle_run_files.restype = ctypes.c_int
         L.oracle_run_files.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_int]
@@ -155,6 +157,26 @@ def simulate(p: Problem, n_threads: int = 0, want_drops: bool = False):
     if rc != 0:
         raise OracleError(rc)
     return tally, drops, {k: int(v) for k, v in zip(STAT_NAMES, stats)}
+
+
+def relaxed(p: Problem, beta: float, g=None, n_threads: int = 0):
+    """Relaxed estimator: returns (h[n_nonzero_rows, kdim, n_types] float64,
+    grad[3, n_types] = dL/d(mean, sd, proportion) for g = dL/dh, or None)."""
+    kmax = max_generation(p)
+    if kmax < 0:
+        raise OracleError(kmax)
+    n_nz = int(np.count_nonzero(p.counts))
+    T = len(p.proportion)
+    h = np.zeros((n_nz, kmax + 1, max(T, 1)), dtype=np.float64)
+    grad = np.zeros((3, max(T, 1)), dtype=np.float64)
+    gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+    if gg is not None and gg.shape != h.shape:
+        raise ValueError("g must have the shape of h")
+    rc = lib().oracle_relaxed(ctypes.byref(p.c), float(beta), gg.ctypes.data_as(_dp) if gg is not None else None,
+                              h.ctypes.data_as(_dp), grad.ctypes.data_as(_dp), int(n_threads))
+    if rc != 0:
+        raise OracleError(rc)
+    return h, (grad[:, :T] if g is not None else None)
 
 
 def run_files(histogram, types, t_max, phi, track_ratio, seed, output, n_threads=0):

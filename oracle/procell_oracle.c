@@ -610,6 +610,162 @@ int oracle_simulate(const oracle_problem *p, int64_t *tally, int64_t *drops,
 }
 
 /* ------------------------------------------------------------------ */
+/* Relaxed (differentiable) estimator and its gradient.                 */
+/* ------------------------------------------------------------------ */
+/*
+ * This project's own relaxation of the plain method (no reference
+ * counterpart): every type j of an initial cell is followed with weight
+ * p_j (exact ratio weighting instead of a type draw); timers are
+ * mean_j + sd_j z with z the first candidates of the plain stream's blocks
+ * (no truncation, no redraws); a node whose division would complete at s
+ * divides with weight d = sigmoid((t_max - s) / beta) and is recorded with
+ * weight 1 - d, so a node's weight W is the product of its ancestors' d.
+ * Fluorescence stays exact: a node at its row's generation cap records
+ * W (1 - d) and drops the rest; at generation 0 a row below phi records
+ * nothing (proliferation.cu:317-381 with the hard test replaced).
+ * Subtrees whose weight W d falls below 1e-15 are not followed.
+ *
+ * The gradient of L with respect to (mean_j, sd_j, p_j), given
+ * g = dL/dh, is accumulated along the same traversal with forward
+ * tangents: ds/dmean = (1 + U) + k, ds/dsd = z_0 (1 + U) + z_1 + ... + z_k
+ * and dlog W/dtheta = -sum over ancestors of (1 - d) / beta ds/dtheta.
+ */
+#define RELAXED_EPS 1e-15
+
+typedef struct {
+    const oracle_setup *s;
+    const oracle_problem *p;
+    double beta;
+    uint64_t cell, row;
+    uint32_t j, T;
+    double pj, mean, sd;
+    int kcap;
+    int ge_phi;
+    const double *g;       /* [n_nz][kdim][T] or NULL */
+    double *h;             /* this thread's [n_nz][kdim][T] or NULL */
+    double *grad;          /* this thread's [3][T] */
+} relaxed_walk;
+
+static double sigmoid(double x) { return 1.0 / (1.0 + exp(-x)); }
+
+static void relaxed_node(relaxed_walk *w, uint64_t node, int k, double s, double a, double b,
+                         double W, double gm, double gs)
+{
+    const uint64_t kdim = (uint64_t)w->s->kmax + 1;
+    const double d = sigmoid((w->s->t_max - s) / w->beta);
+    const double rec = W * (1.0 - d);
+    if (k > 0 || w->ge_phi) {
+        const uint64_t idx = (w->row * kdim + (uint64_t)k) * w->T + w->j;
+        if (w->h)
+            w->h[idx] += w->pj * rec;
+        if (w->g) {
+            const double g = w->g[idx];
+            const double q = d * (1.0 - d) / w->beta;
+            w->grad[0 * w->T + w->j] += w->pj * g * W * ((1.0 - d) * gm + q * a);
+            w->grad[1 * w->T + w->j] += w->pj * g * W * ((1.0 - d) * gs + q * b);
+            w->grad[2 * w->T + w->j] += g * rec;
+        }
+    }
+    if (k == w->kcap || W * d < RELAXED_EPS)
+        return;
+    uint32_t blk[4];
+    double z0, z1;
+    oracle_block(w->s->seed, w->cell, node, 0, blk);
+    oracle_normal_pair(blk, &z0, &z1);
+    const double f = (1.0 - d) / w->beta;
+    const double Wc = W * d, gmc = gm - f * a, gsc = gs - f * b;
+    relaxed_node(w, 2 * node, k + 1, s + fma(z0, w->sd, w->mean), a + 1.0, b + z0, Wc, gmc, gsc);
+    relaxed_node(w, 2 * node + 1, k + 1, s + fma(z1, w->sd, w->mean), a + 1.0, b + z1, Wc, gmc, gsc);
+}
+
+static void relaxed_cell(relaxed_walk *w)
+{
+    uint32_t blk[4];
+    oracle_block(w->s->seed, w->cell, 0, 0, blk);
+    const double U = oracle_u01_open_closed(blk[1], 0u);
+    const uint32_t pw[4] = {blk[2], 0u, blk[3], 0u};
+    double z0, z1;
+    oracle_normal_pair(pw, &z0, &z1);
+    const uint64_t kdim = (uint64_t)w->s->kmax + 1;
+    for (uint32_t j = 0; j < w->T; ++j) {
+        w->j = j;
+        w->pj = w->p->proportion[j];
+        w->mean = w->p->mean[j];
+        w->sd = w->p->sd[j];
+        if (w->mean < 0.0) {
+            /* quiescent: always out of time, recorded at generation 0 */
+            if (w->ge_phi) {
+                const uint64_t idx = (w->row * kdim) * w->T + j;
+                if (w->h)
+                    w->h[idx] += w->pj;
+                if (w->g)
+                    w->grad[2 * w->T + j] += w->g[idx];
+            }
+            continue;
+        }
+        const double tau0 = fma(z0, w->sd, w->mean);
+        relaxed_node(w, 1, 0, tau0 * (1.0 + U), 1.0 + U, z0 * (1.0 + U), 1.0, 0.0, 0.0);
+    }
+}
+
+int oracle_relaxed(const oracle_problem *p, double beta, const double *g, double *h, double *grad,
+                   int n_threads)
+{
+    if (!(beta > 0.0) || !isfinite(beta))
+        return OR_E_INVALID;
+    oracle_setup s;
+    int rc = setup_build(p, &s);
+    if (rc != OR_OK)
+        return rc;
+    const uint32_t T = p->n_types;
+    const uint64_t kdim = (uint64_t)s.kmax + 1;
+    const uint64_t hsz = s.n_nz * kdim * (uint64_t)(T ? T : 1);
+    if (h)
+        memset(h, 0, hsz * sizeof(double));
+    if (grad)
+        memset(grad, 0, 3 * (size_t)T * sizeof(double));
+    if (n_threads <= 0)
+        n_threads = omp_get_max_threads();
+#pragma omp parallel num_threads(n_threads)
+    {
+        double *lh = h ? (double *)calloc(hsz ? hsz : 1, sizeof(double)) : NULL;
+        double *lg = (double *)calloc(3 * (size_t)(T ? T : 1), sizeof(double));
+        relaxed_walk w;
+        memset(&w, 0, sizeof w);
+        w.s = &s;
+        w.p = p;
+        w.beta = beta;
+        w.T = T;
+        w.g = g;
+        w.h = lh;
+        w.grad = lg;
+#pragma omp for schedule(dynamic, 64)
+        for (uint64_t row = 0; row < s.n_nz; ++row) {
+            w.row = row;
+            w.kcap = s.kcap[row];
+            w.ge_phi = s.v[row] >= s.phi;
+            for (uint64_t c = s.start[row]; c < s.start[row + 1]; ++c) {
+                w.cell = c;
+                relaxed_cell(&w);
+            }
+        }
+#pragma omp critical
+        {
+            if (h)
+                for (uint64_t i = 0; i < hsz; ++i)
+                    h[i] += lh[i];
+            if (grad)
+                for (uint64_t i = 0; i < 3 * (uint64_t)T; ++i)
+                    grad[i] += lg[i];
+        }
+        free(lh);
+        free(lg);
+    }
+    setup_free(&s);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* File formats (parser.cu:68-154 read, :156-185 types, :187-217 write). */
 /* ------------------------------------------------------------------ */
 static int cmp_double(const void *a, const void *b)

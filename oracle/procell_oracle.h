@@ -71,6 +71,16 @@ int oracle_max_generation(const oracle_problem *p);
 int oracle_simulate(const oracle_problem *p, int64_t *tally, int64_t *drops,
                     uint64_t *stats, int n_threads);
 
+/*
+ * Relaxed estimator (this project's differentiable relaxation, see the
+ * source): h = expected tally [n_nonzero_rows][kdim][n_types] (file type
+ * order, may be NULL); with g = dL/dh of the same shape (may be NULL),
+ * grad = dL/d(mean, sd, proportion) as [3][n_types] (file order).
+ * beta > 0 is the sigmoid width.  Returns 0 or a negative code.
+ */
+int oracle_relaxed(const oracle_problem *p, double beta, const double *g, double *h, double *grad,
+                   int n_threads);
+
 /* File-level run: reads the two input files, simulates, writes the output
  * text (value<TAB>count[<TAB>per-type...] per line).  Returns 0 or negative. */
 int oracle_run_files(const char *histogram_path, const char *types_path,
