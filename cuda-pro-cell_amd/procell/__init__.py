@@ -96,6 +96,8 @@ EXPORTS = (
     "procell_read_histogram",
     "procell_read_types", "procell_write_histogram", "procell_free", "procell_debug_draws",
     "procell_batch_create", "procell_batch_run", "procell_batch_result", "procell_batch_autotune",
+    "procell_relaxed_create", "procell_relaxed_shape", "procell_relaxed_forward", "procell_relaxed_backward",
+    "procell_relaxed_expansions", "procell_relaxed_destroy",
 )
 
 
@@ -106,6 +108,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with cuda-pro-cell_amd/build.py")
+    # torch wheels carry their own HIP runtime under the same SONAME
+    # (libamdhip64.so.7).  Two runtimes in one process cannot both open the
+    # GPU: whichever initialises second finds no device.  Loading torch first
+    # makes the dynamic loader bind libprocell to torch's copy, so that
+    # tensors, streams and this library share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.POINTER(Params)
     R = ctypes.POINTER(ResultStruct)
@@ -133,6 +144,14 @@ def lib():
         L.procell_batch_run.argtypes = [ctypes.c_void_p, _u64p]
         L.procell_batch_result.argtypes = [ctypes.c_void_p, ctypes.c_uint32, R]
         L.procell_batch_autotune.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint32, ctypes.POINTER(Tuning)]
+    if hasattr(L, "procell_relaxed_create"):
+        L.procell_relaxed_create.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]
+        L.procell_relaxed_shape.argtypes = [ctypes.c_void_p, _u64p, _u32p, _u32p]
+        L.procell_relaxed_forward.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.procell_relaxed_backward.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p]
+        L.procell_relaxed_expansions.argtypes = [ctypes.c_void_p, _u64p]
+        L.procell_relaxed_destroy.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
 
@@ -164,6 +183,10 @@ class Problem:
             self.types[j].mean = float(mean[j])
             self.types[j].sd = float(sd[j])
         self.n_types = n_types
+        self.values, self.counts = values, counts
+        self.proportion = np.array(proportion, dtype=np.float64)
+        self.mean = np.array(mean, dtype=np.float64)
+        self.sd = np.array(sd, dtype=np.float64)
         self.params = Params()
         self.params.rows = ctypes.cast(self.rows, ctypes.POINTER(Row))
         self.params.n_rows = len(values)
@@ -334,6 +357,81 @@ class Batch(Plan):
         _check(lib().procell_batch_autotune(self.handle, s.ctypes.data_as(_u64p), int(reps), ctypes.byref(t)),
                "batch_autotune")
         return {"chunk": t.chunk, "idle_lanes": t.idle_lanes, "block_size": t.block_size}
+
+
+class Relaxed:
+    """procell_relaxed_*: the relaxed estimator of one histogram (t_max, phi,
+    seed and quiescent types fixed by `prob`) at temperature `beta`.
+
+    forward / backward take device tensors (float64, contiguous, on the
+    plan's device): theta [3, n_types] = mean, sd, proportion; h and g
+    [n_rows, kdim, n_types]; grad [3, n_types].  They run on `stream`
+    (torch's current stream by default), in order with the caller's work."""
+
+    def __init__(self, prob: Problem, beta: float):
+        self.prob = prob
+        self.beta = float(beta)
+        self.handle = ctypes.c_void_p()
+        _check(lib().procell_relaxed_create(ctypes.byref(prob.params), self.beta, ctypes.byref(self.handle)),
+               "relaxed_create")
+        n, k, t = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().procell_relaxed_shape(self.handle, ctypes.byref(n), ctypes.byref(k), ctypes.byref(t)),
+               "relaxed_shape")
+        self.shape = (int(n.value), int(k.value), int(t.value))
+
+    @staticmethod
+    def _ptr(x, shape, what):
+        import torch
+
+        if x.dtype != torch.float64 or not x.is_cuda or not x.is_contiguous():
+            raise ValueError(f"{what}: a contiguous float64 CUDA tensor is required")
+        if tuple(x.shape) != tuple(shape):
+            raise ValueError(f"{what}: shape {tuple(x.shape)}, expected {tuple(shape)}")
+        return ctypes.c_void_p(x.data_ptr())
+
+    @staticmethod
+    def _stream(stream):
+        if stream is None:
+            import torch
+
+            return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        return ctypes.c_void_p(int(stream))
+
+    def forward(self, theta, h, stream=None):
+        T = self.shape[2]
+        _check(lib().procell_relaxed_forward(self.handle, self._ptr(theta, (3, T), "theta"),
+                                             self._ptr(h, self.shape, "h"), self._stream(stream)),
+               "relaxed_forward")
+
+    def backward(self, theta, g, grad, stream=None):
+        T = self.shape[2]
+        _check(lib().procell_relaxed_backward(self.handle, self._ptr(theta, (3, T), "theta"),
+                                              self._ptr(g, self.shape, "g"), self._ptr(grad, (3, T), "grad"),
+                                              self._stream(stream)),
+               "relaxed_backward")
+
+    def expansions(self) -> int:
+        """Expansions of the last call (roots drawn plus nodes expanded); waits."""
+        n = ctypes.c_uint64()
+        _check(lib().procell_relaxed_expansions(self.handle, ctypes.byref(n)), "relaxed_expansions")
+        return int(n.value)
+
+    def close(self):
+        if self.handle:
+            lib().procell_relaxed_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def debug_draws(seed, cells, nodes, attempts, device=0):

@@ -125,6 +125,33 @@ def parameter_sets(seed: int = 0, n_sets: int = 512, scale: float = 1.0) -> list
     return sets
 
 
+def calibration_instance(seed: int = 0) -> Workload:
+    """Single-instance calibration problem: 256 log-spaced bins from 1 to 1e4
+    with ~1e6 cells (lognormal-shaped Poisson counts, mu = ln 800, sigma 1),
+    hidden types drawn from the seed -- two proliferating types with means
+    U(30,100) and sd = U(0.2,0.4) x mean plus one quiescent type, ratios
+    Dirichlet(1,1,1) -- t_max 168, phi = smallest populated bin.  The
+    relaxed estimator's temperature is beta = 0.5 (meta["beta"])."""
+    rng = np.random.default_rng([seed, 4])
+    values = [10.0 ** (4.0 * i / 255.0) for i in range(256)]
+    counts = _lognormal_counts(values, 1e6, math.log(800.0), 1.0, rng)
+    means = rng.uniform(30.0, 100.0, size=2)
+    sds = rng.uniform(0.2, 0.4, size=2) * means
+    ratios = rng.dirichlet(np.ones(3))
+    return Workload(
+        name="hist256_1e6_types3_calibration",
+        values=np.array(values, dtype=np.float64),
+        counts=counts,
+        proportion=ratios.astype(np.float64),
+        mean=np.concatenate([means, [-1.0]]).astype(np.float64),
+        sd=np.concatenate([sds, [-1.0]]).astype(np.float64),
+        t_max=168.0,
+        phi=0.0,
+        track_ratio=True,
+        meta={"bins": 256, "target_cells": 1e6, "beta": 0.5},
+    )
+
+
 def scaled(w: Workload, factor: float, name: str | None = None) -> Workload:
     """Same shape with every count multiplied by `factor` (rounded down)."""
     counts = np.floor(w.counts.astype(np.float64) * factor).astype(np.uint64)

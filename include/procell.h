@@ -172,6 +172,41 @@ int procell_batch_result(procell_plan *plan, uint32_t set, procell_result *r); /
  * on the knobs. */
 int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t reps, procell_tuning *chosen);
 
+/* ---- relaxed (differentiable) estimator --------------------------------
+ * No reference counterpart (the reference only samples).  For calibrating
+ * (mean, sd, proportion) of each type to an observed histogram.  The plan
+ * fixes the histogram, t_max, phi, seed, the type count and which types are
+ * quiescent (mean < 0 in *p); beta > 0 is the temperature of
+ * sigmoid((t_max - s) / beta), which replaces the plain method's "division
+ * completes by t_max" test.  Every initial cell takes every type j with
+ * weight p_j; timers are mean_j + sd_j z with z the first normal candidates
+ * of the plain method's Philox stream, untruncated; a node of weight W
+ * records W (1 - d) at (row, generation, type) and its daughters carry W d;
+ * fluorescence and the generation cap are the plain method's.  Subtrees
+ * whose weight falls below 1e-15 are not expanded.
+ *   theta : device [3][n_types] = mean, sd, proportion (file order); the
+ *           entries of quiescent types' mean and sd are ignored.  Means of
+ *           proliferating types must stay well above 0: the tree grows
+ *           until the sigmoid or the generation cap stops it.
+ *   h     : device [n_rows][kdim][n_types] relaxed expected tally (rows =
+ *           nonzero rows in file order, as raw_tally).
+ *   g     : device, shape of h: the cotangent dL/dh.
+ *   grad  : device [3][n_types] = dL/dmean, dL/dsd, dL/dproportion.
+ * The backward pass recomputes every subtree from the stream; nothing is
+ * kept between calls.  Calls run on `stream` (hipStream_t) in order with the
+ * caller's other work; NULL runs on the plan's own stream and waits.  Calls
+ * on one plan must not overlap (they share its per-lane stacks). */
+typedef struct procell_relaxed procell_relaxed;
+int procell_relaxed_create(const procell_params *p, double beta, procell_relaxed **plan);
+int procell_relaxed_shape(const procell_relaxed *plan, uint64_t *n_rows, uint32_t *kdim, uint32_t *n_types);
+int procell_relaxed_forward(procell_relaxed *plan, const double *theta, double *h, void *stream);
+int procell_relaxed_backward(procell_relaxed *plan, const double *theta, const double *g, double *grad,
+                             void *stream);
+/* Expansions of the plan's last call (roots drawn plus nodes whose daughters
+ * were drawn); waits for the device. */
+int procell_relaxed_expansions(procell_relaxed *plan, uint64_t *n);
+int procell_relaxed_destroy(procell_relaxed *plan);
+
 /* ---- text formats ------------------------------------------------------
  * Replace load_fluorescences' file reading (src/io/parser.cu:68-124),
  * load_cell_types (src/io/parser.cu:156-185) and save_fluorescences

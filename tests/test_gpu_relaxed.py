@@ -1,0 +1,180 @@
+"""GPU: the relaxed estimator and its adjoint against the oracle.
+
+The device kernels sum with FP64 atomics in an order of their own and call
+the device exp, so they agree with the oracle to rounding, not bit for bit:
+every relaxed tally entry within 1e-10 relative (plus 1e-12 of the largest
+entry), every gradient entry within 1e-9 relative plus 1e-10 of the largest
+gradient entry.  The torch.autograd.Function must give the library's
+gradient, and the gradient of a normalised-histogram loss must agree with
+FP64 central differences of the device forward pass (common random numbers).
+The plain method is not changed by relaxed calls.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+H_RTOL, H_ATOL = 1e-10, 1e-12
+G_RTOL, G_ATOL = 1e-9, 1e-10
+
+
+def _problems():
+    from procell import workloads
+
+    w = workloads.small_histogram(4)
+    return [
+        # the small standard workload (3 types incl. quiescent), 1e5 cells
+        (dict(values=w.values, counts=w.counts, proportion=w.proportion, mean=w.mean, sd=w.sd,
+              t_max=w.t_max, phi=0.0, seed=7), 0.5),
+        # a generation-0 cut (phi above some values), sharp beta
+        (dict(values=[64.0, 20.0, 5.0, 2.5], counts=[600, 400, 300, 200], proportion=[0.45, 0.35, 0.2],
+              mean=[14.0, 22.0, -1.0], sd=[4.0, 7.0, -1.0], t_max=70.0, phi=3.0, seed=11), 0.05),
+        # deep rows (cap 20), wide beta, one type
+        (dict(values=[2.0**20, 1000.0], counts=[50, 70], proportion=[1.0], mean=[20.0], sd=[3.0],
+              t_max=75.0, phi=1.0, seed=3), 4.0),
+        # t_max 0: everything records at generation 0 with weight 1 - sigmoid(-s/beta)
+        (dict(values=[100.0, 37.5], counts=[40, 50], proportion=[0.5, 0.5], mean=[10.0, 20.0], sd=[3.0, 4.0],
+              t_max=0.0, phi=1.0, seed=5), 1.0),
+    ]
+
+
+def _args(d):
+    return (d["values"], d["counts"], d["proportion"], d["mean"], d["sd"], d["t_max"], d["phi"], d["seed"])
+
+
+def _device_pass(procell_lib, d, beta, g=None, theta=None):
+    import torch
+
+    prob = procell_lib.Problem(*_args(d))
+    th = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]) if theta is None else theta,
+                      dtype=torch.float64, device="cuda")
+    with procell_lib.Relaxed(prob, beta) as rp:
+        h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        rp.forward(th, h)
+        grad = None
+        if g is not None:
+            gt = torch.tensor(g, dtype=torch.float64, device="cuda")
+            grad_t = torch.empty_like(th)
+            rp.backward(th, gt, grad_t)
+            grad = grad_t.cpu().numpy()
+        torch.cuda.synchronize()
+        return h.cpu().numpy(), grad
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_forward_and_gradient_match_oracle(procell_lib, oracle, case):
+    d, beta = _problems()[case]
+    print("relaxed case", case, "beta", beta, flush=True)
+    h_ref, _ = oracle.relaxed(oracle.Problem(*_args(d)), beta)
+    g = np.random.default_rng(case).normal(size=h_ref.shape)
+    _, grad_ref = oracle.relaxed(oracle.Problem(*_args(d)), beta, g)
+    h, grad = _device_pass(procell_lib, d, beta, g)
+    assert h.shape == h_ref.shape
+    assert np.all(np.abs(h - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max())), \
+        np.abs(h - h_ref).max()
+    assert h.sum() > 0
+    scale = max(1.0, np.abs(grad_ref).max())
+    assert np.all(np.abs(grad - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale), (grad, grad_ref)
+
+
+def test_all_quiescent_and_no_division(procell_lib, oracle):
+    d = dict(values=[8.0, 3.0, 0.5], counts=[30, 20, 10], proportion=[0.6, 0.4], mean=[-1.0, -1.0],
+             sd=[-1.0, -1.0], t_max=40.0, phi=1.0, seed=2)
+    h_ref, _ = oracle.relaxed(oracle.Problem(*_args(d)), 0.5)
+    g = np.random.default_rng(9).normal(size=h_ref.shape)
+    _, grad_ref = oracle.relaxed(oracle.Problem(*_args(d)), 0.5, g)
+    h, grad = _device_pass(procell_lib, d, 0.5, g)
+    assert np.allclose(h, h_ref, rtol=1e-14, atol=0)
+    assert np.allclose(grad, grad_ref, rtol=1e-12, atol=1e-12)
+    assert grad[0].tolist() == [0.0, 0.0] and grad[1].tolist() == [0.0, 0.0]
+
+
+def test_autograd_function_and_finite_differences(procell_lib):
+    import torch
+    from procell import calibrate
+
+    d, beta = _problems()[1]
+    prob = procell_lib.Problem(*_args(d))
+    with procell_lib.Relaxed(prob, beta * 20) as rp:
+        bins, idx = calibrate.output_bins(prob, rp.shape[1])
+        idx_t = torch.tensor(idx, device="cuda")
+        target = torch.tensor(np.linspace(1.0, 2.0, len(bins)), dtype=torch.float64, device="cuda")
+
+        def loss(th):
+            h = calibrate.relaxed_tally(rp, th)
+            return calibrate.normalized_l2(calibrate.binned(h, idx_t, len(bins)), target)
+
+        th0 = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
+        th = th0.clone().requires_grad_(True)
+        L = loss(th)
+        L.backward()
+        grad = th.grad.cpu().numpy()
+        # the Function's gradient is the library's backward with the same cotangent
+        with torch.no_grad():
+            w = torch.randn(rp.shape, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(1))
+        th2 = th0.clone().requires_grad_(True)
+        (calibrate.relaxed_tally(rp, th2) * w).sum().backward()
+        direct = torch.empty_like(th0)
+        rp.backward(th0, w, direct)
+        torch.cuda.synchronize()
+        assert torch.allclose(th2.grad, direct, rtol=1e-12, atol=1e-12 * float(direct.abs().max()))
+        # central differences of the device forward
+        for i, j in ((0, 0), (0, 1), (1, 0), (1, 1)):
+            step = 1e-5 * abs(float(th0[i, j]))
+            up, dn = th0.clone(), th0.clone()
+            up[i, j] += step
+            dn[i, j] -= step
+            with torch.no_grad():
+                fd = (loss(up) - loss(dn)).item() / (2 * step)
+            assert abs(fd - grad[i, j]) <= 1e-5 * max(abs(fd), 1e-12) + 1e-12, (i, j, fd, grad[i, j])
+        # proportions along p_a - p_b (the loss is not invariant to their sum)
+        step = 1e-6
+        up, dn = th0.clone(), th0.clone()
+        up[2, 0] += step
+        up[2, 1] -= step
+        dn[2, 0] -= step
+        dn[2, 1] += step
+        with torch.no_grad():
+            fd = (loss(up) - loss(dn)).item() / (2 * step)
+        ad = grad[2, 0] - grad[2, 1]
+        assert abs(fd - ad) <= 1e-5 * max(abs(fd), 1e-12) + 1e-12, (fd, ad)
+
+
+def test_plain_method_unchanged(procell_lib, oracle):
+    import torch
+    from procell import workloads
+
+    w = workloads.small_histogram(1)
+    prob = procell_lib.Problem.from_workload(w, 0)
+    with procell_lib.Plan(prob) as pl:
+        pl.run(123)
+        before = pl.result()
+    th = torch.tensor(np.stack([w.mean, w.sd, w.proportion]), dtype=torch.float64, device="cuda")
+    with procell_lib.Relaxed(prob, 0.5) as rp:
+        h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        rp.forward(th, h)
+        g = torch.ones_like(h)
+        grad = torch.empty_like(th)
+        rp.backward(th, g, grad)
+        torch.cuda.synchronize()
+    with procell_lib.Plan(prob) as pl:
+        pl.run(123)
+        after = pl.result()
+    assert np.array_equal(before.raw_tally, after.raw_tally)
+    tally, _, _ = oracle.simulate(oracle.Problem.from_workload(w, 123))
+    assert np.array_equal(after.raw_tally, tally)
+
+
+def test_own_stream_when_null(procell_lib, oracle):
+    import torch
+
+    d, beta = _problems()[3]
+    prob = procell_lib.Problem(*_args(d))
+    th = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    with procell_lib.Relaxed(prob, beta) as rp:
+        h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        rp.forward(th, h, stream=0)  # NULL: the plan's own stream, waited for
+        got = h.cpu().numpy()
+    h_ref, _ = oracle.relaxed(oracle.Problem(*_args(d)), beta)
+    assert np.all(np.abs(got - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))

@@ -36,7 +36,7 @@ def _has_gpu():
 
 def test_library_exports_every_declared_function(procell_lib):
     declared = _header_functions(procell_lib.HEADER_PATH)
-    assert len(declared) == 21
+    assert len(declared) == 27
     L = procell_lib.lib()
     for name in declared:
         assert hasattr(L, name), name
@@ -256,6 +256,25 @@ def test_batch_fails_loudly_without_device(procell_lib):
     with pytest.raises(procell_lib.ProcellError) as e:
         procell_lib.Batch([prob, prob])
     assert e.value.code == procell_lib.E_NO_DEVICE
+
+
+def test_relaxed_validation_and_no_device(procell_lib):
+    import ctypes
+
+    prob = _readme_problem(procell_lib)
+    h = ctypes.c_void_p()
+    L = procell_lib.lib()
+    for beta in (0.0, -1.0, float("nan"), float("inf")):
+        assert L.procell_relaxed_create(ctypes.byref(prob.params), beta, ctypes.byref(h)) == procell_lib.E_INVALID
+    assert L.procell_relaxed_create(None, 0.5, ctypes.byref(h)) == procell_lib.E_INVALID
+    for null_call in (L.procell_relaxed_forward(None, None, None, None),
+                      L.procell_relaxed_backward(None, None, None, None, None),
+                      L.procell_relaxed_shape(None, None, None, None)):
+        assert null_call == procell_lib.E_INVALID
+    if not _has_gpu():
+        with pytest.raises(procell_lib.ProcellError) as e:
+            procell_lib.Relaxed(prob, 0.5)
+        assert e.value.code == procell_lib.E_NO_DEVICE
 
 
 def test_missing_library_raises(procell_lib, monkeypatch):
