@@ -158,6 +158,110 @@ def batched_measurement(procell, workloads, np, stream, runs=20):
     }
 
 
+def relaxed_measurement(procell, workloads, np, torch, with_cpu=True, reps=50, fit_steps=300):
+    """The relaxed estimator on the single-instance calibration workload
+    (256 bins, ~1e6 cells, two proliferating types and one quiescent, beta
+    0.5): device time of one forward and one adjoint pass, gradient steps
+    per second of an Adam fit through the torch binding, and the parameters
+    that fit recovers from a perturbed start against a plain-method target
+    at the hidden parameters."""
+    from procell import calibrate
+
+    w = workloads.calibration_instance(0)
+    beta = w.meta["beta"]
+    hidden = np.stack([w.mean, w.sd, w.proportion])
+    prolif = w.mean >= 0
+    target_res = procell.simulate(procell.Problem.from_workload(w, 0), want_raw=False)
+    prob = procell.Problem.from_workload(w, 1)
+    plan = procell.Relaxed(prob, beta)
+    bins, _ = calibrate.output_bins(prob, plan.shape[1])
+    target = calibrate.target_histogram(bins, target_res.value, target_res.count)
+    start = hidden.copy()
+    start[0, prolif] *= np.array([1.15, 0.87])[: prolif.sum()]
+    start[1, prolif] *= np.array([1.25, 0.8])[: prolif.sum()]
+    start[2] = 0.5 * start[2] + 0.5 / len(start[2])
+    th = torch.tensor(start, dtype=torch.float64, device="cuda")
+    h = torch.empty(plan.shape, dtype=torch.float64, device="cuda")
+    g = torch.randn(plan.shape, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(0))
+    grad = torch.empty_like(th)
+    times = {}
+    for name, fn in (("forward", lambda: plan.forward(th, h)), ("backward", lambda: plan.backward(th, g, grad))):
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1) / reps
+        times[name + "_expansions"] = plan.expansions()
+    # loss at the hidden parameters and between two plain samples (noise floor)
+    bm = calibrate.BinMap(prob, plan.shape[1])
+    tgt = torch.tensor(target, dtype=torch.float64, device="cuda")
+    with torch.no_grad():
+        loss_hidden = float(calibrate.normalized_l2(
+            bm(calibrate.relaxed_tally(plan, torch.tensor(hidden, dtype=torch.float64, device="cuda"))), tgt))
+    other = procell.simulate(procell.Problem.from_workload(w, 7), want_raw=False)
+    t2 = calibrate.target_histogram(bins, other.value, other.count)
+    noise_floor = float(((t2 / t2.sum() - target / target.sum()) ** 2).sum())
+    calibrate.fit(plan, prob, target, start, steps=10)  # warm-up (torch kernels, allocator)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    theta, losses = calibrate.fit(plan, prob, target, start, steps=fit_steps)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    plan.close()
+    rel = np.abs(theta - hidden) / np.abs(hidden)
+    err = {"mean_rel": rel[0, prolif].tolist(), "sd_rel": rel[1, prolif].tolist(),
+           "proportion_abs": np.abs(theta[2] - hidden[2]).tolist()}
+    exp_pass = times["forward_expansions"]
+    pass_ms = times["forward"] + times["backward"]
+    log(f"relaxed: forward {times['forward']:.4f} ms, backward {times['backward']:.4f} ms, "
+        f"{exp_pass} expansions per pass; fit {fit_steps} steps {fit_s:.2f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    out = {
+        "workload": "hist256_1e6_types3_calibration: 256 log-spaced bins 1..1e4, lognormal Poisson counts "
+                    "(~1e6 cells), 2 proliferating types + 1 quiescent, t_max 168, phi auto, beta 0.5 "
+                    "(workloads.calibration_instance(0))",
+        "cells": w.n_cells,
+        "beta": beta,
+        "forward_ms": times["forward"],
+        "backward_ms": times["backward"],
+        "expansions_per_pass": exp_pass,
+        "expansions_per_s": 2 * exp_pass / (pass_ms * 1e-3),
+        "grad_steps_per_s_kernels": 1e3 / pass_ms,
+        "grad_steps_per_s_fit": fit_steps / fit_s,
+        "fit": {"steps": fit_steps, "optimizer": "Adam lr 0.02 on log mean, log sd, proportion logits",
+                "loss_start": losses[0], "loss_end": losses[-1], "loss_at_hidden": loss_hidden,
+                "plain_noise_floor": noise_floor,
+                "note": "loss at the hidden parameters equals the distance between two plain samples "
+                        "(seeds 0 and 7): the relaxation's bias is below the sampling noise; the fit "
+                        "reaches that floor along a flat valley, so one 1e6-cell histogram identifies "
+                        "(mean, sd, proportion) only loosely",
+                "target": "plain method at the hidden parameters, seed 0; relaxed stream seed 1",
+                "start": start.tolist(), "hidden": hidden.tolist(), "recovered": theta.tolist(),
+                "relative_error": err},
+        "dtype": "f64",
+    }
+    if with_cpu:
+        from oracle import binding as ob
+
+        threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+        op = ob.Problem(w.values, w.counts, start[2], start[0], start[1], w.t_max, w.phi, 1)
+        gh = g.cpu().numpy()
+        t1 = time.perf_counter()
+        ob.relaxed(op, beta, None, threads)
+        ob.relaxed(op, beta, gh, threads)
+        dt = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "grad-steps/s (forward + adjoint)", "cores": threads,
+                               "kind": "port", "sample": f"full workload, one forward and one adjoint pass "
+                                                         f"({dt:.2f} s wall), OpenMP oracle",
+                               "cpu": cpu_model()}
+        log(f"relaxed cpu baseline: {dt:.3f} s per forward + adjoint on {threads} threads")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +270,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=1.5, help="seconds of CPU-baseline wall time")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-method measurement")
+    ap.add_argument("--no-relaxed", action="store_true", help="skip the relaxed-estimator measurement")
+    ap.add_argument("--fit-steps", type=int, default=300, help="Adam steps of the relaxed-estimator fit")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -263,7 +369,6 @@ def main():
             "block_size": stats["block_size"],
             "lds_bytes": stats["lds_bytes"],
             "abi_end_to_end_ms": float(np.median(e2e)),
-            "calibration": "not built (grad-steps/s not measured)",
         },
         "roofline": {
             "bound": "valu",
@@ -282,6 +387,9 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batched_measurement(procell, workloads, np, stream.cuda_stream)
+    if not args.no_relaxed:
+        out["relaxed"] = relaxed_measurement(procell, workloads, np, torch, with_cpu=not args.no_cpu,
+                                             fit_steps=args.fit_steps)
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_budget)
         out["cpu_baseline"] = cb

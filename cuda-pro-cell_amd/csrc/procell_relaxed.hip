@@ -344,7 +344,6 @@ using namespace procell;
 
 struct procell_relaxed {
     int device = 0;
-    hipStream_t stream = nullptr;
     Setup s;
     uint32_t n_waves = 0;
     size_t lds_bytes = 0;
@@ -364,8 +363,6 @@ void destroy_relaxed(procell_relaxed *r)
         (void)hipFree(r->d_block);
     if (r->d_stack)
         (void)hipFree(r->d_stack);
-    if (r->stream)
-        (void)hipStreamDestroy(r->stream);
     delete r;
 }
 
@@ -386,7 +383,7 @@ int relaxed_device(int device, int *n_cu)
 int relaxed_launch(procell_relaxed *r, bool grad, void *stream)
 {
     RELAX_TRY(hipSetDevice(r->device));
-    hipStream_t st = stream ? (hipStream_t)stream : r->stream;
+    hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
     const RelaxArgs &a = r->args;
     const uint64_t n_h = (uint64_t)a.n_rows * a.kdim * a.n_types;
     RELAX_TRY(launch_zero(a.expansions, 1, st));
@@ -410,8 +407,6 @@ int relaxed_launch(procell_relaxed *r, bool grad, void *stream)
             hipLaunchKernelGGL(relaxed_quiescent_kernel<false>, g, dim3(256), 0, st, a);
         RELAX_TRY(hipGetLastError());
     }
-    if (!stream)
-        RELAX_TRY(hipStreamSynchronize(st));
     return PROCELL_OK;
 }
 
@@ -538,7 +533,6 @@ int procell_relaxed_create(const procell_params *p, double beta, procell_relaxed
     const size_t o_wu = take(wave_unit.size() * 8), o_wr = take(wave_row.size() * 4);
     const size_t o_pr = take(prolif.size() * 4 + 4), o_qu = take(quiet.size() * 4 + 4);
     const size_t o_ex = take(8);
-    RELAX_TRY(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     if (hipMalloc(&r->d_block, off) != hipSuccess)
         return fail(PROCELL_E_OUT_OF_MEMORY);
     char *base = (char *)r->d_block;

@@ -73,8 +73,28 @@ def output_bins(prob: Problem, kdim: int):
     return bins, idx
 
 
+class BinMap:
+    """Gather/scatter indices that merge a relaxed tally into output bins,
+    prepared once on the device (no host synchronisation per call)."""
+
+    def __init__(self, prob: Problem, kdim: int, device="cuda"):
+        self.bins, idx = output_bins(prob, kdim)
+        flat = idx.reshape(-1)
+        pos = np.nonzero(flat >= 0)[0]
+        self.n_bins = len(self.bins)
+        self.pos = torch.tensor(pos, dtype=torch.int64, device=device)
+        self.bin = torch.tensor(flat[pos], dtype=torch.int64, device=device)
+
+    def __call__(self, h: torch.Tensor) -> torch.Tensor:
+        """Sum of h over types and over the (row, generation) entries of each bin."""
+        flat = h.sum(-1).reshape(-1)
+        out = torch.zeros(self.n_bins, dtype=h.dtype, device=h.device)
+        return out.index_add(0, self.bin, flat.index_select(0, self.pos))
+
+
 def binned(h: torch.Tensor, idx: torch.Tensor, n_bins: int) -> torch.Tensor:
-    """Sum of h over types and over the (row, generation) entries of each bin."""
+    """Sum of h over types and over the (row, generation) entries of each bin
+    (idx from output_bins; -1 entries are skipped)."""
     flat = h.sum(-1).reshape(-1)
     sel = idx.reshape(-1)
     keep = sel >= 0
@@ -94,3 +114,42 @@ def target_histogram(bins: np.ndarray, value: np.ndarray, count: np.ndarray) -> 
 def normalized_l2(h_bins: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
     """Squared L2 distance between the two histograms, each divided by its sum."""
     return ((h_bins / h_bins.sum() - target / target.sum()) ** 2).sum()
+
+
+def assemble_theta(log_mean, log_sd, logits, quiet):
+    """theta [3, n_types] from unconstrained parameters: mean = exp(log_mean),
+    sd = exp(log_sd) for proliferating types (-1 for quiescent ones, whose
+    entries the library ignores), proportions = softmax(logits)."""
+    neg = torch.full_like(log_mean, -1.0)
+    mean = torch.where(quiet, neg, torch.exp(log_mean))
+    sd = torch.where(quiet, neg, torch.exp(log_sd))
+    return torch.stack([mean, sd, torch.softmax(logits, 0)])
+
+
+def fit(plan: Relaxed, prob: Problem, target: np.ndarray, theta0: np.ndarray, steps: int = 200,
+        lr: float = 0.02):
+    """Adam on the squared L2 distance between the normalised relaxed
+    histogram and `target` (counts on the output bin grid of `prob`), over
+    log mean, log sd and proportion logits, starting from theta0.  Returns
+    (theta [3, n_types] numpy, per-step losses)."""
+    dev = torch.device("cuda")
+    bm = BinMap(prob, plan.shape[1], dev)
+    tgt = torch.tensor(target, dtype=torch.float64, device=dev)
+    th0 = torch.tensor(theta0, dtype=torch.float64, device=dev)
+    quiet = th0[0] < 0
+    one = torch.ones_like(th0[0])
+    log_mean = torch.log(torch.where(quiet, one, th0[0])).clone().requires_grad_(True)
+    log_sd = torch.log(torch.where(quiet, one, th0[1])).clone().requires_grad_(True)
+    logits = torch.log(th0[2]).clone().requires_grad_(True)
+    opt = torch.optim.Adam([log_mean, log_sd, logits], lr=lr)
+    losses = []
+    for _ in range(steps):
+        theta = assemble_theta(log_mean, log_sd, logits, quiet)
+        loss = normalized_l2(bm(relaxed_tally(plan, theta)), tgt)
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+        losses.append(loss.detach())
+    with torch.no_grad():
+        theta = assemble_theta(log_mean, log_sd, logits, quiet)
+    return theta.cpu().numpy(), [float(x) for x in torch.stack(losses).cpu()] if losses else []

@@ -193,9 +193,10 @@ int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t r
  *   g     : device, shape of h: the cotangent dL/dh.
  *   grad  : device [3][n_types] = dL/dmean, dL/dsd, dL/dproportion.
  * The backward pass recomputes every subtree from the stream; nothing is
- * kept between calls.  Calls run on `stream` (hipStream_t) in order with the
- * caller's other work; NULL runs on the plan's own stream and waits.  Calls
- * on one plan must not overlap (they share its per-lane stacks). */
+ * kept between calls.  Calls are enqueued on `stream` (a hipStream_t; NULL is
+ * the default stream), in order with the caller's other work on it, and
+ * return without waiting.  Calls on one plan must not overlap (they share
+ * its per-lane stacks). */
 typedef struct procell_relaxed procell_relaxed;
 int procell_relaxed_create(const procell_params *p, double beta, procell_relaxed **plan);
 int procell_relaxed_shape(const procell_relaxed *plan, uint64_t *n_rows, uint32_t *kdim, uint32_t *n_types);

@@ -165,16 +165,29 @@ def test_plain_method_unchanged(procell_lib, oracle):
     assert np.array_equal(after.raw_tally, tally)
 
 
-def test_own_stream_when_null(procell_lib, oracle):
+def test_stream_order_with_torch_work(procell_lib, oracle):
+    """theta written by a torch kernel just before the call (no
+    synchronisation) on the default stream (NULL) and on a side stream: the
+    library's kernels run after it, on the same stream."""
     import torch
 
     d, beta = _problems()[3]
     prob = procell_lib.Problem(*_args(d))
-    th = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    with procell_lib.Relaxed(prob, beta) as rp:
-        h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
-        rp.forward(th, h, stream=0)  # NULL: the plan's own stream, waited for
-        got = h.cpu().numpy()
+    ref = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
     h_ref, _ = oracle.relaxed(oracle.Problem(*_args(d)), beta)
-    assert np.all(np.abs(got - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
+    side = torch.cuda.Stream()
+    with procell_lib.Relaxed(prob, beta) as rp:
+        for st in (None, side):
+            torch.cuda.synchronize()
+            with torch.cuda.stream(st if st is not None else torch.cuda.default_stream()):
+                th = torch.zeros_like(ref)
+                for _ in range(20):  # queue work ahead of the write that matters
+                    th.mul_(1.0)
+                th.add_(ref)
+                h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+                rp.forward(th, h, stream=0 if st is None else st.cuda_stream)
+                got = h.cpu().numpy() if st is None else None
+            if st is not None:
+                side.synchronize()
+                got = h.cpu().numpy()
+            assert np.all(np.abs(got - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
