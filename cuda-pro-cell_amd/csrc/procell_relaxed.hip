@@ -55,27 +55,45 @@ constexpr double kRelaxEps = 1e-15;        // smallest W d that is still expande
 constexpr uint32_t kFrameFields = 6;       // s, b, W, d, dlogW/dmean, dlogW/dsd
 constexpr uint32_t kWindowDoubles = 1024;  // LDS window of a wavefront (8 KB)
 
-struct RelaxArgs {
+// One parameter set of a relaxed plan (a single-set plan has one).  Rows,
+// initial cells, tally entries and type blocks of the sets are concatenated.
+struct RelaxSet {
     uint64_t seed;
     double t_max;
-    double beta;
+    uint64_t cell_base;    // first initial cell (stream cell ids are relative to it)
+    uint64_t h_base;       // first tally entry of the set's [rows][kdim][n_types]
+    uint32_t row_base;     // first nonzero row
     uint32_t n_rows;
-    uint32_t kdim;             // generations + 1
+    uint32_t kdim;         // generations + 1
     uint32_t n_types;
-    uint32_t n_prolif;         // proliferating types (mean >= 0 at plan creation)
+    uint32_t theta_base;   // first entry of the set's [3][n_types] block of theta / gradient
+    uint32_t prolif_base;  // first entry in the proliferating-type list
+    uint32_t n_prolif;     // proliferating types (mean >= 0 when the plan was made)
+    uint32_t quiet_base;
     uint32_t n_quiet;
+    uint32_t pad[3];
+};
+
+struct RelaxArgs {
+    double beta;
+    uint32_t n_sets;
+    uint32_t n_rows;           // all sets
+    uint32_t kdim_max;         // stack levels per lane
     uint32_t pad;
-    const uint32_t *prolif;    // [n_prolif] file index of each proliferating type
-    const uint32_t *quiet;     // [n_quiet] file index of each quiescent type
-    const double *theta;       // [3][n_types]: mean, sd, proportion, file order
-    const uint32_t *row_start; // [n_rows + 1] first initial cell of each nonzero row
+    const RelaxSet *sets;
+    const uint32_t *wave_set;  // [n_waves] set of each wavefront
+    const uint64_t *wave_unit; // [n_waves][2] first and end walk of the range (set-local)
+    const uint32_t *wave_row;  // [n_waves][2] first and last (global) row of the range
+    const uint32_t *row_start; // [n_rows + 1] first initial cell of each row (all sets)
     const uint32_t *row_info;  // [n_rows] generation cap | (value >= phi) << 8
-    const uint64_t *wave_unit; // [n_waves + 1] first walk of each wavefront's range
-    const uint32_t *wave_row;  // [n_waves][2] first and last row of the range
-    double *stack;             // [n_waves][kdim][kFrameFields][64]
-    double *h;                 // [n_rows][kdim][n_types] relaxed tally (forward)
-    const double *g;           // [n_rows][kdim][n_types] cotangent (backward)
-    double *grad;              // [3][n_types] gradient (backward)
+    const uint32_t *row_set;   // [n_rows] set of each row
+    const uint32_t *prolif;    // file index of each proliferating type, per set
+    const uint32_t *quiet;     // file index of each quiescent type, per set
+    const double *theta;       // per set [3][n_types]: mean, sd, proportion, file order
+    double *stack;             // [n_waves][kdim_max][kFrameFields][64]
+    double *h;                 // per set [rows][kdim][n_types] relaxed tally (forward)
+    const double *g;           // cotangent, layout of h (backward)
+    double *grad;              // per set [3][n_types] gradient (backward)
     unsigned long long *expansions;  // [1] expansions of the call (roots drawn + nodes expanded)
 };
 
@@ -87,19 +105,21 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
     extern __shared__ double lds[];  // [kWindowDoubles] window | [3][n_types] gradient sums
     const uint32_t lane = threadIdx.x;
     const uint32_t w = blockIdx.x;
-    const uint32_t T = a.n_types;
-    const uint32_t KT = a.kdim * T;
-    uint64_t next = a.wave_unit[w];
-    const uint64_t u_end = a.wave_unit[w + 1];
+    const RelaxSet S = a.sets[a.wave_set[w]];
+    const uint32_t T = S.n_types;
+    const uint32_t KT = S.kdim * T;
+    uint64_t next = a.wave_unit[2 * w];
+    const uint64_t u_end = a.wave_unit[2 * w + 1];
     if (next >= u_end)
         return;
     const uint32_t r_lo = a.wave_row[2 * w], r_hi = a.wave_row[2 * w + 1];
     const uint32_t win_n = (r_hi - r_lo + 1) * KT;
     const bool in_lds = win_n <= kWindowDoubles;
+    const uint64_t hw0 = S.h_base + (uint64_t)(r_lo - S.row_base) * KT;  // tally index of the window
     double *gsum = lds + kWindowDoubles;
     if (in_lds)
         for (uint32_t i = lane; i < win_n; i += 64)
-            lds[i] = Grad ? a.g[(uint64_t)r_lo * KT + i] : 0.0;
+            lds[i] = Grad ? a.g[hw0 + i] : 0.0;
     if (Grad)
         for (uint32_t i = lane; i < 3 * T; i += 64)
             gsum[i] = 0.0;
@@ -107,9 +127,11 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
 
     const dev::Coeffs K = dev::coeffs();
     const dev::Tables tb = dev::global_tables();
-    const double t_max = a.t_max, beta = a.beta;
-    const uint32_t n_prolif = a.n_prolif;
-    double *stk = a.stack + (uint64_t)w * a.kdim * kFrameFields * 64 + lane;
+    const double t_max = S.t_max, beta = a.beta;
+    const uint32_t n_prolif = S.n_prolif;
+    const double *theta = a.theta + S.theta_base;
+    const uint32_t *prolif = a.prolif + S.prolif_base;
+    double *stk = a.stack + (uint64_t)w * a.kdim_max * kFrameFields * 64 + lane;
 
     bool active = false;
     uint64_t cell = 0, node = 0, pending = 0;
@@ -129,16 +151,17 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
                 const uint64_t u = next + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (u < u_end) {
                     cell = u / n_prolif;
-                    j = a.prolif[u - cell * n_prolif];
+                    j = prolif[u - cell * n_prolif];
+                    const uint64_t gcell = S.cell_base + cell;
                     row = row_cursor;
-                    while (a.row_start[row + 1] <= cell)
+                    while (a.row_start[row + 1] <= gcell)
                         ++row;
                     const uint32_t info = a.row_info[row];
                     kcap = info & 255u;
                     ge_phi = ((info >> 8) & 1u) != 0;
-                    mean = a.theta[j];
-                    sd = a.theta[T + j];
-                    pj = a.theta[2 * T + j];
+                    mean = theta[j];
+                    sd = theta[T + j];
+                    pj = theta[2 * T + j];
                     node = 0;  // the next step draws the root from block (cell, 0, 0)
                     pending = 0;
                     active = true;
@@ -148,7 +171,7 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             if (next > u_end)
                 next = u_end;
             if (next < u_end) {
-                const uint64_t c = next / n_prolif;
+                const uint64_t c = S.cell_base + next / n_prolif;
                 while (a.row_start[row_cursor + 1] <= c)
                     ++row_cursor;
             }
@@ -162,7 +185,7 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
         //      both daughters of the current node ----
         const bool pre = node == 0;
         ++n_exp;
-        dev::Words wd = dev::draw_block(a.seed, cell, node, 0u);
+        dev::Words wd = dev::draw_block(S.seed, cell, node, 0u);
         if (pre) {
             u1 = 1.0 + dev::uniform_oc(wd.w1, 0u);
             wd = dev::Words{wd.w2, 0u, wd.w3, 0u};
@@ -206,10 +229,11 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
                 if (in_lds)
                     atomicAdd(lds + (row - r_lo) * KT + col, v);
                 else
-                    unsafeAtomicAdd(a.h + (uint64_t)row * KT + col, v);
+                    unsafeAtomicAdd(a.h + S.h_base + (uint64_t)(row - S.row_base) * KT + col, v);
             }
         } else {
-            const double gv = in_lds ? lds[(row - r_lo) * KT + col] : a.g[(uint64_t)row * KT + col];
+            const double gv = in_lds ? lds[(row - r_lo) * KT + col]
+                                     : a.g[S.h_base + (uint64_t)(row - S.row_base) * KT + col];
             if (recordA) {
                 const double q = dA * (1.0 - dA) / beta;
                 acc_m += pj * gv * WA * ((1.0 - dA) * gmA + q * aA);
@@ -280,13 +304,13 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             for (uint32_t i = lane; i < win_n; i += 64) {
                 const double v = lds[i];
                 if (v != 0.0)
-                    unsafeAtomicAdd(a.h + (uint64_t)r_lo * KT + i, v);
+                    unsafeAtomicAdd(a.h + hw0 + i, v);
             }
     } else {
         for (uint32_t i = lane; i < 3 * T; i += 64) {
             const double v = gsum[i];
             if (v != 0.0)
-                unsafeAtomicAdd(a.grad + i, v);
+                unsafeAtomicAdd(a.grad + S.theta_base + i, v);
         }
     }
 }
@@ -296,19 +320,21 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
 template <bool Grad>
 __global__ void relaxed_quiescent_kernel(RelaxArgs a)
 {
-    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)a.n_rows * a.n_quiet)
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= a.n_rows)
         return;
-    const uint32_t row = (uint32_t)(t / a.n_quiet);
-    const uint32_t j = a.quiet[t - (uint64_t)row * a.n_quiet];
-    if (((a.row_info[row] >> 8) & 1u) == 0)
+    const RelaxSet &S = a.sets[a.row_set[row]];
+    if (S.n_quiet == 0 || ((a.row_info[row] >> 8) & 1u) == 0)
         return;
     const double cnt = (double)(a.row_start[row + 1] - a.row_start[row]);
-    const uint64_t idx = (uint64_t)row * a.kdim * a.n_types + j;
-    if (!Grad)
-        a.h[idx] = cnt * a.theta[2 * a.n_types + j];
-    else
-        unsafeAtomicAdd(a.grad + 2 * a.n_types + j, cnt * a.g[idx]);
+    const uint64_t base = S.h_base + (uint64_t)(row - S.row_base) * S.kdim * S.n_types;
+    for (uint32_t q = 0; q < S.n_quiet; ++q) {
+        const uint32_t j = a.quiet[S.quiet_base + q];
+        if (!Grad)
+            a.h[base + j] = cnt * a.theta[S.theta_base + 2 * S.n_types + j];
+        else
+            unsafeAtomicAdd(a.grad + S.theta_base + 2 * S.n_types + j, cnt * a.g[base + j]);
+    }
 }
 
 thread_local hipError_t g_relaxed_hip = hipSuccess;
@@ -337,35 +363,6 @@ double walk_cost(uint32_t kcap, double t_max, double mean)
     return ldexp(1.0, (int)g + 1) + 1.0;
 }
 
-}  // namespace
-}  // namespace procell
-
-using namespace procell;
-
-struct procell_relaxed {
-    int device = 0;
-    Setup s;
-    uint32_t n_waves = 0;
-    size_t lds_bytes = 0;
-    RelaxArgs args{};
-    void *d_block = nullptr;
-    double *d_stack = nullptr;
-};
-
-namespace {
-
-void destroy_relaxed(procell_relaxed *r)
-{
-    if (!r)
-        return;
-    (void)hipSetDevice(r->device);
-    if (r->d_block)
-        (void)hipFree(r->d_block);
-    if (r->d_stack)
-        (void)hipFree(r->d_stack);
-    delete r;
-}
-
 int relaxed_device(int device, int *n_cu)
 {
     int count = 0;
@@ -380,34 +377,540 @@ int relaxed_device(int device, int *n_cu)
     return PROCELL_OK;
 }
 
-int relaxed_launch(procell_relaxed *r, bool grad, void *stream)
+// Device tables of a relaxed plan over n parameter sets.
+struct RelaxCore {
+    int device = 0;
+    uint32_t n_sets = 0;
+    std::vector<Setup> setups;
+    std::vector<RelaxSet> sets;
+    std::vector<uint32_t> quiet_flag;  // per type of every set: 1 = quiescent
+    uint32_t n_waves = 0;
+    size_t lds_bytes = 0;
+    bool any_prolif = false, any_quiet = false;
+    uint64_t h_total = 0;      // tally entries of all sets
+    uint32_t types_total = 0;  // types of all sets
+    RelaxArgs args{};
+    void *d_block = nullptr;
+    double *d_stack = nullptr;
+};
+
+void core_free(RelaxCore &c)
 {
-    RELAX_TRY(hipSetDevice(r->device));
-    hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
-    const RelaxArgs &a = r->args;
-    const uint64_t n_h = (uint64_t)a.n_rows * a.kdim * a.n_types;
-    RELAX_TRY(launch_zero(a.expansions, 1, st));
-    if (!grad)
-        RELAX_TRY(launch_zero((unsigned long long *)a.h, n_h, st));
-    else
-        RELAX_TRY(launch_zero((unsigned long long *)a.grad, 3ull * a.n_types, st));
-    if (a.n_prolif && r->n_waves) {
-        if (grad)
-            hipLaunchKernelGGL(relaxed_kernel<true>, dim3(r->n_waves), dim3(64), r->lds_bytes, st, a);
-        else
-            hipLaunchKernelGGL(relaxed_kernel<false>, dim3(r->n_waves), dim3(64), r->lds_bytes, st, a);
-        RELAX_TRY(hipGetLastError());
+    (void)hipSetDevice(c.device);
+    if (c.d_block)
+        (void)hipFree(c.d_block);
+    if (c.d_stack)
+        (void)hipFree(c.d_stack);
+    c.d_block = nullptr;
+    c.d_stack = nullptr;
+}
+
+// Splits walks [0, n_units) of one set into n_w ranges of equal estimated
+// cost (unit u = cell u / n_prolif, type prolif[u % n_prolif]).
+void split_set(const Setup &s, const std::vector<double> &unit_cost, uint32_t n_prolif, uint32_t n_w,
+               uint32_t row_base, uint64_t *unit_out, uint32_t *row_out)
+{
+    const uint32_t n_rows = (uint32_t)s.v.size();
+    const uint64_t n_units = s.n_cells * (uint64_t)n_prolif;
+    double total = 0.0;
+    for (uint32_t i = 0; i < n_rows; ++i)
+        total += unit_cost[i] * (double)(s.start[i + 1] - s.start[i]) * n_prolif;
+    std::vector<uint64_t> cut(n_w + 1, n_units);
+    cut[0] = 0;
+    uint32_t wv = 1;
+    double acc = 0.0;
+    for (uint32_t i = 0; i < n_rows && wv < n_w; ++i) {
+        const uint64_t u0 = s.start[i] * n_prolif, u1 = s.start[i + 1] * n_prolif;
+        const double per = unit_cost[i];
+        while (wv < n_w) {
+            const double need = (total * (double)wv / (double)n_w - acc) / per;
+            if (need >= (double)(u1 - u0))
+                break;
+            uint64_t c = u0 + (uint64_t)std::max(0.0, ceil(need));
+            c = std::min(c, u1);
+            cut[wv] = std::max(c, cut[wv - 1]);
+            ++wv;
+        }
+        acc += per * (double)(u1 - u0);
     }
-    const uint64_t nq = (uint64_t)a.n_rows * a.n_quiet;
-    if (nq) {
-        const dim3 g((unsigned)((nq + 255) / 256));
+    for (uint32_t v = 1; v <= n_w; ++v)
+        cut[v] = std::max(cut[v], cut[v - 1]);
+    auto row_of = [&](uint64_t cell) {
+        return (uint32_t)(std::upper_bound(s.start.begin(), s.start.end() - 1, cell) - s.start.begin() - 1);
+    };
+    for (uint32_t v = 0; v < n_w; ++v) {
+        unit_out[2 * v] = cut[v];
+        unit_out[2 * v + 1] = cut[v + 1];
+        row_out[2 * v] = row_out[2 * v + 1] = row_base;
+        if (cut[v] < cut[v + 1]) {
+            row_out[2 * v] = row_base + row_of(cut[v] / n_prolif);
+            row_out[2 * v + 1] = row_base + row_of((cut[v + 1] - 1) / n_prolif);
+        }
+    }
+}
+
+// Validates the sets, sizes the launch (resident wavefronts of the plan's
+// device; `wave_factor` times more for many sets, so that the dispatcher
+// evens out the cost estimate's errors between sets) and uploads the tables.
+int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, bool want_bins)
+{
+    if (!ps || n == 0)
+        return PROCELL_E_INVALID;
+    if (!(beta > 0.0) || !isfinite(beta))
+        return PROCELL_E_INVALID;
+    for (uint32_t i = 1; i < n; ++i)
+        if (ps[i].device != ps[0].device)
+            return PROCELL_E_INVALID;
+    c.n_sets = n;
+    c.setups.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const int rc = build_setup(&ps[i], c.setups[i], want_bins);
+        if (rc != PROCELL_OK)
+            return rc;
+    }
+    int n_cu = 0;
+    {
+        const int rc = relaxed_device(ps[0].device, &n_cu);
+        if (rc != PROCELL_OK)
+            return rc;
+    }
+    c.device = ps[0].device;
+
+    // ---- concatenation ----
+    std::vector<uint32_t> prolif, quiet, row_start, row_info, row_set;
+    c.sets.resize(n);
+    uint64_t cells = 0;
+    uint32_t rows = 0, kdim_max = 1, t_max_types = 1;
+    std::vector<uint64_t> units(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const Setup &s = c.setups[i];
+        RelaxSet &S = c.sets[i];
+        memset(&S, 0, sizeof S);
+        S.seed = ps[i].seed;
+        S.t_max = ps[i].t_max;
+        S.cell_base = cells;
+        S.h_base = c.h_total;
+        S.row_base = rows;
+        S.n_rows = (uint32_t)s.v.size();
+        S.kdim = (uint32_t)s.kmax + 1;
+        S.n_types = std::max(1u, ps[i].n_types);
+        S.theta_base = 3 * c.types_total;
+        S.prolif_base = (uint32_t)prolif.size();
+        S.quiet_base = (uint32_t)quiet.size();
+        for (uint32_t j = 0; j < ps[i].n_types; ++j) {
+            const bool q = ps[i].types[j].mean < 0.0;
+            (q ? quiet : prolif).push_back(j);
+            c.quiet_flag.push_back(q ? 1u : 0u);
+        }
+        for (uint32_t j = ps[i].n_types; j < S.n_types; ++j)
+            c.quiet_flag.push_back(1u);
+        S.n_prolif = (uint32_t)prolif.size() - S.prolif_base;
+        S.n_quiet = (uint32_t)quiet.size() - S.quiet_base;
+        c.any_prolif = c.any_prolif || (S.n_prolif && s.n_cells);
+        c.any_quiet = c.any_quiet || S.n_quiet;
+        for (uint32_t r = 0; r < S.n_rows; ++r) {
+            row_start.push_back((uint32_t)(cells + s.start[r]));
+            row_info.push_back(s.row_info[r]);
+            row_set.push_back(i);
+        }
+        units[i] = s.n_cells * (uint64_t)S.n_prolif;
+        cells += s.n_cells;
+        if (cells > kMaxCells)
+            return PROCELL_E_TOO_MANY_CELLS;
+        rows += S.n_rows;
+        c.h_total += (uint64_t)S.n_rows * S.kdim * S.n_types;
+        c.types_total += S.n_types;
+        kdim_max = std::max(kdim_max, S.kdim);
+        t_max_types = std::max(t_max_types, S.n_types);
+    }
+    row_start.push_back((uint32_t)cells);
+
+    // ---- wavefronts ----
+    c.lds_bytes = relaxed_lds_bytes(t_max_types);
+    int occ = 0, occ_f = 0;
+    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, relaxed_kernel<true>, 64, c.lds_bytes));
+    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, relaxed_kernel<false>, 64, c.lds_bytes));
+    occ = std::min(occ, occ_f);
+    if (occ < 1)
+        return PROCELL_E_HIP;
+    if (ps[0].tuning.blocks_per_cu)
+        occ = std::min(occ, (int)ps[0].tuning.blocks_per_cu);
+    const uint64_t budget = (uint64_t)occ * (uint64_t)n_cu * (n > 1 ? 2u : 1u);
+    std::vector<std::vector<double>> ucost(n);
+    std::vector<double> set_cost(n, 0.0);
+    double all_cost = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const Setup &s = c.setups[i];
+        const RelaxSet &S = c.sets[i];
+        ucost[i].assign(S.n_rows, 1.0);
+        if (!units[i])
+            continue;
+        for (uint32_t r = 0; r < S.n_rows; ++r) {
+            double cr = 0.0;
+            for (uint32_t q = 0; q < S.n_prolif; ++q)
+                cr += walk_cost(s.row_info[r] & 255u, ps[i].t_max, ps[i].types[prolif[S.prolif_base + q]].mean);
+            ucost[i][r] = cr / S.n_prolif;
+            set_cost[i] += ucost[i][r] * (double)(s.start[r + 1] - s.start[r]) * S.n_prolif;
+        }
+        all_cost += set_cost[i];
+    }
+    std::vector<uint32_t> waves_of(n, 0);
+    uint64_t n_waves = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!units[i])
+            continue;
+        double share = all_cost > 0.0 ? set_cost[i] / all_cost * (double)budget : 1.0;
+        uint64_t wv = (uint64_t)std::max(1.0, floor(share + 0.5));
+        wv = std::min<uint64_t>(wv, (units[i] + 63) / 64);
+        waves_of[i] = (uint32_t)std::max<uint64_t>(1, wv);
+        n_waves += waves_of[i];
+    }
+    if (n_waves > 0x7FFFFFFFull)
+        return PROCELL_E_TOO_MANY_CELLS;
+    c.n_waves = (uint32_t)n_waves;
+    const uint32_t NW = std::max(1u, c.n_waves);
+    std::vector<uint32_t> wave_set(NW, 0), wave_row(2 * NW, 0);
+    std::vector<uint64_t> wave_unit(2 * NW, 0);
+    uint32_t wpos = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!waves_of[i])
+            continue;
+        split_set(c.setups[i], ucost[i], c.sets[i].n_prolif, waves_of[i], c.sets[i].row_base,
+                  &wave_unit[2 * wpos], &wave_row[2 * wpos]);
+        for (uint32_t v = 0; v < waves_of[i]; ++v)
+            wave_set[wpos + v] = i;
+        wpos += waves_of[i];
+    }
+
+    // ---- upload ----
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_sets = take(c.sets.size() * sizeof(RelaxSet));
+    const size_t o_ws = take(wave_set.size() * 4), o_wu = take(wave_unit.size() * 8);
+    const size_t o_wr = take(wave_row.size() * 4), o_rs = take(row_start.size() * 4);
+    const size_t o_ri = take(row_info.size() * 4 + 4), o_rt = take(row_set.size() * 4 + 4);
+    const size_t o_pr = take(prolif.size() * 4 + 4), o_qu = take(quiet.size() * 4 + 4);
+    const size_t o_ex = take(8);
+    if (hipMalloc(&c.d_block, off) != hipSuccess)
+        return PROCELL_E_OUT_OF_MEMORY;
+    char *base = (char *)c.d_block;
+    auto up = [&](size_t o, const void *src, size_t bytes) {
+        return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    RELAX_TRY(up(o_sets, c.sets.data(), c.sets.size() * sizeof(RelaxSet)));
+    RELAX_TRY(up(o_ws, wave_set.data(), wave_set.size() * 4));
+    RELAX_TRY(up(o_wu, wave_unit.data(), wave_unit.size() * 8));
+    RELAX_TRY(up(o_wr, wave_row.data(), wave_row.size() * 4));
+    RELAX_TRY(up(o_rs, row_start.data(), row_start.size() * 4));
+    RELAX_TRY(up(o_ri, row_info.data(), row_info.size() * 4));
+    RELAX_TRY(up(o_rt, row_set.data(), row_set.size() * 4));
+    RELAX_TRY(up(o_pr, prolif.data(), prolif.size() * 4));
+    RELAX_TRY(up(o_qu, quiet.data(), quiet.size() * 4));
+    RelaxArgs &a = c.args;
+    a.beta = beta;
+    a.n_sets = n;
+    a.n_rows = rows;
+    a.kdim_max = kdim_max;
+    a.sets = (const RelaxSet *)(base + o_sets);
+    a.wave_set = (const uint32_t *)(base + o_ws);
+    a.wave_unit = (const uint64_t *)(base + o_wu);
+    a.wave_row = (const uint32_t *)(base + o_wr);
+    a.row_start = (const uint32_t *)(base + o_rs);
+    a.row_info = (const uint32_t *)(base + o_ri);
+    a.row_set = (const uint32_t *)(base + o_rt);
+    a.prolif = (const uint32_t *)(base + o_pr);
+    a.quiet = (const uint32_t *)(base + o_qu);
+    a.expansions = (unsigned long long *)(base + o_ex);
+    if (c.n_waves) {
+        const size_t stack_bytes = (size_t)c.n_waves * kdim_max * kFrameFields * 64 * sizeof(double);
+        if (hipMalloc((void **)&c.d_stack, stack_bytes) != hipSuccess)
+            return PROCELL_E_OUT_OF_MEMORY;
+        a.stack = c.d_stack;
+    }
+    return PROCELL_OK;
+}
+
+// Enqueues one pass: the walk kernel and the quiescent kernel (the caller
+// has cleared h, or the gradient, and the expansion counter).
+hipError_t core_launch(const RelaxCore &c, const RelaxArgs &a, bool grad, hipStream_t st)
+{
+    if (c.any_prolif && c.n_waves) {
+        if (grad)
+            hipLaunchKernelGGL(relaxed_kernel<true>, dim3(c.n_waves), dim3(64), c.lds_bytes, st, a);
+        else
+            hipLaunchKernelGGL(relaxed_kernel<false>, dim3(c.n_waves), dim3(64), c.lds_bytes, st, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    if (c.any_quiet && a.n_rows) {
+        const dim3 g((a.n_rows + 255) / 256);
         if (grad)
             hipLaunchKernelGGL(relaxed_quiescent_kernel<true>, g, dim3(256), 0, st, a);
         else
             hipLaunchKernelGGL(relaxed_quiescent_kernel<false>, g, dim3(256), 0, st, a);
-        RELAX_TRY(hipGetLastError());
+        return hipGetLastError();
     }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Batched calibration: per set, the squared L2 distance between the
+// normalised relaxed histogram (h summed over types, (row, generation)
+// entries merged into the output bins value / 2^k) and a normalised target,
+// its cotangent dL/dh, and an Adam step on (log mean, log sd, proportion
+// logits).
+// ---------------------------------------------------------------------------
+struct CalibSet {
+    uint32_t bin_base;   // first output bin of the set (targets, per-bin scratch)
+    uint32_t n_bins;
+    uint32_t ptr_base;   // first entry of the set's bin -> entries CSR offsets (n_bins + 1 of them)
+    uint32_t ent_base;   // first (row, generation) entry of the set in ent_bin
+    uint32_t n_ent;      // rows x kdim
+    uint32_t pad[3];
+};
+
+struct CalibArgs {
+    uint32_t n_sets;
+    uint32_t pad;
+    double lr, beta1, beta2, eps;
+    const RelaxSet *sets;
+    const CalibSet *csets;
+    const uint32_t *bin_ptr;   // CSR offsets into bin_ent, per set
+    const uint32_t *bin_ent;   // set-local entry ids (row * kdim + k) of each bin
+    const int32_t *ent_bin;    // bin of each (row, generation) entry, or -1
+    const double *q;           // normalised targets per bin
+    const uint32_t *quiet_flag;  // per type of every set
+    double *hb;                // per-bin scratch: relaxed histogram, then dL/dhb
+    double *loss;              // [n_sets]
+    const double *h;
+    double *g;
+    const double *grad;        // dL/dtheta per set [3][T]
+    double *x;                 // per set [3][T]: log mean, log sd, logits
+    double *m1, *m2;           // Adam moments, layout of x
+    double *theta;             // per set [3][T]
+    unsigned long long *step;  // [1] Adam steps taken
+};
+
+__device__ double block_sum_256(double v, double *red)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    const uint32_t wv = threadIdx.x >> 6, ln = threadIdx.x & 63u;
+    __syncthreads();
+    if (ln == 0)
+        red[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (uint32_t i = 0; i < (blockDim.x >> 6); ++i)
+        s += red[i];
+    return s;
+}
+
+// One workgroup (256 threads) per set.  L = sum_b (hb_b / S - q_b)^2 with
+// S = sum_b hb_b; dL/dhb_a = (2 / S) (r_a - sum_b r_b hb_b / S), r = hb / S - q.
+__global__ __launch_bounds__(256) void calib_loss_kernel(CalibArgs c)
+{
+    __shared__ double red[4];
+    const uint32_t set = blockIdx.x;
+    const RelaxSet &S = c.sets[set];
+    const CalibSet &C = c.csets[set];
+    const uint32_t T = S.n_types;
+    const double *h = c.h + S.h_base;
+    double *hb = c.hb + C.bin_base;
+    const uint32_t *ptr = c.bin_ptr + C.ptr_base;
+    double part = 0.0;
+    for (uint32_t b = threadIdx.x; b < C.n_bins; b += blockDim.x) {
+        double v = 0.0;
+        for (uint32_t e = ptr[b]; e < ptr[b + 1]; ++e) {
+            const uint64_t at = (uint64_t)c.bin_ent[e] * T;
+            for (uint32_t j = 0; j < T; ++j)
+                v += h[at + j];
+        }
+        hb[b] = v;
+        part += v;
+    }
+    const double tot = block_sum_256(part, red);
+    const double inv = tot > 0.0 ? 1.0 / tot : 0.0;
+    double pl = 0.0, pc = 0.0;
+    for (uint32_t b = threadIdx.x; b < C.n_bins; b += blockDim.x) {
+        const double p = hb[b] * inv;
+        const double r = p - c.q[C.bin_base + b];
+        pl += r * r;
+        pc += r * p;
+        hb[b] = r;
+    }
+    const double L = block_sum_256(pl, red);
+    const double Cc = block_sum_256(pc, red);
+    for (uint32_t b = threadIdx.x; b < C.n_bins; b += blockDim.x)
+        hb[b] = tot > 0.0 ? 2.0 * inv * (hb[b] - Cc) : 0.0;
+    __syncthreads();
+    double *g = c.g + S.h_base;
+    for (uint32_t e = threadIdx.x; e < C.n_ent; e += blockDim.x) {
+        const int32_t b = c.ent_bin[C.ent_base + e];
+        const double gv = b >= 0 ? hb[b] : 0.0;
+        for (uint32_t j = 0; j < T; ++j)
+            g[(uint64_t)e * T + j] = gv;
+    }
+    if (threadIdx.x == 0)
+        c.loss[set] = L;
+}
+
+// theta of one set from its unconstrained parameters.
+__device__ void calib_theta(const CalibArgs &c, const RelaxSet &S)
+{
+    const uint32_t T = S.n_types, tb = S.theta_base;
+    const uint32_t *qf = c.quiet_flag + tb / 3;
+    double mx = c.x[tb + 2 * T];
+    for (uint32_t j = 1; j < T; ++j)
+        mx = fmax(mx, c.x[tb + 2 * T + j]);
+    double sum = 0.0;
+    for (uint32_t j = 0; j < T; ++j)
+        sum += exp(c.x[tb + 2 * T + j] - mx);
+    for (uint32_t j = 0; j < T; ++j) {
+        c.theta[tb + j] = qf[j] ? -1.0 : exp(c.x[tb + j]);
+        c.theta[tb + T + j] = qf[j] ? -1.0 : exp(c.x[tb + T + j]);
+        c.theta[tb + 2 * T + j] = exp(c.x[tb + 2 * T + j] - mx) / sum;
+    }
+}
+
+__device__ __forceinline__ void adam_update(double &x, double &m, double &v, double gr, const CalibArgs &c,
+                                            double bc1, double bc2)
+{
+    m = c.beta1 * m + (1.0 - c.beta1) * gr;
+    v = c.beta2 * v + (1.0 - c.beta2) * gr * gr;
+    const double denom = sqrt(v) / sqrt(bc2) + c.eps;
+    x = x - (c.lr / bc1) * (m / denom);
+}
+
+// One workgroup; thread s updates set s (and s + blockDim, ...).  Chain rule
+// to the unconstrained parameters: d/dlog mean = dL/dmean mean, d/dlog sd =
+// dL/dsd sd, d/dlogit_j = p_j (dL/dp_j - sum_k p_k dL/dp_k).
+__global__ __launch_bounds__(256) void calib_adam_kernel(CalibArgs c)
+{
+    const unsigned long long t = c.step[0] + 1ull;
+    const double bc1 = 1.0 - pow(c.beta1, (double)t), bc2 = 1.0 - pow(c.beta2, (double)t);
+    for (uint32_t set = threadIdx.x; set < c.n_sets; set += blockDim.x) {
+        const RelaxSet &S = c.sets[set];
+        const uint32_t T = S.n_types, tb = S.theta_base;
+        const uint32_t *qf = c.quiet_flag + tb / 3;
+        const double *th = c.theta + tb;
+        const double *gr = c.grad + tb;
+        double dot = 0.0;
+        for (uint32_t j = 0; j < T; ++j)
+            dot += th[2 * T + j] * gr[2 * T + j];
+        for (uint32_t j = 0; j < T; ++j) {
+            if (!qf[j]) {
+                adam_update(c.x[tb + j], c.m1[tb + j], c.m2[tb + j], gr[j] * th[j], c, bc1, bc2);
+                adam_update(c.x[tb + T + j], c.m1[tb + T + j], c.m2[tb + T + j], gr[T + j] * th[T + j], c, bc1,
+                            bc2);
+            }
+            const uint32_t z = tb + 2 * T + j;
+            adam_update(c.x[z], c.m1[z], c.m2[z], th[2 * T + j] * (gr[2 * T + j] - dot), c, bc1, bc2);
+        }
+        calib_theta(c, S);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        c.step[0] = t;
+}
+
+__global__ void calib_init_kernel(CalibArgs c)
+{
+    for (uint32_t set = blockIdx.x * blockDim.x + threadIdx.x; set < c.n_sets; set += gridDim.x * blockDim.x)
+        calib_theta(c, c.sets[set]);
+}
+
+}  // namespace
+}  // namespace procell
+
+using namespace procell;
+
+struct procell_relaxed {
+    RelaxCore core;
+};
+
+struct procell_calib {
+    RelaxCore core;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    CalibArgs cargs{};
+    RelaxArgs fwd{}, bwd{};
+    void *d_calib = nullptr;        // tables of the loss kernel
+    void *d_state = nullptr;        // x, m1, m2, theta, grad, h, g, per-bin scratch, loss, step
+    unsigned long long *d_clear = nullptr;  // h | grad | expansion counter: cleared every step
+    size_t clear_words = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace {
+
+void destroy_relaxed(procell_relaxed *r)
+{
+    if (!r)
+        return;
+    core_free(r->core);
+    delete r;
+}
+
+void destroy_calib(procell_calib *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->core.device);
+    if (c->graph_exec)
+        (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph)
+        (void)hipGraphDestroy(c->graph);
+    if (c->d_calib)
+        (void)hipFree(c->d_calib);
+    if (c->d_state)
+        (void)hipFree(c->d_state);
+    if (c->own_stream && c->stream)
+        (void)hipStreamDestroy(c->stream);
+    core_free(c->core);
+    delete c;
+}
+
+int relaxed_call(procell_relaxed *r, bool grad, const double *theta, double *h, const double *g, double *gr,
+                 void *stream)
+{
+    RELAX_TRY(hipSetDevice(r->core.device));
+    hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
+    RelaxArgs a = r->core.args;
+    a.theta = theta;
+    a.h = h;
+    a.g = g;
+    a.grad = gr;
+    RELAX_TRY(launch_zero(a.expansions, 1, st));
+    if (!grad)
+        RELAX_TRY(launch_zero((unsigned long long *)h, r->core.h_total, st));
+    else
+        RELAX_TRY(launch_zero((unsigned long long *)gr, 3ull * r->core.types_total, st));
+    RELAX_TRY(core_launch(r->core, a, grad, st));
     return PROCELL_OK;
+}
+
+// One calibration step: clear, forward, loss and cotangent, adjoint, Adam.
+hipError_t calib_enqueue(procell_calib *c, hipStream_t st)
+{
+    hipError_t e = launch_zero(c->d_clear, c->clear_words, st);
+    if (e != hipSuccess)
+        return e;
+    if ((e = core_launch(c->core, c->fwd, false, st)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(calib_loss_kernel, dim3(c->core.n_sets), dim3(256), 0, st, c->cargs);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    if ((e = core_launch(c->core, c->bwd, true, st)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(calib_adam_kernel, dim3(1), dim3(256), 0, st, c->cargs);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -424,139 +927,10 @@ int procell_relaxed_create(const procell_params *p, double beta, procell_relaxed
     procell_relaxed *r = new (std::nothrow) procell_relaxed();
     if (!r)
         return PROCELL_E_OUT_OF_MEMORY;
-    int rc = build_setup(p, r->s, false);
+    const int rc = core_build(r->core, p, 1, beta, false);
     if (rc != PROCELL_OK) {
-        delete r;
-        return rc;
-    }
-    int n_cu = 0;
-    rc = relaxed_device(p->device, &n_cu);
-    if (rc != PROCELL_OK) {
-        delete r;
-        return rc;
-    }
-    r->device = p->device;
-    auto fail = [&](int code) {
         destroy_relaxed(r);
-        return code;
-    };
-    const Setup &s = r->s;
-    RelaxArgs &a = r->args;
-    a.seed = p->seed;
-    a.t_max = p->t_max;
-    a.beta = beta;
-    a.n_rows = (uint32_t)s.v.size();
-    a.kdim = (uint32_t)s.kmax + 1;
-    a.n_types = p->n_types;
-    std::vector<uint32_t> prolif, quiet;
-    for (uint32_t j = 0; j < p->n_types; ++j)
-        (p->types[j].mean < 0.0 ? quiet : prolif).push_back(j);
-    a.n_prolif = (uint32_t)prolif.size();
-    a.n_quiet = (uint32_t)quiet.size();
-
-    // ---- wavefronts: resident ones only (each owns a stack) ----
-    r->lds_bytes = relaxed_lds_bytes(std::max(1u, a.n_types));
-    int occ = 0;
-    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, relaxed_kernel<true>, 64, r->lds_bytes));
-    int occ_f = 0;
-    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, relaxed_kernel<false>, 64, r->lds_bytes));
-    occ = std::min(occ, occ_f);
-    if (occ < 1)
-        return fail(PROCELL_E_HIP);
-    if (p->tuning.blocks_per_cu)
-        occ = std::min(occ, (int)p->tuning.blocks_per_cu);
-    const uint64_t n_units = s.n_cells * (uint64_t)a.n_prolif;
-    uint64_t n_waves = (uint64_t)occ * (uint64_t)n_cu;
-    n_waves = std::min<uint64_t>(n_waves, std::max<uint64_t>(1, (n_units + 63) / 64));
-    r->n_waves = n_units ? (uint32_t)n_waves : 0u;
-
-    // ---- balanced ranges of walks: unit u is (cell u / n_prolif, type
-    //      prolif[u % n_prolif]) ----
-    const uint32_t W = std::max(1u, r->n_waves);
-    std::vector<uint64_t> wave_unit(W + 1, n_units);
-    std::vector<uint32_t> wave_row(2 * W, 0);
-    if (n_units) {
-        std::vector<double> row_cost(a.n_rows);
-        double total = 0.0;
-        for (uint32_t i = 0; i < a.n_rows; ++i) {
-            const uint32_t kc = s.row_info[i] & 255u;
-            double c = 0.0;
-            for (uint32_t j : prolif)
-                c += walk_cost(kc, p->t_max, p->types[j].mean);
-            row_cost[i] = c / a.n_prolif;  // per unit
-            total += row_cost[i] * (double)(s.start[i + 1] - s.start[i]) * a.n_prolif;
-        }
-        wave_unit[0] = 0;
-        uint32_t wv = 1;
-        double acc = 0.0;
-        for (uint32_t i = 0; i < a.n_rows && wv < W; ++i) {
-            const uint64_t u0 = s.start[i] * a.n_prolif, u1 = s.start[i + 1] * a.n_prolif;
-            const double per = row_cost[i];
-            while (wv < W) {
-                const double target = total * (double)wv / (double)W;
-                const double need = (target - acc) / per;
-                if (need >= (double)(u1 - u0))
-                    break;
-                uint64_t cut = u0 + (uint64_t)std::max(0.0, ceil(need));
-                cut = std::min(cut, u1);
-                wave_unit[wv] = std::max(cut, wave_unit[wv - 1]);
-                ++wv;
-            }
-            acc += per * (double)(u1 - u0);
-        }
-        for (; wv <= W; ++wv)
-            wave_unit[wv] = n_units;
-        for (uint32_t v = 1; v <= W; ++v)
-            wave_unit[v] = std::max(wave_unit[v], wave_unit[v - 1]);
-        auto row_of = [&](uint64_t cell) {
-            return (uint32_t)(std::upper_bound(s.start.begin(), s.start.end() - 1, cell) - s.start.begin() - 1);
-        };
-        for (uint32_t v = 0; v < W; ++v) {
-            if (wave_unit[v] >= wave_unit[v + 1])
-                continue;
-            wave_row[2 * v] = row_of(wave_unit[v] / a.n_prolif);
-            wave_row[2 * v + 1] = row_of((wave_unit[v + 1] - 1) / a.n_prolif);
-        }
-    }
-
-    // ---- device tables ----
-    std::vector<uint32_t> row_start(a.n_rows + 1);
-    for (uint32_t i = 0; i <= a.n_rows; ++i)
-        row_start[i] = (uint32_t)s.start[i];
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-        const size_t o = off;
-        off = (off + bytes + 255) / 256 * 256;
-        return o;
-    };
-    const size_t o_rs = take(row_start.size() * 4), o_ri = take(s.row_info.size() * 4 + 4);
-    const size_t o_wu = take(wave_unit.size() * 8), o_wr = take(wave_row.size() * 4);
-    const size_t o_pr = take(prolif.size() * 4 + 4), o_qu = take(quiet.size() * 4 + 4);
-    const size_t o_ex = take(8);
-    if (hipMalloc(&r->d_block, off) != hipSuccess)
-        return fail(PROCELL_E_OUT_OF_MEMORY);
-    char *base = (char *)r->d_block;
-    auto up = [&](size_t o, const void *src, size_t bytes) {
-        return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
-    };
-    RELAX_TRY(up(o_rs, row_start.data(), row_start.size() * 4));
-    RELAX_TRY(up(o_ri, s.row_info.data(), s.row_info.size() * 4));
-    RELAX_TRY(up(o_wu, wave_unit.data(), wave_unit.size() * 8));
-    RELAX_TRY(up(o_wr, wave_row.data(), wave_row.size() * 4));
-    RELAX_TRY(up(o_pr, prolif.data(), prolif.size() * 4));
-    RELAX_TRY(up(o_qu, quiet.data(), quiet.size() * 4));
-    a.row_start = (const uint32_t *)(base + o_rs);
-    a.row_info = (const uint32_t *)(base + o_ri);
-    a.wave_unit = (const uint64_t *)(base + o_wu);
-    a.wave_row = (const uint32_t *)(base + o_wr);
-    a.prolif = (const uint32_t *)(base + o_pr);
-    a.quiet = (const uint32_t *)(base + o_qu);
-    a.expansions = (unsigned long long *)(base + o_ex);
-    if (r->n_waves) {
-        const size_t stack_bytes = (size_t)r->n_waves * a.kdim * kFrameFields * 64 * sizeof(double);
-        if (hipMalloc((void **)&r->d_stack, stack_bytes) != hipSuccess)
-            return fail(PROCELL_E_OUT_OF_MEMORY);
-        a.stack = r->d_stack;
+        return rc;
     }
     *out = r;
     return PROCELL_OK;
@@ -566,12 +940,13 @@ int procell_relaxed_shape(const procell_relaxed *r, uint64_t *n_rows, uint32_t *
 {
     if (!r)
         return PROCELL_E_INVALID;
+    const RelaxSet &S = r->core.sets[0];
     if (n_rows)
-        *n_rows = r->args.n_rows;
+        *n_rows = S.n_rows;
     if (kdim)
-        *kdim = r->args.kdim;
+        *kdim = S.kdim;
     if (n_types)
-        *n_types = r->args.n_types;
+        *n_types = S.n_types;
     return PROCELL_OK;
 }
 
@@ -579,11 +954,7 @@ int procell_relaxed_forward(procell_relaxed *r, const double *theta, double *h, 
 {
     if (!r || !theta || !h)
         return PROCELL_E_INVALID;
-    r->args.theta = theta;
-    r->args.h = h;
-    r->args.g = nullptr;
-    r->args.grad = nullptr;
-    return relaxed_launch(r, false, stream);
+    return relaxed_call(r, false, theta, h, nullptr, nullptr, stream);
 }
 
 int procell_relaxed_backward(procell_relaxed *r, const double *theta, const double *g, double *grad,
@@ -591,21 +962,17 @@ int procell_relaxed_backward(procell_relaxed *r, const double *theta, const doub
 {
     if (!r || !theta || !g || !grad)
         return PROCELL_E_INVALID;
-    r->args.theta = theta;
-    r->args.h = nullptr;
-    r->args.g = g;
-    r->args.grad = grad;
-    return relaxed_launch(r, true, stream);
+    return relaxed_call(r, true, theta, nullptr, g, grad, stream);
 }
 
 int procell_relaxed_expansions(procell_relaxed *r, uint64_t *n)
 {
     if (!r || !n)
         return PROCELL_E_INVALID;
-    RELAX_TRY(hipSetDevice(r->device));
+    RELAX_TRY(hipSetDevice(r->core.device));
     RELAX_TRY(hipDeviceSynchronize());
     unsigned long long v = 0;
-    RELAX_TRY(hipMemcpy(&v, r->args.expansions, sizeof v, hipMemcpyDeviceToHost));
+    RELAX_TRY(hipMemcpy(&v, r->core.args.expansions, sizeof v, hipMemcpyDeviceToHost));
     *n = v;
     return PROCELL_OK;
 }
@@ -613,6 +980,266 @@ int procell_relaxed_expansions(procell_relaxed *r, uint64_t *n)
 int procell_relaxed_destroy(procell_relaxed *r)
 {
     destroy_relaxed(r);
+    return PROCELL_OK;
+}
+
+int procell_calib_create(const procell_params *sets, uint32_t n_sets, double beta, const procell_result *targets,
+                         const procell_adam *adam, procell_calib **out)
+{
+    if (!sets || !targets || !adam || !out || n_sets == 0)
+        return PROCELL_E_INVALID;
+    *out = nullptr;
+    if (!(adam->lr > 0.0) || !(adam->beta1 >= 0.0 && adam->beta1 < 1.0) || !(adam->beta2 >= 0.0 && adam->beta2 < 1.0) ||
+        !(adam->eps > 0.0))
+        return PROCELL_E_INVALID;
+    procell_calib *c = new (std::nothrow) procell_calib();
+    if (!c)
+        return PROCELL_E_OUT_OF_MEMORY;
+    auto fail = [&](int code) {
+        destroy_calib(c);
+        return code;
+    };
+#define CALIB_TRY(expr)                       \
+    do {                                      \
+        hipError_t e_ = (expr);               \
+        if (e_ != hipSuccess) {               \
+            g_relaxed_hip = e_;               \
+            return fail(PROCELL_E_HIP);       \
+        }                                     \
+    } while (0)
+    int rc = core_build(c->core, sets, n_sets, beta, true);
+    if (rc != PROCELL_OK)
+        return fail(rc);
+    RelaxCore &core = c->core;
+    for (uint32_t i = 0; i < n_sets; ++i)
+        for (uint32_t j = 0; j < sets[i].n_types; ++j)
+            if (sets[i].types[j].mean >= 0.0 && !(sets[i].types[j].mean > 0.0 && sets[i].types[j].sd > 0.0 &&
+                                                  sets[i].types[j].proportion > 0.0))
+                return fail(PROCELL_E_INVALID);  // log parameters need mean, sd, proportion > 0
+
+    // ---- bins, targets, (row, generation) -> bin maps ----
+    std::vector<CalibSet> cs(n_sets);
+    std::vector<uint32_t> bin_ptr, bin_ent;
+    std::vector<int32_t> ent_bin;
+    std::vector<double> q;
+    for (uint32_t i = 0; i < n_sets; ++i) {
+        const Setup &s = core.setups[i];
+        const RelaxSet &S = core.sets[i];
+        CalibSet &C = cs[i];
+        memset(&C, 0, sizeof C);
+        C.bin_base = (uint32_t)q.size();
+        C.n_bins = (uint32_t)s.bins.size();
+        C.ptr_base = (uint32_t)bin_ptr.size();
+        C.ent_base = (uint32_t)ent_bin.size();
+        C.n_ent = S.n_rows * S.kdim;
+        const procell_result &tr = targets[i];
+        if (tr.n_out && (!tr.value || !tr.count))
+            return fail(PROCELL_E_INVALID);
+        std::vector<double> tq(C.n_bins, 0.0);
+        double tsum = 0.0;
+        for (uint64_t r = 0; r < tr.n_out; ++r) {
+            auto it = std::lower_bound(s.bins.begin(), s.bins.end(), tr.value[r]);
+            if (it == s.bins.end() || *it != tr.value[r] || tr.count[r] < 0)
+                return fail(PROCELL_E_INVALID);  // a target value outside the set's output bins
+            tq[it - s.bins.begin()] += (double)tr.count[r];
+            tsum += (double)tr.count[r];
+        }
+        for (double &x : tq)
+            q.push_back(tsum > 0.0 ? x / tsum : 0.0);
+        std::vector<std::vector<uint32_t>> members(C.n_bins);
+        for (uint32_t r = 0; r < S.n_rows; ++r) {
+            const uint32_t kc = s.row_info[r] & 255u;
+            const bool ge = ((s.row_info[r] >> 8) & 1u) != 0;
+            double f = s.v[r];
+            for (uint32_t k = 0; k < S.kdim; ++k) {
+                int32_t b = -1;
+                if (k <= kc && (k > 0 || ge)) {
+                    auto it = std::lower_bound(s.bins.begin(), s.bins.end(), f);
+                    if (it != s.bins.end() && *it == f) {
+                        b = (int32_t)(it - s.bins.begin());
+                        members[b].push_back(r * S.kdim + k);
+                    }
+                }
+                ent_bin.push_back(b);
+                f = f / 2;
+            }
+        }
+        for (uint32_t b = 0; b < C.n_bins; ++b) {
+            bin_ptr.push_back((uint32_t)bin_ent.size());
+            for (uint32_t e : members[b])
+                bin_ent.push_back(e);
+        }
+        bin_ptr.push_back((uint32_t)bin_ent.size());
+    }
+    // CSR offsets are relative to the whole bin_ent array
+    const size_t n_bins_total = q.size();
+
+    // ---- device memory ----
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_cs = take(cs.size() * sizeof(CalibSet)), o_bp = take(bin_ptr.size() * 4 + 4);
+    const size_t o_be = take(bin_ent.size() * 4 + 4), o_eb = take(ent_bin.size() * 4 + 4);
+    const size_t o_q = take(q.size() * 8 + 8), o_qf = take(core.quiet_flag.size() * 4 + 4);
+    if (hipMalloc(&c->d_calib, off) != hipSuccess)
+        return fail(PROCELL_E_OUT_OF_MEMORY);
+    char *base = (char *)c->d_calib;
+    auto up = [&](size_t o, const void *src, size_t bytes) {
+        return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    CALIB_TRY(up(o_cs, cs.data(), cs.size() * sizeof(CalibSet)));
+    CALIB_TRY(up(o_bp, bin_ptr.data(), bin_ptr.size() * 4));
+    CALIB_TRY(up(o_be, bin_ent.data(), bin_ent.size() * 4));
+    CALIB_TRY(up(o_eb, ent_bin.data(), ent_bin.size() * 4));
+    CALIB_TRY(up(o_q, q.data(), q.size() * 8));
+    CALIB_TRY(up(o_qf, core.quiet_flag.data(), core.quiet_flag.size() * 4));
+
+    const size_t P = 3ull * core.types_total;
+    size_t so = 0;
+    auto stake = [&](size_t bytes) {
+        const size_t o = so;
+        so = (so + bytes + 255) / 256 * 256;
+        return o;
+    };
+    // cleared every step: h | grad | expansions (one contiguous block)
+    const size_t s_h = stake(0);
+    const size_t clear_bytes = core.h_total * 8 + P * 8 + 8;
+    so = (clear_bytes + 255) / 256 * 256;
+    const size_t s_x = stake(P * 8), s_m1 = stake(P * 8), s_m2 = stake(P * 8), s_th = stake(P * 8);
+    const size_t s_g = stake(core.h_total * 8 + 8), s_hb = stake(n_bins_total * 8 + 8);
+    const size_t s_loss = stake(n_sets * 8), s_step = stake(8);
+    if (hipMalloc(&c->d_state, so) != hipSuccess)
+        return fail(PROCELL_E_OUT_OF_MEMORY);
+    char *sb = (char *)c->d_state;
+    std::vector<double> x(P, 0.0);
+    for (uint32_t i = 0; i < n_sets; ++i) {
+        const RelaxSet &S = core.sets[i];
+        const uint32_t T = S.n_types;
+        for (uint32_t j = 0; j < sets[i].n_types; ++j) {
+            const procell_type &ty = sets[i].types[j];
+            if (ty.mean >= 0.0) {
+                x[S.theta_base + j] = log(ty.mean);
+                x[S.theta_base + T + j] = log(ty.sd);
+            }
+            x[S.theta_base + 2 * T + j] = ty.proportion > 0.0 ? log(ty.proportion) : -745.0;
+        }
+        for (uint32_t j = sets[i].n_types; j < T; ++j)
+            x[S.theta_base + 2 * T + j] = -745.0;  // padding type of an empty type table
+    }
+    CALIB_TRY(hipMemcpy(sb + s_x, x.data(), P * 8, hipMemcpyHostToDevice));
+    CALIB_TRY(hipMemset(sb + s_m1, 0, P * 8));
+    CALIB_TRY(hipMemset(sb + s_m2, 0, P * 8));
+    CALIB_TRY(hipMemset(sb + s_step, 0, 8));
+    CALIB_TRY(hipMemset(sb, 0, clear_bytes));
+    c->d_clear = (unsigned long long *)(sb + s_h);
+    c->clear_words = clear_bytes / 8;
+    double *d_h = (double *)(sb + s_h);
+    double *d_grad = d_h + core.h_total;
+    unsigned long long *d_exp = (unsigned long long *)(d_grad + P);
+
+    CalibArgs &ca = c->cargs;
+    ca.n_sets = n_sets;
+    ca.lr = adam->lr;
+    ca.beta1 = adam->beta1;
+    ca.beta2 = adam->beta2;
+    ca.eps = adam->eps;
+    ca.sets = core.args.sets;
+    ca.csets = (const CalibSet *)(base + o_cs);
+    ca.bin_ptr = (const uint32_t *)(base + o_bp);
+    ca.bin_ent = (const uint32_t *)(base + o_be);
+    ca.ent_bin = (const int32_t *)(base + o_eb);
+    ca.q = (const double *)(base + o_q);
+    ca.quiet_flag = (const uint32_t *)(base + o_qf);
+    ca.hb = (double *)(sb + s_hb);
+    ca.loss = (double *)(sb + s_loss);
+    ca.h = d_h;
+    ca.g = (double *)(sb + s_g);
+    ca.grad = d_grad;
+    ca.x = (double *)(sb + s_x);
+    ca.m1 = (double *)(sb + s_m1);
+    ca.m2 = (double *)(sb + s_m2);
+    ca.theta = (double *)(sb + s_th);
+    ca.step = (unsigned long long *)(sb + s_step);
+    c->fwd = core.args;
+    c->fwd.theta = ca.theta;
+    c->fwd.h = d_h;
+    c->fwd.expansions = d_exp;
+    c->bwd = c->fwd;
+    c->bwd.h = nullptr;
+    c->bwd.g = ca.g;
+    c->bwd.grad = d_grad;
+
+    if (sets[0].stream) {
+        c->stream = (hipStream_t)sets[0].stream;
+    } else {
+        CALIB_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    hipLaunchKernelGGL(calib_init_kernel, dim3((n_sets + 255) / 256), dim3(256), 0, c->stream, ca);
+    CALIB_TRY(hipGetLastError());
+    CALIB_TRY(hipStreamSynchronize(c->stream));
+    *out = c;
+    return PROCELL_OK;
+#undef CALIB_TRY
+}
+
+int procell_calib_step(procell_calib *c, uint32_t n_steps)
+{
+    if (!c)
+        return PROCELL_E_INVALID;
+    RELAX_TRY(hipSetDevice(c->core.device));
+    if (!c->graph_exec) {
+        RELAX_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t e = calib_enqueue(c, c->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            g_relaxed_hip = e != hipSuccess ? e : e2;
+            return PROCELL_E_HIP;
+        }
+        c->graph = g;
+        RELAX_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    }
+    for (uint32_t i = 0; i < n_steps; ++i)
+        RELAX_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+    return PROCELL_OK;
+}
+
+int procell_calib_theta(procell_calib *c, double *theta, uint64_t capacity)
+{
+    if (!c || !theta || capacity < 3ull * c->core.types_total)
+        return PROCELL_E_INVALID;
+    RELAX_TRY(hipSetDevice(c->core.device));
+    RELAX_TRY(hipStreamSynchronize(c->stream));
+    RELAX_TRY(hipMemcpy(theta, c->cargs.theta, 3ull * c->core.types_total * 8, hipMemcpyDeviceToHost));
+    return PROCELL_OK;
+}
+
+int procell_calib_loss(procell_calib *c, double *loss, uint32_t capacity, uint64_t *steps)
+{
+    if (!c || !loss || capacity < c->core.n_sets)
+        return PROCELL_E_INVALID;
+    RELAX_TRY(hipSetDevice(c->core.device));
+    RELAX_TRY(hipStreamSynchronize(c->stream));
+    RELAX_TRY(hipMemcpy(loss, c->cargs.loss, c->core.n_sets * 8ull, hipMemcpyDeviceToHost));
+    if (steps) {
+        unsigned long long s = 0;
+        RELAX_TRY(hipMemcpy(&s, c->cargs.step, 8, hipMemcpyDeviceToHost));
+        *steps = s;
+    }
+    return PROCELL_OK;
+}
+
+void *procell_calib_stream(procell_calib *c) { return c ? (void *)c->stream : nullptr; }
+
+int procell_calib_destroy(procell_calib *c)
+{
+    destroy_calib(c);
     return PROCELL_OK;
 }
 

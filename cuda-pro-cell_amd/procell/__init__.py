@@ -80,6 +80,11 @@ class ResultStruct(ctypes.Structure):
                 ("raw_tally_len", ctypes.c_uint64), ("stats", Stats)]
 
 
+class AdamStruct(ctypes.Structure):
+    _fields_ = [("lr", ctypes.c_double), ("beta1", ctypes.c_double), ("beta2", ctypes.c_double),
+                ("eps", ctypes.c_double)]
+
+
 class ProcellError(RuntimeError):
     def __init__(self, code: int, what: str = ""):
         msg = strerror(code) if _lib is not None else str(code)
@@ -98,6 +103,8 @@ EXPORTS = (
     "procell_batch_create", "procell_batch_run", "procell_batch_result", "procell_batch_autotune",
     "procell_relaxed_create", "procell_relaxed_shape", "procell_relaxed_forward", "procell_relaxed_backward",
     "procell_relaxed_expansions", "procell_relaxed_destroy",
+    "procell_calib_create", "procell_calib_step", "procell_calib_theta", "procell_calib_loss",
+    "procell_calib_stream", "procell_calib_destroy",
 )
 
 
@@ -152,6 +159,15 @@ def lib():
                                                ctypes.c_void_p, ctypes.c_void_p]
         L.procell_relaxed_expansions.argtypes = [ctypes.c_void_p, _u64p]
         L.procell_relaxed_destroy.argtypes = [ctypes.c_void_p]
+    if hasattr(L, "procell_calib_create"):
+        L.procell_calib_create.argtypes = [P, ctypes.c_uint32, ctypes.c_double, R, ctypes.POINTER(AdamStruct),
+                                           ctypes.POINTER(ctypes.c_void_p)]
+        L.procell_calib_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        L.procell_calib_theta.argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint64]
+        L.procell_calib_loss.argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint32, _u64p]
+        L.procell_calib_stream.restype = ctypes.c_void_p
+        L.procell_calib_stream.argtypes = [ctypes.c_void_p]
+        L.procell_calib_destroy.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
 
@@ -419,6 +435,83 @@ class Relaxed:
     def close(self):
         if self.handle:
             lib().procell_relaxed_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Calibration:
+    """procell_calib_*: batched calibration of many parameter sets, each to
+    its own target histogram, with the relaxed estimator at temperature
+    `beta` and an on-device Adam step replayed from one hipGraph.
+
+    problems: their histograms, t_max, phi, relaxed-stream seeds and the
+    starting parameters; targets: per set (value, count) arrays of a target
+    histogram in the plain method's output form (e.g. a Result)."""
+
+    def __init__(self, problems, beta, targets, lr=0.02, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+        self.problems = list(problems)
+        n = len(self.problems)
+        if len(targets) != n:
+            raise ValueError("one target per parameter set")
+        arr = (Params * max(n, 1))(*[p.params for p in self.problems])
+        if stream is not None and n:
+            arr[0].stream = ctypes.c_void_p(int(stream))
+        self._keep = []
+        tr = (ResultStruct * max(n, 1))()
+        for i, tg in enumerate(targets):
+            value, count = (tg.value, tg.count) if hasattr(tg, "value") else tg
+            v = np.ascontiguousarray(value, dtype=np.float64)
+            c = np.ascontiguousarray(count, dtype=np.int64)
+            self._keep += [v, c]
+            tr[i].capacity = len(v)
+            tr[i].n_out = len(v)
+            tr[i].value = v.ctypes.data_as(_dp)
+            tr[i].count = c.ctypes.data_as(_i64p)
+        adam = AdamStruct(lr, beta1, beta2, eps)
+        self.handle = ctypes.c_void_p()
+        _check(lib().procell_calib_create(arr, n, float(beta), tr, ctypes.byref(adam), ctypes.byref(self.handle)),
+               "calib_create")
+        self.n_types = [max(1, p.n_types) for p in self.problems]
+
+    def step(self, n: int = 1):
+        _check(lib().procell_calib_step(self.handle, int(n)), "calib_step")
+
+    def theta(self):
+        """Current parameters: one [3, n_types] array (mean, sd, proportion) per set."""
+        out = np.zeros(3 * sum(self.n_types), dtype=np.float64)
+        _check(lib().procell_calib_theta(self.handle, out.ctypes.data_as(_dp), len(out)), "calib_theta")
+        res, at = [], 0
+        for T in self.n_types:
+            res.append(out[at: at + 3 * T].reshape(3, T).copy())
+            at += 3 * T
+        return res
+
+    def loss(self):
+        """(loss of each set at the last step's forward pass, steps taken)."""
+        out = np.zeros(max(len(self.problems), 1), dtype=np.float64)
+        steps = ctypes.c_uint64()
+        _check(lib().procell_calib_loss(self.handle, out.ctypes.data_as(_dp), len(out), ctypes.byref(steps)),
+               "calib_loss")
+        return out[: len(self.problems)], int(steps.value)
+
+    @property
+    def stream(self) -> int:
+        return lib().procell_calib_stream(self.handle) or 0
+
+    def close(self):
+        if self.handle:
+            lib().procell_calib_destroy(self.handle)
             self.handle = ctypes.c_void_p()
 
     def __enter__(self):

@@ -152,6 +152,48 @@ def calibration_instance(seed: int = 0) -> Workload:
     )
 
 
+def calibration_batch(seed: int = 0, n_sets: int = 256, cells: float = 2e5):
+    """Batched calibration problem: n_sets candidate histograms, each of
+    256 log-spaced bins from 1 to 1e4 with ~`cells` cells (lognormal-shaped
+    Poisson counts, mu = ln 800, sigma 1), generating types drawn per set --
+    1-3 proliferating types with means U(30,100) and sd = U(0.2,0.4) x mean,
+    plus a quiescent type when that makes 2-4 types in all (always for one
+    proliferating type, else half the time) -- ratios Dirichlet(2,...,2),
+    t_max U(96,240), phi = smallest populated bin.  Returns (generating
+    workloads, starting workloads): the start perturbs every mean and sd by
+    a factor U(0.85,1.15) and mixes the ratios halfway towards uniform."""
+    rng = np.random.default_rng([seed, 5])
+    values = [10.0 ** (4.0 * i / 255.0) for i in range(256)]
+    gen, start = [], []
+    for i in range(n_sets):
+        counts = _lognormal_counts(values, cells, math.log(800.0), 1.0, rng)
+        n_prolif = int(rng.integers(1, 4))
+        quiescent = n_prolif == 1 or bool(rng.integers(0, 2))
+        means = rng.uniform(30.0, 100.0, size=n_prolif)
+        sds = rng.uniform(0.2, 0.4, size=n_prolif) * means
+        if quiescent:
+            means = np.concatenate([means, [-1.0]])
+            sds = np.concatenate([sds, [-1.0]])
+        n_types = len(means)
+        ratios = rng.dirichlet(np.full(n_types, 2.0))
+        t_max = float(rng.uniform(96.0, 240.0))
+        w = Workload(name=f"calib{i}", values=np.array(values, dtype=np.float64), counts=counts,
+                     proportion=ratios.astype(np.float64), mean=means.astype(np.float64),
+                     sd=sds.astype(np.float64), t_max=t_max, phi=0.0, track_ratio=True,
+                     meta={"bins": 256, "target_cells": cells})
+        f_m = rng.uniform(0.85, 1.15, size=n_types)
+        f_s = rng.uniform(0.85, 1.15, size=n_types)
+        prolif = means >= 0
+        m0 = np.where(prolif, means * f_m, -1.0)
+        s0 = np.where(prolif, sds * f_s, -1.0)
+        p0 = 0.5 * ratios + 0.5 / n_types
+        p0 = p0 / p0.sum()
+        gen.append(w)
+        start.append(Workload(name=f"calib{i}_start", values=w.values, counts=counts, proportion=p0,
+                              mean=m0, sd=s0, t_max=t_max, phi=0.0, track_ratio=True, meta=dict(w.meta)))
+    return gen, start
+
+
 def scaled(w: Workload, factor: float, name: str | None = None) -> Workload:
     """Same shape with every count multiplied by `factor` (rounded down)."""
     counts = np.floor(w.counts.astype(np.float64) * factor).astype(np.uint64)

@@ -208,6 +208,43 @@ int procell_relaxed_backward(procell_relaxed *plan, const double *theta, const d
 int procell_relaxed_expansions(procell_relaxed *plan, uint64_t *n);
 int procell_relaxed_destroy(procell_relaxed *plan);
 
+/* ---- batched calibration ----------------------------------------------
+ * No reference counterpart.  Fits each parameter set's types to a target
+ * histogram with the relaxed estimator above: per set, the loss is the
+ * squared L2 distance between the normalised relaxed histogram (h summed
+ * over types, each (row, generation) entry merged into its output value
+ * v / 2^k) and the normalised target; one step runs, for all sets at once,
+ * the relaxed forward pass, the loss and its cotangent, the adjoint pass and
+ * an Adam update of (log mean, log sd) of every proliferating type and the
+ * proportion logits of every type (proportions = softmax of the logits), all
+ * on the device, replayed from one hipGraph captured at the first step.
+ *   sets    : [n_sets] histograms, t_max, phi, seeds of the relaxed streams
+ *             and the starting parameters (proliferating types need mean,
+ *             sd and proportion > 0); the device, and the stream when not
+ *             NULL, of sets[0] apply to the whole calibration.
+ *   targets : [n_sets] target histograms in the plain method's output form
+ *             (value[n_out], count[n_out]); every value must be one of the
+ *             set's output values (e.g. a procell_simulate result).
+ *   adam    : lr, beta1, beta2, eps (the update of Kingma & Ba, 2015). */
+typedef struct {
+    double lr;
+    double beta1;
+    double beta2;
+    double eps;
+} procell_adam;
+typedef struct procell_calib procell_calib;
+int procell_calib_create(const procell_params *sets, uint32_t n_sets, double beta, const procell_result *targets,
+                         const procell_adam *adam, procell_calib **calib);
+/* Enqueues n_steps steps (graph replays) on the calibration's stream. */
+int procell_calib_step(procell_calib *calib, uint32_t n_steps);
+/* Waits; current parameters: per set [3][n_types] (mean, sd, proportion;
+ * -1 for a quiescent type's mean and sd), sets concatenated. */
+int procell_calib_theta(procell_calib *calib, double *theta, uint64_t capacity);
+/* Waits; loss of each set at the last step's forward pass, and the steps taken. */
+int procell_calib_loss(procell_calib *calib, double *loss, uint32_t capacity, uint64_t *steps);
+void *procell_calib_stream(procell_calib *calib);                   /* hipStream_t */
+int procell_calib_destroy(procell_calib *calib);
+
 /* ---- text formats ------------------------------------------------------
  * Replace load_fluorescences' file reading (src/io/parser.cu:68-124),
  * load_cell_types (src/io/parser.cu:156-185) and save_fluorescences

@@ -1,0 +1,98 @@
+"""GPU: batched calibration against an independent composition.
+
+One calibration step (relaxed forward of all sets in one launch, loss and
+cotangent kernel, adjoint of all sets, on-device Adam) must equal, set by
+set, the single-set relaxed plan composed with torch: BinMap, the
+normalised-L2 loss, torch.autograd through RelaxedTally, exp and softmax,
+and torch.optim.Adam with the same hyper-parameters.  The device sums with
+FP64 atomics in its own order, so agreement is to 1e-9 relative (loss and
+parameters), checked after each of three steps.  Graph replays enqueued
+together equal steps taken one at a time.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LR, B1, B2, EPS, BETA = 0.05, 0.9, 0.999, 1e-8, 0.5
+
+
+def _sets(procell_lib, n=3, cells=3000.0):
+    from procell import workloads
+
+    gen, start = workloads.calibration_batch(11, n, cells)
+    targets = [procell_lib.simulate(procell_lib.Problem.from_workload(w, 100 + i), want_raw=False)
+               for i, w in enumerate(gen)]
+    probs = [procell_lib.Problem.from_workload(w, 7 + i) for i, w in enumerate(start)]
+    return gen, start, targets, probs
+
+
+def _torch_reference(procell_lib, prob, w, target, steps):
+    import torch
+    from procell import calibrate
+
+    plan = procell_lib.Relaxed(prob, BETA)
+    bm = calibrate.BinMap(prob, plan.shape[1])
+    tgt = torch.tensor(calibrate.target_histogram(bm.bins, target.value, target.count), dtype=torch.float64,
+                       device="cuda")
+    quiet = torch.tensor(w.mean < 0, device="cuda")
+    one = np.ones(len(w.mean))
+    lm = torch.tensor(np.log(np.where(w.mean < 0, one, w.mean)), dtype=torch.float64, device="cuda",
+                      requires_grad=True)
+    ls = torch.tensor(np.log(np.where(w.mean < 0, one, w.sd)), dtype=torch.float64, device="cuda",
+                      requires_grad=True)
+    z = torch.tensor(np.log(w.proportion), dtype=torch.float64, device="cuda", requires_grad=True)
+    opt = torch.optim.Adam([lm, ls, z], lr=LR, betas=(B1, B2), eps=EPS)
+    out = []
+    for _ in range(steps):
+        theta = calibrate.assemble_theta(lm, ls, z, quiet)
+        loss = calibrate.normalized_l2(bm(calibrate.relaxed_tally(plan, theta)), tgt)
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            out.append((float(loss), calibrate.assemble_theta(lm, ls, z, quiet).cpu().numpy()))
+    plan.close()
+    return out
+
+
+def test_steps_equal_torch_composition(procell_lib):
+    gen, start, targets, probs = _sets(procell_lib)
+    refs = [_torch_reference(procell_lib, p, w, t, 3) for p, w, t in zip(probs, start, targets)]
+    with procell_lib.Calibration(probs, BETA, targets, lr=LR, beta1=B1, beta2=B2, eps=EPS) as cal:
+        for step in range(3):
+            cal.step(1)
+            loss, n = cal.loss()
+            theta = cal.theta()
+            assert n == step + 1
+            for i, ref in enumerate(refs):
+                l_ref, th_ref = ref[step]
+                print("step", step, "set", i, loss[i], l_ref, flush=True)
+                assert abs(loss[i] - l_ref) <= 1e-9 * abs(l_ref), (step, i, loss[i], l_ref)
+                assert np.allclose(theta[i], th_ref, rtol=1e-9, atol=1e-12), (step, i, theta[i], th_ref)
+
+
+def test_graph_replays_equal_single_steps(procell_lib):
+    gen, start, targets, probs = _sets(procell_lib, 4, 2000.0)
+    with procell_lib.Calibration(probs, BETA, targets, lr=LR) as a, \
+            procell_lib.Calibration(probs, BETA, targets, lr=LR) as b:
+        for _ in range(5):
+            a.step(1)
+            a.theta()
+        b.step(5)
+        la, na = a.loss()
+        lb, nb = b.loss()
+        assert na == nb == 5
+        assert np.allclose(la, lb, rtol=1e-9, atol=0)
+        for x, y in zip(a.theta(), b.theta()):
+            assert np.allclose(x, y, rtol=1e-8, atol=1e-12)
+
+
+def test_invalid_targets_and_parameters(procell_lib):
+    gen, start, targets, probs = _sets(procell_lib, 2, 1000.0)
+    bad = [(np.array([12345.678]), np.array([3]))] + targets[1:]
+    with pytest.raises(procell_lib.ProcellError) as e:
+        procell_lib.Calibration(probs, BETA, bad)
+    assert e.value.code == procell_lib.E_INVALID
+    with pytest.raises(procell_lib.ProcellError):
+        procell_lib.Calibration(probs, 0.0, targets)
