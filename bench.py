@@ -262,6 +262,142 @@ def relaxed_measurement(procell, workloads, np, torch, with_cpu=True, reps=50, f
     return out
 
 
+def recovery(np, gen, thetas):
+    """Parameter recovery against the generating values: relative errors of
+    the proliferating types' means and sds, absolute errors of the
+    proportions (all sets pooled)."""
+    em, es, ep = [], [], []
+    for w, th in zip(gen, thetas):
+        pr = w.mean >= 0
+        em += list(np.abs(th[0][pr] - w.mean[pr]) / w.mean[pr])
+        es += list(np.abs(th[1][pr] - w.sd[pr]) / w.sd[pr])
+        ep += list(np.abs(th[2] - w.proportion))
+    em, es, ep = np.array(em), np.array(es), np.array(ep)
+    return {"mean_rel_median": float(np.median(em)), "mean_rel_p90": float(np.percentile(em, 90)),
+            "mean_within_2pct": float(np.mean(em <= 0.02)),
+            "sd_rel_median": float(np.median(es)), "sd_rel_p90": float(np.percentile(es, 90)),
+            "sd_within_2pct": float(np.mean(es <= 0.02)),
+            "proportion_abs_median": float(np.median(ep)), "proportion_within_0.02": float(np.mean(ep <= 0.02))}
+
+
+def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_rounds=300, with_nm=True):
+    """Batched calibration: 256 candidate parameter sets x 2e5 cells, 2-4
+    types, t_max U(96,240), targets from the plain method at the generating
+    parameters; 200 steps of relaxed forward + loss + adjoint + on-device
+    Adam replayed from one hipGraph per step.  Beside it, the derivative-free
+    baseline: Nelder-Mead per set over the plain method (common random
+    numbers), all sets evaluated by one batched plain run per round."""
+    from procell import calibrate
+
+    gen, start = workloads.calibration_batch(0, 256, 2e5)
+    n = len(gen)
+    tseeds = np.arange(n, dtype=np.uint64) + 1000
+    with procell.Batch([procell.Problem.from_workload(w, 0) for w in gen]) as b:
+        b.run(tseeds)
+        targets = [b.result(i, want_raw=False) for i in range(n)]
+    probs = [procell.Problem.from_workload(w, 50000 + i) for i, w in enumerate(start)]
+    cal = procell.Calibration(probs, 0.5, targets, lr=0.02)
+    traj, gpu_s, exp_step = [], 0.0, 0
+    rec0 = recovery(np, gen, [np.stack([w.mean, w.sd, w.proportion]) for w in start])
+    traj.append({"steps": 0, "gpu_s": 0.0, **rec0})
+    for k in range(0, steps, chunk):
+        t0 = time.perf_counter()
+        cal.step(chunk)
+        loss, done = cal.loss()
+        gpu_s += time.perf_counter() - t0
+        exp_step = cal.last_expansions
+        traj.append({"steps": done, "gpu_s": gpu_s, "loss_median": float(np.median(loss)),
+                     **recovery(np, gen, cal.theta())})
+    cal.close()
+    last = traj[-1]
+    log(f"calibration: {n} sets, {steps} steps, {gpu_s:.2f} s ({gpu_s / steps * 1e3:.2f} ms/step), "
+        f"{exp_step} expansions/step; mean rel err median {rec0['mean_rel_median']:.4f} -> "
+        f"{last['mean_rel_median']:.4f}")
+    out = {
+        "workload": "256 candidate sets x 2e5 cells: 256 log-spaced bins 1..1e4, 1-3 proliferating types "
+                    "(means U(30,100), sd 0.2-0.4 x mean) + quiescent (2-4 types), Dirichlet(2) ratios, "
+                    "t_max U(96,240), beta 0.5; start = generating x U(0.85,1.15) per mean / sd, ratios "
+                    "halfway to uniform (workloads.calibration_batch(0, 256, 2e5))",
+        "cells": int(sum(w.n_cells for w in gen)),
+        "steps": steps,
+        "ms_per_step": gpu_s / steps * 1e3,
+        "grad_steps_per_s": steps / gpu_s,
+        "set_steps_per_s": n * steps / gpu_s,
+        "expansions_per_step": exp_step,
+        "expansions_per_s": exp_step * steps / gpu_s,
+        "optimizer": "Adam lr 0.02 (log mean, log sd, proportion logits), on the device",
+        "targets": "plain method at the generating parameters, seeds 1000+i; relaxed seeds 50000+i",
+        "trajectory": traj,
+    }
+    if with_nm:
+        out["derivative_free"] = nelder_mead_baseline(procell, calibrate, np, gen, start, targets, nm_rounds, traj)
+    return out
+
+
+def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, grad_traj):
+    """Nelder-Mead per set over the plain method: one batched plain run of
+    all sets per round (each set's next simplex point, its own fixed seed),
+    GPU time = the batch runs' kernel times."""
+    n = len(gen)
+    prolif = [w.mean >= 0 for w in start]
+    qs, binsets = [], []
+    for w, tg in zip(gen, targets):
+        p = procell.Problem.from_workload(w, 0)
+        bins, _ = calibrate.output_bins(p, p.tally_shape()[1])
+        h = calibrate.target_histogram(bins, tg.value, tg.count)
+        binsets.append(bins)
+        qs.append(h / h.sum())
+    lo = np.log([5.0, 0.5])
+    hi = np.log([500.0, 500.0])
+    coro = [calibrate.nelder_mead(calibrate.pack(w.mean, w.sd, w.proportion), 0.1) for w in start]
+    xs = [next(c) for c in coro]
+    best = [(np.inf, x) for x in xs]
+    seeds = np.arange(n, dtype=np.uint64) + 70000
+    gpu_ms, traj, wall0 = 0.0, [], time.perf_counter()
+    for r in range(rounds):
+        probs = []
+        for i, (w, x) in enumerate(zip(start, xs)):
+            k = int(prolif[i].sum())
+            xe = x.copy()
+            xe[:k] = np.clip(xe[:k], lo[0], hi[0])
+            xe[k:2 * k] = np.clip(xe[k:2 * k], lo[1], hi[1])
+            m, s, p = calibrate.unpack(xe, prolif[i])
+            probs.append(procell.Problem(w.values, w.counts, p, m, s, w.t_max, w.phi, 0,
+                                         flags=procell.FLAG_TIMING if i == 0 else 0))
+        with procell.Batch(probs) as b:
+            b.run(seeds)
+            res = [b.result(i, want_raw=False) for i in range(n)]
+            gpu_ms += float(np.sum(b.kernel_times()))
+        fs = [calibrate.plain_loss(binsets[i], res[i].value, res[i].count, qs[i]) for i in range(n)]
+        for i in range(n):
+            if fs[i] < best[i][0]:
+                best[i] = (fs[i], xs[i].copy())
+        xs = [c.send(f) for c, f in zip(coro, fs)]
+        if (r + 1) % 25 == 0 or r + 1 == rounds:
+            th = [np.stack(calibrate.unpack(bx, prolif[i])) for i, (_, bx) in enumerate(best)]
+            traj.append({"rounds": r + 1, "gpu_s": gpu_ms * 1e-3, **recovery(np, gen, th)})
+    wall = time.perf_counter() - wall0
+    # GPU time each method needs to reach the gradient method's final median mean error
+    goal = grad_traj[-1]["mean_rel_median"]
+    t_grad = next((p["gpu_s"] for p in grad_traj if p["mean_rel_median"] <= goal), None)
+    t_nm = next((p["gpu_s"] for p in traj if p["mean_rel_median"] <= goal), None)
+    log(f"nelder-mead: {rounds} rounds, {gpu_ms * 1e-3:.2f} s GPU ({wall:.1f} s wall); mean rel err median "
+        f"{traj[-1]['mean_rel_median']:.4f}; goal {goal:.4f}: gradient {t_grad} s, nelder-mead {t_nm} s")
+    return {
+        "method": "Nelder-Mead (reflection 1, expansion 2, contraction 1/2, shrink 1/2, initial step 0.1) per set "
+                  "on log mean, log sd, log proportions; objective = normalised-L2 of the plain histogram "
+                  "(fixed seed per set) to the target",
+        "rounds": rounds,
+        "gpu_s": gpu_ms * 1e-3,
+        "wall_s": wall,
+        "trajectory": traj,
+        "goal_mean_rel_median": goal,
+        "gpu_s_to_goal_gradient": t_grad,
+        "gpu_s_to_goal_nelder_mead": t_nm,
+        "gpu_time_ratio": (t_nm / t_grad) if (t_nm and t_grad) else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +408,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-method measurement")
     ap.add_argument("--no-relaxed", action="store_true", help="skip the relaxed-estimator measurement")
     ap.add_argument("--fit-steps", type=int, default=300, help="Adam steps of the relaxed-estimator fit")
+    ap.add_argument("--no-calib", action="store_true", help="skip the batched-calibration measurement")
+    ap.add_argument("--calib-steps", type=int, default=200, help="steps of the batched calibration")
+    ap.add_argument("--nm-rounds", type=int, default=300, help="rounds of the Nelder-Mead baseline (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -390,6 +529,9 @@ def main():
     if not args.no_relaxed:
         out["relaxed"] = relaxed_measurement(procell, workloads, np, torch, with_cpu=not args.no_cpu,
                                              fit_steps=args.fit_steps)
+    if not args.no_calib:
+        out["calibration"] = calibration_measurement(procell, workloads, np, steps=args.calib_steps,
+                                                     nm_rounds=args.nm_rounds, with_nm=args.nm_rounds > 0)
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_budget)
         out["cpu_baseline"] = cb

@@ -52,7 +52,7 @@ namespace procell {
 namespace {
 
 constexpr double kRelaxEps = 1e-15;        // smallest W d that is still expanded
-constexpr uint32_t kFrameFields = 6;       // s, b, W, d, dlogW/dmean, dlogW/dsd
+constexpr uint32_t kFrameFields = 6;       // s, W, d (both passes), b, dlogW/dmean, dlogW/dsd (adjoint)
 constexpr uint32_t kWindowDoubles = 1024;  // LDS window of a wavefront (8 KB)
 
 // One parameter set of a relaxed plan (a single-set plan has one).  Rows,
@@ -255,11 +255,13 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
         if (expA && expB) {
             double *f = stk + (uint64_t)kA * kFrameFields * 64;
             f[0 * 64] = sB;
-            f[1 * 64] = bB;
-            f[2 * 64] = WA;
-            f[3 * 64] = dB;
-            f[4 * 64] = gmA;
-            f[5 * 64] = gsA;
+            f[1 * 64] = WA;
+            f[2 * 64] = dB;
+            if (Grad) {  // the forward pass needs no tangents
+                f[3 * 64] = bB;
+                f[4 * 64] = gmA;
+                f[5 * 64] = gsA;
+            }
             pending |= 1ull << kA;
         }
         if (expA || expB) {
@@ -279,11 +281,13 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             k = L;
             const double *f = stk + (uint64_t)L * kFrameFields * 64;
             s = f[0 * 64];
-            b = f[1 * 64];
-            W = f[2 * 64];
-            d = f[3 * 64];
-            gm = f[4 * 64];
-            gs = f[5 * 64];
+            W = f[1 * 64];
+            d = f[2 * 64];
+            if (Grad) {
+                b = f[3 * 64];
+                gm = f[4 * 64];
+                gs = f[5 * 64];
+            }
         } else {
             if (Grad) {
                 atomicAdd(gsum + j, acc_m);
@@ -1220,7 +1224,7 @@ int procell_calib_theta(procell_calib *c, double *theta, uint64_t capacity)
     return PROCELL_OK;
 }
 
-int procell_calib_loss(procell_calib *c, double *loss, uint32_t capacity, uint64_t *steps)
+int procell_calib_loss(procell_calib *c, double *loss, uint32_t capacity, uint64_t *steps, uint64_t *expansions)
 {
     if (!c || !loss || capacity < c->core.n_sets)
         return PROCELL_E_INVALID;
@@ -1231,6 +1235,11 @@ int procell_calib_loss(procell_calib *c, double *loss, uint32_t capacity, uint64
         unsigned long long s = 0;
         RELAX_TRY(hipMemcpy(&s, c->cargs.step, 8, hipMemcpyDeviceToHost));
         *steps = s;
+    }
+    if (expansions) {
+        unsigned long long s = 0;
+        RELAX_TRY(hipMemcpy(&s, c->fwd.expansions, 8, hipMemcpyDeviceToHost));
+        *expansions = s;
     }
     return PROCELL_OK;
 }

@@ -164,7 +164,7 @@ def lib():
                                            ctypes.POINTER(ctypes.c_void_p)]
         L.procell_calib_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
         L.procell_calib_theta.argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint64]
-        L.procell_calib_loss.argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint32, _u64p]
+        L.procell_calib_loss.argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint32, _u64p, _u64p]
         L.procell_calib_stream.restype = ctypes.c_void_p
         L.procell_calib_stream.argtypes = [ctypes.c_void_p]
         L.procell_calib_destroy.argtypes = [ctypes.c_void_p]
@@ -500,9 +500,10 @@ class Calibration:
     def loss(self):
         """(loss of each set at the last step's forward pass, steps taken)."""
         out = np.zeros(max(len(self.problems), 1), dtype=np.float64)
-        steps = ctypes.c_uint64()
-        _check(lib().procell_calib_loss(self.handle, out.ctypes.data_as(_dp), len(out), ctypes.byref(steps)),
-               "calib_loss")
+        steps, exp = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().procell_calib_loss(self.handle, out.ctypes.data_as(_dp), len(out), ctypes.byref(steps),
+                                        ctypes.byref(exp)), "calib_loss")
+        self.last_expansions = int(exp.value)
         return out[: len(self.problems)], int(steps.value)
 
     @property

@@ -153,3 +153,71 @@ def fit(plan: Relaxed, prob: Problem, target: np.ndarray, theta0: np.ndarray, st
     with torch.no_grad():
         theta = assemble_theta(log_mean, log_sd, logits, quiet)
     return theta.cpu().numpy(), [float(x) for x in torch.stack(losses).cpu()] if losses else []
+
+
+# ---------------------------------------------------------------------------
+# Derivative-free baseline: Nelder-Mead over the plain method.
+# ---------------------------------------------------------------------------
+def pack(mean, sd, proportion):
+    """Unconstrained vector of one set: log mean and log sd of each
+    proliferating type, then the log proportions of all types."""
+    pr = np.asarray(mean) >= 0
+    return np.concatenate([np.log(np.asarray(mean)[pr]), np.log(np.asarray(sd)[pr]),
+                           np.log(np.maximum(proportion, 1e-300))])
+
+
+def unpack(x, prolif):
+    """Inverse of pack: (mean, sd, proportion) with -1 for quiescent types."""
+    prolif = np.asarray(prolif, dtype=bool)
+    n = int(prolif.sum())
+    mean = np.full(len(prolif), -1.0)
+    sd = np.full(len(prolif), -1.0)
+    mean[prolif] = np.exp(x[:n])
+    sd[prolif] = np.exp(x[n:2 * n])
+    z = x[2 * n:]
+    p = np.exp(z - z.max())
+    return mean, sd, p / p.sum()
+
+
+def nelder_mead(x0, step=0.1):
+    """Nelder & Mead (1965) simplex search as a coroutine: yields the next
+    point to evaluate and receives its objective value through send().
+    Standard coefficients: reflection 1, expansion 2, contraction 1/2,
+    shrink 1/2; initial simplex x0 + step e_i."""
+    d = len(x0)
+    pts = [np.array(x0, dtype=np.float64)] + [np.array(x0, dtype=np.float64) + step * np.eye(d)[i]
+                                               for i in range(d)]
+    f = []
+    for p in pts:
+        f.append((yield p))
+    while True:
+        order = np.argsort(f)
+        pts = [pts[i] for i in order]
+        f = [f[i] for i in order]
+        xo = np.mean(pts[:-1], axis=0)
+        xr = xo + (xo - pts[-1])
+        fr = yield xr
+        if f[0] <= fr < f[-2]:
+            pts[-1], f[-1] = xr, fr
+        elif fr < f[0]:
+            xe = xo + 2.0 * (xr - xo)
+            fe = yield xe
+            pts[-1], f[-1] = (xe, fe) if fe < fr else (xr, fr)
+        else:
+            xc = xo + 0.5 * ((xr if fr < f[-1] else pts[-1]) - xo)
+            fc = yield xc
+            if fc < min(fr, f[-1]):
+                pts[-1], f[-1] = xc, fc
+            else:
+                for i in range(1, d + 1):
+                    pts[i] = pts[0] + 0.5 * (pts[i] - pts[0])
+                    f[i] = yield pts[i]
+
+
+def plain_loss(bins, value, count, target_q):
+    """Normalised-L2 distance of a plain-method output to a normalised target
+    on the bin grid."""
+    h = target_histogram(bins, value, count)
+    s = h.sum()
+    r = (h / s if s > 0 else h) - target_q
+    return float((r * r).sum())

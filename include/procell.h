@@ -240,8 +240,11 @@ int procell_calib_step(procell_calib *calib, uint32_t n_steps);
 /* Waits; current parameters: per set [3][n_types] (mean, sd, proportion;
  * -1 for a quiescent type's mean and sd), sets concatenated. */
 int procell_calib_theta(procell_calib *calib, double *theta, uint64_t capacity);
-/* Waits; loss of each set at the last step's forward pass, and the steps taken. */
-int procell_calib_loss(procell_calib *calib, double *loss, uint32_t capacity, uint64_t *steps);
+/* Waits; loss of each set at the last step's forward pass, the steps taken
+ * and the expansions of the last step (forward and adjoint pass); steps and
+ * expansions may be NULL. */
+int procell_calib_loss(procell_calib *calib, double *loss, uint32_t capacity, uint64_t *steps,
+                       uint64_t *expansions);
 void *procell_calib_stream(procell_calib *calib);                   /* hipStream_t */
 int procell_calib_destroy(procell_calib *calib);
 
