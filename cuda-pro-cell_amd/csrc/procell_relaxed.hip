@@ -54,6 +54,7 @@ namespace {
 constexpr double kRelaxEps = 1e-15;        // smallest W d that is still expanded
 constexpr uint32_t kFrameFields = 6;       // s, W, d (both passes), b, dlogW/dmean, dlogW/dsd (adjoint)
 constexpr uint32_t kWindowDoubles = 1024;  // LDS window of a wavefront (8 KB)
+constexpr uint32_t kRowCap = 64;           // rows of a wavefront's range staged in LDS
 
 // One parameter set of a relaxed plan (a single-set plan has one).  Rows,
 // initial cells, tally entries and type blocks of the sets are concatenated.
@@ -102,7 +103,9 @@ __device__ __forceinline__ double sigmoid(double x) { return 1.0 / (1.0 + exp(-x
 template <bool Grad>
 __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
 {
-    extern __shared__ double lds[];  // [kWindowDoubles] window | [3][n_types] gradient sums
+    // LDS: [kWindowDoubles] window | [3][T] gradient sums | [3][T] theta of the set |
+    //      u32 [kRowCap + 1] row starts | [kRowCap] row info | [T] proliferating types
+    extern __shared__ double lds[];
     const uint32_t lane = threadIdx.x;
     const uint32_t w = blockIdx.x;
     const RelaxSet S = a.sets[a.wave_set[w]];
@@ -117,20 +120,38 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
     const bool in_lds = win_n <= kWindowDoubles;
     const uint64_t hw0 = S.h_base + (uint64_t)(r_lo - S.row_base) * KT;  // tally index of the window
     double *gsum = lds + kWindowDoubles;
+    double *theta = gsum + 3 * T;
+    uint32_t *rs = (uint32_t *)(theta + 3 * T);
+    uint32_t *ri = rs + kRowCap + 1;
+    uint32_t *prolif = ri + kRowCap;
+    const uint32_t nr = r_hi - r_lo + 1;
+    const bool rows_lds = nr <= kRowCap;
     if (in_lds)
         for (uint32_t i = lane; i < win_n; i += 64)
             lds[i] = Grad ? a.g[hw0 + i] : 0.0;
-    if (Grad)
-        for (uint32_t i = lane; i < 3 * T; i += 64)
-            gsum[i] = 0.0;
+    for (uint32_t i = lane; i < 3 * T; i += 64) {
+        gsum[i] = 0.0;
+        theta[i] = a.theta[S.theta_base + i];
+    }
+    for (uint32_t i = lane; i < S.n_prolif; i += 64)
+        prolif[i] = a.prolif[S.prolif_base + i];
+    if (rows_lds) {
+        for (uint32_t i = lane; i <= nr; i += 64)
+            rs[i] = a.row_start[r_lo + i];
+        for (uint32_t i = lane; i < nr; i += 64)
+            ri[i] = a.row_info[r_lo + i];
+    }
     __syncthreads();
+    // row r's start and info are row_start[r - roff], row_info[r - roff]
+    const uint32_t roff = rows_lds ? r_lo : 0u;
+    const uint32_t *row_start = rows_lds ? rs : a.row_start;
+    const uint32_t *row_info = rows_lds ? ri : a.row_info;
+    const bool small_units = u_end <= 0xFFFFFFFFull;
 
     const dev::Coeffs K = dev::coeffs();
     const dev::Tables tb = dev::global_tables();
     const double t_max = S.t_max, beta = a.beta;
     const uint32_t n_prolif = S.n_prolif;
-    const double *theta = a.theta + S.theta_base;
-    const uint32_t *prolif = a.prolif + S.prolif_base;
     double *stk = a.stack + (uint64_t)w * a.kdim_max * kFrameFields * 64 + lane;
 
     bool active = false;
@@ -150,13 +171,13 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             if (!active) {
                 const uint64_t u = next + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (u < u_end) {
-                    cell = u / n_prolif;
-                    j = prolif[u - cell * n_prolif];
+                    cell = small_units ? (uint64_t)((uint32_t)u / n_prolif) : u / n_prolif;
+                    j = prolif[(uint32_t)(u - cell * n_prolif)];
                     const uint64_t gcell = S.cell_base + cell;
                     row = row_cursor;
-                    while (a.row_start[row + 1] <= gcell)
+                    while (row_start[row + 1 - roff] <= gcell)
                         ++row;
-                    const uint32_t info = a.row_info[row];
+                    const uint32_t info = row_info[row - roff];
                     kcap = info & 255u;
                     ge_phi = ((info >> 8) & 1u) != 0;
                     mean = theta[j];
@@ -171,8 +192,9 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             if (next > u_end)
                 next = u_end;
             if (next < u_end) {
-                const uint64_t c = S.cell_base + next / n_prolif;
-                while (a.row_start[row_cursor + 1] <= c)
+                const uint64_t c = S.cell_base + (small_units ? (uint64_t)((uint32_t)next / n_prolif)
+                                                              : next / n_prolif);
+                while (row_start[row_cursor + 1 - roff] <= c)
                     ++row_cursor;
             }
         }
@@ -352,7 +374,10 @@ thread_local hipError_t g_relaxed_hip = hipSuccess;
         }                                    \
     } while (0)
 
-size_t relaxed_lds_bytes(uint32_t n_types) { return (kWindowDoubles + 3ull * n_types) * sizeof(double); }
+size_t relaxed_lds_bytes(uint32_t n_types)
+{
+    return (kWindowDoubles + 6ull * n_types) * sizeof(double) + (2ull * kRowCap + 1 + n_types) * sizeof(uint32_t);
+}
 
 // Expected expansions of one walk of a type with this mean in a row with
 // this generation cap: a full binary tree down to the generation the mean
