@@ -32,8 +32,9 @@
 // evaluates both daughters of a node as it expands it, so every loop step of
 // every busy lane is one expansion (one Philox block, one Box-Muller pair,
 // two sigmoids); expandable right daughters wait in a per-lane stack in
-// device memory, laid out [wavefront][level][field][lane] so that a
-// wavefront's pushes and pops at one level are coalesced.  Recorded weights
+// device memory, one 64-byte line per frame laid out [wavefront][lane][level]
+// (lanes of a wavefront sit at different levels, so a field-major layout
+// would write a partial line per field and lane).  Recorded weights
 // go to an LDS window over the rows of the wavefront's range (the cotangent
 // window in the backward pass) and reach the tally with FP64 atomics at the
 // end.
@@ -52,7 +53,9 @@ namespace procell {
 namespace {
 
 constexpr double kRelaxEps = 1e-15;        // smallest W d that is still expanded
-constexpr uint32_t kFrameFields = 6;       // s, W, d (both passes), b, dlogW/dmean, dlogW/dsd (adjoint)
+// A stacked node: s, W, d (both passes), b, dlogW/dmean, dlogW/dsd (adjoint),
+// padded to one 64-byte line so that a push or a pop of a lane moves one line.
+constexpr uint32_t kFrameDoubles = 8;
 constexpr uint32_t kWindowDoubles = 1024;  // LDS window of a wavefront (8 KB)
 constexpr uint32_t kRowCap = 64;           // rows of a wavefront's range staged in LDS
 
@@ -91,7 +94,7 @@ struct RelaxArgs {
     const uint32_t *prolif;    // file index of each proliferating type, per set
     const uint32_t *quiet;     // file index of each quiescent type, per set
     const double *theta;       // per set [3][n_types]: mean, sd, proportion, file order
-    double *stack;             // [n_waves][kdim_max][kFrameFields][64]
+    double *stack;             // [n_waves][64 lanes][kdim_max] frames of kFrameDoubles
     double *h;                 // per set [rows][kdim][n_types] relaxed tally (forward)
     const double *g;           // cotangent, layout of h (backward)
     double *grad;              // per set [3][n_types] gradient (backward)
@@ -152,7 +155,7 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
     const dev::Tables tb = dev::global_tables();
     const double t_max = S.t_max, beta = a.beta;
     const uint32_t n_prolif = S.n_prolif;
-    double *stk = a.stack + (uint64_t)w * a.kdim_max * kFrameFields * 64 + lane;
+    double *stk = a.stack + ((uint64_t)w * 64 + lane) * a.kdim_max * kFrameDoubles;
 
     bool active = false;
     uint64_t cell = 0, node = 0, pending = 0;
@@ -275,14 +278,13 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
         // ---- next node: left daughter (right one stacked), the only
         //      expandable daughter, or the deepest stacked node ----
         if (expA && expB) {
-            double *f = stk + (uint64_t)kA * kFrameFields * 64;
-            f[0 * 64] = sB;
-            f[1 * 64] = WA;
-            f[2 * 64] = dB;
+            double2 *f = (double2 *)(stk + (uint64_t)kA * kFrameDoubles);
+            f[0] = make_double2(sB, WA);
             if (Grad) {  // the forward pass needs no tangents
-                f[3 * 64] = bB;
-                f[4 * 64] = gmA;
-                f[5 * 64] = gsA;
+                f[1] = make_double2(dB, bB);
+                f[2] = make_double2(gmA, gsA);
+            } else {
+                f[1].x = dB;
             }
             pending |= 1ull << kA;
         }
@@ -301,14 +303,18 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             pending &= ~(1ull << L);
             node = (node >> (k - L)) | 1ull;
             k = L;
-            const double *f = stk + (uint64_t)L * kFrameFields * 64;
-            s = f[0 * 64];
-            W = f[1 * 64];
-            d = f[2 * 64];
+            const double2 *f = (const double2 *)(stk + (uint64_t)L * kFrameDoubles);
+            const double2 f0 = f[0];
+            s = f0.x;
+            W = f0.y;
             if (Grad) {
-                b = f[3 * 64];
-                gm = f[4 * 64];
-                gs = f[5 * 64];
+                const double2 f1 = f[1], f2 = f[2];
+                d = f1.x;
+                b = f1.y;
+                gm = f2.x;
+                gs = f2.y;
+            } else {
+                d = f[1].x;
             }
         } else {
             if (Grad) {
@@ -655,7 +661,7 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     a.quiet = (const uint32_t *)(base + o_qu);
     a.expansions = (unsigned long long *)(base + o_ex);
     if (c.n_waves) {
-        const size_t stack_bytes = (size_t)c.n_waves * kdim_max * kFrameFields * 64 * sizeof(double);
+        const size_t stack_bytes = (size_t)c.n_waves * 64 * kdim_max * kFrameDoubles * sizeof(double);
         if (hipMalloc((void **)&c.d_stack, stack_bytes) != hipSuccess)
             return PROCELL_E_OUT_OF_MEMORY;
         a.stack = c.d_stack;
