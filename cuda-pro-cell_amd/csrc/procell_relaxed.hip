@@ -101,7 +101,23 @@ struct RelaxArgs {
     unsigned long long *expansions;  // [1] expansions of the call (roots drawn + nodes expanded)
 };
 
-__device__ __forceinline__ double sigmoid(double x) { return 1.0 / (1.0 + exp(-x)); }
+// 1 / x for x >= 1: the hardware reciprocal estimate refined by two Newton
+// steps (each squares the relative error), within an ulp or two of the
+// quotient.
+__device__ __forceinline__ double recip_ge1(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+// sigmoid(y / beta) with ib = 1 / beta.  The exponent is capped at 700 so
+// that 1 + e stays finite (an infinite one would turn the Newton steps into
+// inf x 0); there the sigmoid is below 1e-304 either way.
+__device__ __forceinline__ double sigmoid_scaled(double y, double ib)
+{
+    return recip_ge1(1.0 + exp(fmin(-(y * ib), 700.0)));
+}
 
 template <bool Grad>
 __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
@@ -153,7 +169,7 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
 
     const dev::Coeffs K = dev::coeffs();
     const dev::Tables tb = dev::global_tables();
-    const double t_max = S.t_max, beta = a.beta;
+    const double t_max = S.t_max, ib = 1.0 / a.beta;
     const uint32_t n_prolif = S.n_prolif;
     double *stk = a.stack + ((uint64_t)w * 64 + lane) * a.kdim_max * kFrameDoubles;
 
@@ -229,7 +245,7 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             gsA = 0.0;
             kA = 0;
         } else {
-            const double f = (1.0 - d) / beta;
+            const double f = (1.0 - d) * ib;
             const double a_cur = u1 + (double)k;
             WA = W * d;
             gmA = gm - f * a_cur;
@@ -241,8 +257,8 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             kA = k + 1;
         }
         const double aA = u1 + (double)kA;
-        const double dA = sigmoid((t_max - sA) / beta);
-        const double dB = pre ? 0.0 : sigmoid((t_max - sB) / beta);
+        const double dA = sigmoid_scaled(t_max - sA, ib);
+        const double dB = pre ? 0.0 : sigmoid_scaled(t_max - sB, ib);
         const double recA = WA * (1.0 - dA);
         const double recB = WA * (1.0 - dB);
         const bool recordA = kA > 0 || ge_phi;
@@ -260,13 +276,13 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             const double gv = in_lds ? lds[(row - r_lo) * KT + col]
                                      : a.g[S.h_base + (uint64_t)(row - S.row_base) * KT + col];
             if (recordA) {
-                const double q = dA * (1.0 - dA) / beta;
+                const double q = dA * (1.0 - dA) * ib;
                 acc_m += pj * gv * WA * ((1.0 - dA) * gmA + q * aA);
                 acc_s += pj * gv * WA * ((1.0 - dA) * gsA + q * bA);
                 acc_p += gv * recA;
             }
             if (!pre) {
-                const double q = dB * (1.0 - dB) / beta;
+                const double q = dB * (1.0 - dB) * ib;
                 acc_m += pj * gv * WA * ((1.0 - dB) * gmA + q * aA);
                 acc_s += pj * gv * WA * ((1.0 - dB) * gsA + q * bB);
                 acc_p += gv * recB;
