@@ -96,3 +96,43 @@ def test_invalid_targets_and_parameters(procell_lib):
     assert e.value.code == procell_lib.E_INVALID
     with pytest.raises(procell_lib.ProcellError):
         procell_lib.Calibration(probs, 0.0, targets)
+
+
+def test_edge_sets_equal_torch_composition(procell_lib):
+    """One proliferating type alone, an all-quiescent set (no walks at all;
+    its normalised histogram does not depend on the proportions) and four
+    types with a quiescent one, in one calibration."""
+    from procell import workloads
+
+    vals = [10.0 ** (4.0 * i / 63.0) for i in range(64)]
+    rng = np.random.default_rng(2)
+    counts = rng.poisson(30.0, 64)
+    W = workloads.Workload
+    gen = [
+        W("one", np.array(vals), counts.astype(np.uint64), np.array([1.0]), np.array([45.0]), np.array([12.0]),
+          150.0, 0.0),
+        W("quiet", np.array(vals), counts.astype(np.uint64), np.array([0.3, 0.7]), np.array([-1.0, -1.0]),
+          np.array([-1.0, -1.0]), 150.0, 0.0),
+        W("four", np.array(vals), counts.astype(np.uint64), np.array([0.2, 0.3, 0.4, 0.1]),
+          np.array([40.0, 70.0, 55.0, -1.0]), np.array([10.0, 20.0, 15.0, -1.0]), 180.0, 0.0),
+    ]
+    start = []
+    for w in gen:
+        m = np.where(w.mean >= 0, w.mean * 1.1, -1.0)
+        s = np.where(w.mean >= 0, w.sd * 0.9, -1.0)
+        p = 0.5 * w.proportion + 0.5 / len(w.proportion)
+        start.append(W(w.name + "0", w.values, w.counts, p / p.sum(), m, s, w.t_max, w.phi))
+    targets = [procell_lib.simulate(procell_lib.Problem.from_workload(w, 300 + i), want_raw=False)
+               for i, w in enumerate(gen)]
+    probs = [procell_lib.Problem.from_workload(w, 9 + i) for i, w in enumerate(start)]
+    refs = [_torch_reference(procell_lib, p, w, t, 2) for p, w, t in zip(probs, start, targets)]
+    with procell_lib.Calibration(probs, BETA, targets, lr=LR, beta1=B1, beta2=B2, eps=EPS) as cal:
+        for step in range(2):
+            cal.step(1)
+            loss, _ = cal.loss()
+            theta = cal.theta()
+            for i, ref in enumerate(refs):
+                l_ref, th_ref = ref[step]
+                assert abs(loss[i] - l_ref) <= 1e-9 * abs(l_ref) + 1e-15, (step, i, loss[i], l_ref)
+                assert np.allclose(theta[i], th_ref, rtol=1e-9, atol=1e-12), (step, i, theta[i], th_ref)
+        assert np.allclose(cal.theta()[1][2], start[1].proportion, rtol=1e-12)

@@ -191,3 +191,51 @@ def test_stream_order_with_torch_work(procell_lib, oracle):
                 side.synchronize()
                 got = h.cpu().numpy()
             assert np.all(np.abs(got - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
+
+
+def test_global_fallbacks_match_oracle(procell_lib, oracle):
+    """One cell per row and few wavefronts (blocks_per_cu = 1): every
+    wavefront's range spans more rows than the LDS row stage holds (64) and a
+    tally window larger than the LDS window (1024 entries), so the kernel
+    searches rows in device memory and records with global FP64 atomics."""
+    import torch
+
+    n = 40000
+    values = [10.0 ** (5.0 * i / (n - 1)) for i in range(n)]
+    d = dict(values=values, counts=[1] * n, proportion=[0.7, 0.3], mean=[30.0, -1.0], sd=[8.0, -1.0],
+             t_max=100.0, phi=1.0, seed=21)
+    beta = 0.5
+    h_ref, _ = oracle.relaxed(oracle.Problem(*_args(d)), beta)
+    g = np.random.default_rng(5).normal(size=h_ref.shape)
+    _, grad_ref = oracle.relaxed(oracle.Problem(*_args(d)), beta, g)
+    prob = procell_lib.Problem(*_args(d), blocks_per_cu=1)
+    th = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
+    with procell_lib.Relaxed(prob, beta) as rp:
+        h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        rp.forward(th, h)
+        gt = torch.tensor(g, dtype=torch.float64, device="cuda")
+        grad = torch.empty_like(th)
+        rp.backward(th, gt, grad)
+        h, grad = h.cpu().numpy(), grad.cpu().numpy()
+    assert np.all(np.abs(h - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
+    scale = max(1.0, np.abs(grad_ref).max())
+    assert np.all(np.abs(grad - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale), (grad, grad_ref)
+
+
+def test_many_types_match_oracle(procell_lib, oracle):
+    """70 types (69 proliferating, one quiescent): gradient sums and the
+    staged type list cover more than one wavefront's width."""
+    rng = np.random.default_rng(8)
+    T = 70
+    means = list(rng.uniform(25.0, 60.0, T - 1)) + [-1.0]
+    sds = [0.25 * m for m in means[:-1]] + [-1.0]
+    p = rng.dirichlet(np.ones(T))
+    d = dict(values=[300.0, 90.0, 12.0], counts=[40, 30, 20], proportion=list(p), mean=means, sd=sds,
+             t_max=110.0, phi=2.0, seed=4)
+    h_ref, _ = oracle.relaxed(oracle.Problem(*_args(d)), 1.0)
+    g = np.random.default_rng(6).normal(size=h_ref.shape)
+    _, grad_ref = oracle.relaxed(oracle.Problem(*_args(d)), 1.0, g)
+    h, grad = _device_pass(procell_lib, d, 1.0, g)
+    assert np.all(np.abs(h - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
+    scale = max(1.0, np.abs(grad_ref).max())
+    assert np.all(np.abs(grad - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale)
