@@ -29,6 +29,13 @@ METRIC = "cell divisions/s on 1 GPU (plain) + calib grad-steps/s batched vs plai
 # Nominal VALU work per division (lane operations): Philox4x32-10 block,
 # two 53-bit uniforms, FP64 Box-Muller pair, timer update and compares.
 LANE_OPS_PER_DIVISION = 180
+# Nominal VALU work per expansion of the relaxed estimator (lane operations):
+# the Philox block, two uniforms and the Box-Muller pair as for a division
+# (98 + 8 + 60), both daughters' times (4), two sigmoids (2 x (scale, exp
+# ~22, add, reciprocal + 2 Newton steps ~5) ~ 56) and the two records (8);
+# the adjoint adds the tangents and gradient sums of both daughters (34).
+LANE_OPS_PER_EXPANSION_FWD = 234
+LANE_OPS_PER_EXPANSION_ADJ = 268
 # MI355X VALU issue peak: 256 CUs x 64 lanes/clk x 2.4 GHz (lane-ops/s).
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "simulation_kernels_hbm_traffic.txt")
@@ -230,6 +237,16 @@ def relaxed_measurement(procell, workloads, np, torch, with_cpu=True, reps=50, f
         "backward_ms": times["backward"],
         "expansions_per_pass": exp_pass,
         "expansions_per_s": 2 * exp_pass / (pass_ms * 1e-3),
+        "roofline": {
+            "bound": "valu",
+            "forward_frac": exp_pass * LANE_OPS_PER_EXPANSION_FWD / (times["forward"] * 1e-3) / VALU_PEAK_LANE_OPS,
+            "adjoint_frac": exp_pass * LANE_OPS_PER_EXPANSION_ADJ / (times["backward"] * 1e-3) / VALU_PEAK_LANE_OPS,
+            "work_per_unit": f"{LANE_OPS_PER_EXPANSION_FWD} (forward) / {LANE_OPS_PER_EXPANSION_ADJ} (adjoint) "
+                             "lane-ops per expansion",
+            "peak": VALU_PEAK_LANE_OPS / 1e12,
+            "unit": "Tlane-op/s",
+            "timed": "torch events around 50 calls each (fill + walk + quiescent kernels)",
+        },
         "grad_steps_per_s_kernels": 1e3 / pass_ms,
         "grad_steps_per_s_fit": fit_steps / fit_s,
         "fit": {"steps": fit_steps, "optimizer": "Adam lr 0.02 on log mean, log sd, proportion logits",
@@ -280,7 +297,8 @@ def recovery(np, gen, thetas):
             "proportion_abs_median": float(np.median(ep)), "proportion_within_0.02": float(np.mean(ep <= 0.02))}
 
 
-def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_rounds=300, with_nm=True):
+def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_rounds=300, with_nm=True,
+                            with_cpu=True):
     """Batched calibration: 256 candidate parameter sets x 2e5 cells, 2-4
     types, t_max U(96,240), targets from the plain method at the generating
     parameters; 200 steps of relaxed forward + loss + adjoint + on-device
@@ -325,13 +343,36 @@ def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_roun
         "set_steps_per_s": n * steps / gpu_s,
         "expansions_per_step": exp_step,
         "expansions_per_s": exp_step * steps / gpu_s,
+        "roofline_frac_nominal": exp_step / 2 * (LANE_OPS_PER_EXPANSION_FWD + LANE_OPS_PER_EXPANSION_ADJ)
+                                 / (gpu_s / steps) / VALU_PEAK_LANE_OPS,
         "optimizer": "Adam lr 0.02 (log mean, log sd, proportion logits), on the device",
         "targets": "plain method at the generating parameters, seeds 1000+i; relaxed seeds 50000+i",
         "trajectory": traj,
     }
     if with_nm:
         out["derivative_free"] = nelder_mead_baseline(procell, calibrate, np, gen, start, targets, nm_rounds, traj)
+    if with_cpu:
+        out["cpu_baseline"] = calibration_cpu_baseline(calibrate, np, start, targets)
     return out
+
+
+def calibration_cpu_baseline(calibrate, np, start, targets, n_sample=4):
+    """The OpenMP oracle's relaxed forward + adjoint (the work of one step)
+    on the first n_sample sets, scaled to all sets."""
+    from oracle import binding as ob
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+    for i, w in enumerate(start[:n_sample]):
+        op = ob.Problem(w.values, w.counts, w.proportion, w.mean, w.sd, w.t_max, w.phi, 50000 + i)
+        h, _ = ob.relaxed(op, 0.5, None, threads)
+        ob.relaxed(op, 0.5, np.ones_like(h), threads)
+    dt = time.perf_counter() - t0
+    per_step = dt * len(start) / n_sample
+    log(f"calibration cpu baseline: {dt:.2f} s for {n_sample} sets -> {per_step:.1f} s per step of {len(start)} sets")
+    return {"value": 1.0 / per_step, "unit": "grad-steps/s (all sets, forward + adjoint)", "cores": threads,
+            "kind": "port", "sample": f"first {n_sample} of {len(start)} sets, one forward and one adjoint pass "
+                                      f"({dt:.2f} s wall), scaled by {len(start) / n_sample:g}", "cpu": cpu_model()}
 
 
 def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, grad_traj):
@@ -543,7 +584,8 @@ def main():
                                              fit_steps=args.fit_steps)
     if not args.no_calib:
         out["calibration"] = calibration_measurement(procell, workloads, np, steps=args.calib_steps,
-                                                     nm_rounds=args.nm_rounds, with_nm=args.nm_rounds > 0)
+                                                     nm_rounds=args.nm_rounds, with_nm=args.nm_rounds > 0,
+                                                     with_cpu=not args.no_cpu)
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_budget)
         out["cpu_baseline"] = cb
