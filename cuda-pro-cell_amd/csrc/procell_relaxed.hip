@@ -1253,8 +1253,14 @@ int procell_calib_step(procell_calib *c, uint32_t n_steps)
             g_relaxed_hip = e != hipSuccess ? e : e2;
             return PROCELL_E_HIP;
         }
+        const hipError_t e3 = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+        if (e3 != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            c->graph_exec = nullptr;
+            g_relaxed_hip = e3;
+            return PROCELL_E_HIP;
+        }
         c->graph = g;
-        RELAX_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
     }
     for (uint32_t i = 0; i < n_steps; ++i)
         RELAX_TRY(hipGraphLaunch(c->graph_exec, c->stream));
