@@ -99,6 +99,12 @@ struct RelaxArgs {
     const double *g;           // cotangent, layout of h (backward)
     double *grad;              // per set [3][n_types] gradient (backward)
     unsigned long long *expansions;  // [1] expansions of the call (roots drawn + nodes expanded)
+    // Cell stride: a pass walks only the initial cells c (set-local id) with
+    // (c + phase) & stride_mask == 0; phase = *step & stride_mask when step
+    // is set (calibration: the step counter), else phase & stride_mask.
+    const unsigned long long *step;
+    uint32_t stride_mask;
+    uint32_t phase;
 };
 
 // 1 / x for x >= 1: the hardware reciprocal estimate refined by two Newton
@@ -182,15 +188,22 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
     double acc_m = 0.0, acc_s = 0.0, acc_p = 0.0;
     uint32_t row_cursor = r_lo;  // row of the first walk not yet taken (wave-uniform)
     uint32_t n_exp = 0;
+    const uint32_t smask = a.stride_mask;
+    const uint32_t sphase = (a.step ? (uint32_t)a.step[0] : a.phase) & smask;
 
     for (;;) {
-        // ---- refill idle lanes from the wavefront's range ----
-        const uint64_t idle = __ballot(!active);
-        if (idle && next < u_end) {
+        // ---- refill idle lanes from the wavefront's range (with a cell
+        //      stride, claimed walks of skipped cells leave their lane idle
+        //      and it claims again, at most stride rounds per step) ----
+        for (uint32_t round = 0; round <= smask; ++round) {
+            const uint64_t idle = __ballot(!active);
+            if (!idle || next >= u_end)
+                break;
             if (!active) {
                 const uint64_t u = next + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (u < u_end) {
-                    cell = small_units ? (uint64_t)((uint32_t)u / n_prolif) : u / n_prolif;
+                const uint64_t cu = small_units ? (uint64_t)((uint32_t)u / n_prolif) : u / n_prolif;
+                if (u < u_end && (((uint32_t)cu + sphase) & smask) == 0u) {
+                    cell = cu;
                     j = prolif[(uint32_t)(u - cell * n_prolif)];
                     const uint64_t gcell = S.cell_base + cell;
                     row = row_cursor;
@@ -217,8 +230,11 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
                     ++row_cursor;
             }
         }
-        if (__ballot(active) == 0)
-            break;
+        if (__ballot(active) == 0) {
+            if (next >= u_end)
+                break;
+            continue;  // every claimed cell was skipped by the stride: claim again
+        }
         if (!active)
             continue;
 
@@ -374,7 +390,12 @@ __global__ void relaxed_quiescent_kernel(RelaxArgs a)
     const RelaxSet &S = a.sets[a.row_set[row]];
     if (S.n_quiet == 0 || ((a.row_info[row] >> 8) & 1u) == 0)
         return;
-    const double cnt = (double)(a.row_start[row + 1] - a.row_start[row]);
+    // cells of the row walked by this pass: local ids c in [lo, hi) with
+    // (c + phase) a multiple of the stride
+    const uint32_t stride = a.stride_mask + 1u;
+    const uint32_t phase = (a.step ? (uint32_t)a.step[0] : a.phase) & a.stride_mask;
+    const uint64_t lo = a.row_start[row] - S.cell_base + phase, hi = a.row_start[row + 1] - S.cell_base + phase;
+    const double cnt = (double)((hi + stride - 1) / stride - (lo + stride - 1) / stride);
     const uint64_t base = S.h_base + (uint64_t)(row - S.row_base) * S.kdim * S.n_types;
     for (uint32_t q = 0; q < S.n_quiet; ++q) {
         const uint32_t j = a.quiet[S.quiet_base + q];
@@ -1016,6 +1037,15 @@ int procell_relaxed_backward(procell_relaxed *r, const double *theta, const doub
     return relaxed_call(r, true, theta, nullptr, g, grad, stream);
 }
 
+int procell_relaxed_set_stride(procell_relaxed *r, uint32_t stride, uint32_t phase)
+{
+    if (!r || stride == 0 || (stride & (stride - 1)) != 0 || stride > (1u << 16))
+        return PROCELL_E_INVALID;
+    r->core.args.stride_mask = stride - 1;
+    r->core.args.phase = phase & (stride - 1);
+    return PROCELL_OK;
+}
+
 int procell_relaxed_expansions(procell_relaxed *r, uint64_t *n)
 {
     if (!r || !n)
@@ -1035,9 +1065,11 @@ int procell_relaxed_destroy(procell_relaxed *r)
 }
 
 int procell_calib_create(const procell_params *sets, uint32_t n_sets, double beta, const procell_result *targets,
-                         const procell_adam *adam, procell_calib **out)
+                         const procell_adam *adam, uint32_t cell_stride, procell_calib **out)
 {
     if (!sets || !targets || !adam || !out || n_sets == 0)
+        return PROCELL_E_INVALID;
+    if (cell_stride == 0 || (cell_stride & (cell_stride - 1)) != 0 || cell_stride > (1u << 16))
         return PROCELL_E_INVALID;
     *out = nullptr;
     if (!(adam->lr > 0.0) || !(adam->beta1 >= 0.0 && adam->beta1 < 1.0) || !(adam->beta2 >= 0.0 && adam->beta2 < 1.0) ||
@@ -1215,6 +1247,8 @@ int procell_calib_create(const procell_params *sets, uint32_t n_sets, double bet
     ca.theta = (double *)(sb + s_th);
     ca.step = (unsigned long long *)(sb + s_step);
     c->fwd = core.args;
+    c->fwd.step = ca.step;
+    c->fwd.stride_mask = cell_stride - 1;
     c->fwd.theta = ca.theta;
     c->fwd.h = d_h;
     c->fwd.expansions = d_exp;

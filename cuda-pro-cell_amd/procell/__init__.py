@@ -102,7 +102,7 @@ EXPORTS = (
     "procell_read_types", "procell_write_histogram", "procell_free", "procell_debug_draws",
     "procell_batch_create", "procell_batch_run", "procell_batch_result", "procell_batch_autotune",
     "procell_relaxed_create", "procell_relaxed_shape", "procell_relaxed_forward", "procell_relaxed_backward",
-    "procell_relaxed_expansions", "procell_relaxed_destroy",
+    "procell_relaxed_set_stride", "procell_relaxed_expansions", "procell_relaxed_destroy",
     "procell_calib_create", "procell_calib_step", "procell_calib_theta", "procell_calib_loss",
     "procell_calib_stream", "procell_calib_destroy",
 )
@@ -158,10 +158,11 @@ def lib():
         L.procell_relaxed_backward.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p]
         L.procell_relaxed_expansions.argtypes = [ctypes.c_void_p, _u64p]
+        L.procell_relaxed_set_stride.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
         L.procell_relaxed_destroy.argtypes = [ctypes.c_void_p]
     if hasattr(L, "procell_calib_create"):
         L.procell_calib_create.argtypes = [P, ctypes.c_uint32, ctypes.c_double, R, ctypes.POINTER(AdamStruct),
-                                           ctypes.POINTER(ctypes.c_void_p)]
+                                           ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
         L.procell_calib_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
         L.procell_calib_theta.argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint64]
         L.procell_calib_loss.argtypes = [ctypes.c_void_p, _dp, ctypes.c_uint32, _u64p, _u64p]
@@ -426,6 +427,11 @@ class Relaxed:
                                               self._stream(stream)),
                "relaxed_backward")
 
+    def set_stride(self, stride: int, phase: int = 0):
+        """Later passes walk only the initial cells c with (c + phase) a
+        multiple of `stride` (a power of two); 1 walks every cell."""
+        _check(lib().procell_relaxed_set_stride(self.handle, int(stride), int(phase)), "relaxed_set_stride")
+
     def expansions(self) -> int:
         """Expansions of the last call (roots drawn plus nodes expanded); waits."""
         n = ctypes.c_uint64()
@@ -457,9 +463,11 @@ class Calibration:
 
     problems: their histograms, t_max, phi, relaxed-stream seeds and the
     starting parameters; targets: per set (value, count) arrays of a target
-    histogram in the plain method's output form (e.g. a Result)."""
+    histogram in the plain method's output form (e.g. a Result); cell_stride:
+    a power of two k walks the initial cells of phase (step mod k) only."""
 
-    def __init__(self, problems, beta, targets, lr=0.02, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+    def __init__(self, problems, beta, targets, lr=0.02, beta1=0.9, beta2=0.999, eps=1e-8, stream=None,
+                 cell_stride=1):
         self.problems = list(problems)
         n = len(self.problems)
         if len(targets) != n:
@@ -480,8 +488,8 @@ class Calibration:
             tr[i].count = c.ctypes.data_as(_i64p)
         adam = AdamStruct(lr, beta1, beta2, eps)
         self.handle = ctypes.c_void_p()
-        _check(lib().procell_calib_create(arr, n, float(beta), tr, ctypes.byref(adam), ctypes.byref(self.handle)),
-               "calib_create")
+        _check(lib().procell_calib_create(arr, n, float(beta), tr, ctypes.byref(adam), int(cell_stride),
+                                          ctypes.byref(self.handle)), "calib_create")
         self.n_types = [max(1, p.n_types) for p in self.problems]
 
     def step(self, n: int = 1):

@@ -203,6 +203,11 @@ int procell_relaxed_shape(const procell_relaxed *plan, uint64_t *n_rows, uint32_
 int procell_relaxed_forward(procell_relaxed *plan, const double *theta, double *h, void *stream);
 int procell_relaxed_backward(procell_relaxed *plan, const double *theta, const double *g, double *grad,
                              void *stream);
+/* Cell stride of the plan's later passes: only the initial cells c (ids
+ * within the set) with (c + phase) a multiple of `stride` (a power of two,
+ * at most 65536) are walked, so the passes of phases 0 .. stride-1 sum to
+ * the full pass; 1 (the default) walks every cell. */
+int procell_relaxed_set_stride(procell_relaxed *plan, uint32_t stride, uint32_t phase);
 /* Expansions of the plan's last call (roots drawn plus nodes whose daughters
  * were drawn); waits for the device. */
 int procell_relaxed_expansions(procell_relaxed *plan, uint64_t *n);
@@ -225,7 +230,11 @@ int procell_relaxed_destroy(procell_relaxed *plan);
  *   targets : [n_sets] target histograms in the plain method's output form
  *             (value[n_out], count[n_out]); every value must be one of the
  *             set's output values (e.g. a procell_simulate result).
- *   adam    : lr, beta1, beta2, eps (the update of Kingma & Ba, 2015). */
+ *   adam    : lr, beta1, beta2, eps (the update of Kingma & Ba, 2015).
+ *   cell_stride : 1 walks every initial cell each step; a power of two k
+ *             walks the cells of phase (step mod k) only (see
+ *             procell_relaxed_set_stride): k times cheaper steps whose
+ *             estimates cover every cell once per k steps. */
 typedef struct {
     double lr;
     double beta1;
@@ -234,7 +243,7 @@ typedef struct {
 } procell_adam;
 typedef struct procell_calib procell_calib;
 int procell_calib_create(const procell_params *sets, uint32_t n_sets, double beta, const procell_result *targets,
-                         const procell_adam *adam, procell_calib **calib);
+                         const procell_adam *adam, uint32_t cell_stride, procell_calib **calib);
 /* Enqueues n_steps steps (graph replays) on the calibration's stream. */
 int procell_calib_step(procell_calib *calib, uint32_t n_steps);
 /* Waits; current parameters: per set [3][n_types] (mean, sd, proportion;

@@ -27,7 +27,7 @@ def _sets(procell_lib, n=3, cells=3000.0):
     return gen, start, targets, probs
 
 
-def _torch_reference(procell_lib, prob, w, target, steps):
+def _torch_reference(procell_lib, prob, w, target, steps, stride=1):
     import torch
     from procell import calibrate
 
@@ -44,7 +44,8 @@ def _torch_reference(procell_lib, prob, w, target, steps):
     z = torch.tensor(np.log(w.proportion), dtype=torch.float64, device="cuda", requires_grad=True)
     opt = torch.optim.Adam([lm, ls, z], lr=LR, betas=(B1, B2), eps=EPS)
     out = []
-    for _ in range(steps):
+    for step in range(steps):
+        plan.set_stride(stride, step % stride)
         theta = calibrate.assemble_theta(lm, ls, z, quiet)
         loss = calibrate.normalized_l2(bm(calibrate.relaxed_tally(plan, theta)), tgt)
         opt.zero_grad()
@@ -136,3 +137,21 @@ def test_edge_sets_equal_torch_composition(procell_lib):
                 assert abs(loss[i] - l_ref) <= 1e-9 * abs(l_ref) + 1e-15, (step, i, loss[i], l_ref)
                 assert np.allclose(theta[i], th_ref, rtol=1e-9, atol=1e-12), (step, i, theta[i], th_ref)
         assert np.allclose(cal.theta()[1][2], start[1].proportion, rtol=1e-12)
+
+
+def test_strided_steps_equal_torch_composition(procell_lib):
+    """Cell stride 2: step s walks the cells of phase s mod 2, as a single-set
+    plan with set_stride(2, s mod 2) composed with torch does."""
+    gen, start, targets, probs = _sets(procell_lib, 3, 4000.0)
+    refs = [_torch_reference(procell_lib, p, w, t, 3, stride=2) for p, w, t in zip(probs, start, targets)]
+    with procell_lib.Calibration(probs, BETA, targets, lr=LR, beta1=B1, beta2=B2, eps=EPS, cell_stride=2) as cal:
+        for step in range(3):
+            cal.step(1)
+            loss, _ = cal.loss()
+            theta = cal.theta()
+            for i, ref in enumerate(refs):
+                l_ref, th_ref = ref[step]
+                assert abs(loss[i] - l_ref) <= 1e-9 * abs(l_ref), (step, i, loss[i], l_ref)
+                assert np.allclose(theta[i], th_ref, rtol=1e-9, atol=1e-12), (step, i, theta[i], th_ref)
+    with pytest.raises(procell_lib.ProcellError):
+        procell_lib.Calibration(probs, BETA, targets, cell_stride=3)

@@ -239,3 +239,36 @@ def test_many_types_match_oracle(procell_lib, oracle):
     assert np.all(np.abs(h - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
     scale = max(1.0, np.abs(grad_ref).max())
     assert np.all(np.abs(grad - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale)
+
+
+def test_strided_passes_sum_to_full(procell_lib):
+    """With cell stride 4 the passes of phases 0..3 walk disjoint sets of
+    initial cells covering all of them: their tallies and gradients sum to
+    the full pass's (to rounding: different atomic orders)."""
+    import torch
+
+    d, beta = _problems()[0]
+    prob = procell_lib.Problem(*_args(d))
+    th = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
+    with procell_lib.Relaxed(prob, beta) as rp:
+        g = torch.tensor(np.random.default_rng(4).normal(size=rp.shape), dtype=torch.float64, device="cuda")
+        h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        gr = torch.empty_like(th)
+        rp.forward(th, h)
+        rp.backward(th, g, gr)
+        full_h, full_g = h.cpu().numpy(), gr.cpu().numpy()
+        n_full = rp.expansions()
+        sum_h, sum_g, n_sum = np.zeros_like(full_h), np.zeros_like(full_g), 0
+        for phase in range(4):
+            rp.set_stride(4, phase)
+            rp.forward(th, h)
+            n_sum += rp.expansions()
+            rp.backward(th, g, gr)
+            part = h.cpu().numpy()
+            assert 0.15 * full_h.sum() < part.sum() < 0.35 * full_h.sum()
+            sum_h += part
+            sum_g += gr.cpu().numpy()
+        rp.set_stride(1)
+    assert n_sum == n_full
+    assert np.all(np.abs(sum_h - full_h) <= 1e-12 * np.abs(full_h) + 1e-12 * np.abs(full_h).max())
+    assert np.all(np.abs(sum_g - full_g) <= 1e-10 * np.abs(full_g) + 1e-10 * np.abs(full_g).max())
