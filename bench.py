@@ -349,8 +349,27 @@ def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_roun
         "targets": "plain method at the generating parameters, seeds 1000+i; relaxed seeds 50000+i",
         "trajectory": traj,
     }
+    # the same calibration with cell stride 4 (each step walks a quarter of the cells; the
+    # adjoint another quarter than the forward pass)
+    cal = procell.Calibration(probs, 0.5, targets, lr=0.02, cell_stride=4)
+    straj, sgpu = [{"steps": 0, "gpu_s": 0.0, **rec0}], 0.0
+    for k in range(0, steps, chunk):
+        t0 = time.perf_counter()
+        cal.step(chunk)
+        loss, done = cal.loss()
+        sgpu += time.perf_counter() - t0
+        straj.append({"steps": done, "gpu_s": sgpu, "loss_median": float(np.median(loss)),
+                      **recovery(np, gen, cal.theta())})
+    cal.close()
+    log(f"calibration, cell stride 4: {steps} steps, {sgpu:.2f} s ({sgpu / steps * 1e3:.2f} ms/step); "
+        f"mean rel err median {straj[-1]['mean_rel_median']:.4f}")
+    out["strided"] = {"cell_stride": 4, "ms_per_step": sgpu / steps * 1e3, "grad_steps_per_s": steps / sgpu,
+                      "trajectory": straj,
+                      "note": "forward pass on the cells of phase step mod 4, adjoint on phase step + 1 mod 4 "
+                              "(double sampling: an unbiased gradient of the squared loss)"}
     if with_nm:
-        out["derivative_free"] = nelder_mead_baseline(procell, calibrate, np, gen, start, targets, nm_rounds, traj)
+        out["derivative_free"] = nelder_mead_baseline(procell, calibrate, np, gen, start, targets, nm_rounds, traj,
+                                                      straj)
     if with_cpu:
         out["cpu_baseline"] = calibration_cpu_baseline(calibrate, np, start, targets)
     return out
@@ -375,7 +394,7 @@ def calibration_cpu_baseline(calibrate, np, start, targets, n_sample=4):
                                       f"({dt:.2f} s wall), scaled by {len(start) / n_sample:g}", "cpu": cpu_model()}
 
 
-def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, grad_traj):
+def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, grad_traj, strided_traj=None):
     """Nelder-Mead per set over the plain method: one batched plain run of
     all sets per round (each set's next simplex point, its own fixed seed),
     GPU time = the batch runs' kernel times."""
@@ -446,9 +465,21 @@ def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, gr
         "gpu_s_to_common_gradient": c_grad,
         "gpu_s_to_common_nelder_mead": c_nm,
         "gpu_time_ratio_at_common": (c_nm / c_grad) if (c_nm and c_grad) else None,
+        **strided_ratio(first, strided_traj, traj),
         "note": "gpu_time_ratio = Nelder-Mead GPU time / gradient GPU time to the same median mean error; "
                 "> 1 favours the gradient method",
     }
+
+
+def strided_ratio(first, straj, nm_traj):
+    """GPU times of the strided calibration and of Nelder-Mead to the accuracy both reach."""
+    if not straj:
+        return {}
+    common = max(straj[-1]["mean_rel_median"], nm_traj[-1]["mean_rel_median"])
+    a, b = first(straj, common), first(nm_traj, common)
+    return {"strided_common_mean_rel_median": common, "gpu_s_to_common_strided": a,
+            "gpu_s_to_common_nelder_mead_vs_strided": b,
+            "gpu_time_ratio_strided_at_common": (b / a) if (a and b) else None}
 
 
 def main():

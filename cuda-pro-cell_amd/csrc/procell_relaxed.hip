@@ -100,8 +100,9 @@ struct RelaxArgs {
     double *grad;              // per set [3][n_types] gradient (backward)
     unsigned long long *expansions;  // [1] expansions of the call (roots drawn + nodes expanded)
     // Cell stride: a pass walks only the initial cells c (set-local id) with
-    // (c + phase) & stride_mask == 0; phase = *step & stride_mask when step
-    // is set (calibration: the step counter), else phase & stride_mask.
+    // (c + phase) & stride_mask == 0; phase = (*step + phase) & stride_mask
+    // when step is set (calibration: the step counter, the adjoint pass one
+    // class ahead of the forward pass), else phase & stride_mask.
     const unsigned long long *step;
     uint32_t stride_mask;
     uint32_t phase;
@@ -125,8 +126,8 @@ __device__ __forceinline__ double sigmoid_scaled(double y, double ib)
     return recip_ge1(1.0 + exp(fmin(-(y * ib), 700.0)));
 }
 
-template <bool Grad>
-__global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
+template <bool Grad, bool Strided>
+__device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
 {
     // LDS: [kWindowDoubles] window | [3][T] gradient sums | [3][T] theta of the set |
     //      u32 [kRowCap + 1] row starts | [kRowCap] row info | [T] proliferating types
@@ -188,8 +189,8 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
     double acc_m = 0.0, acc_s = 0.0, acc_p = 0.0;
     uint32_t row_cursor = r_lo;  // row of the first walk not yet taken (wave-uniform)
     uint32_t n_exp = 0;
-    const uint32_t smask = a.stride_mask;
-    const uint32_t sphase = (a.step ? (uint32_t)a.step[0] : a.phase) & smask;
+    const uint32_t smask = Strided ? a.stride_mask : 0u;
+    const uint32_t sphase = Strided ? ((a.step ? (uint32_t)a.step[0] : 0u) + a.phase) & smask : 0u;
 
     for (;;) {
         // ---- refill idle lanes from the wavefront's range (with a cell
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             if (!active) {
                 const uint64_t u = next + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
                 const uint64_t cu = small_units ? (uint64_t)((uint32_t)u / n_prolif) : u / n_prolif;
-                if (u < u_end && (((uint32_t)cu + sphase) & smask) == 0u) {
+                if (u < u_end && (!Strided || (((uint32_t)cu + sphase) & smask) == 0u)) {
                     cell = cu;
                     j = prolif[(uint32_t)(u - cell * n_prolif)];
                     const uint64_t gcell = S.cell_base + cell;
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
             }
         }
         if (__ballot(active) == 0) {
-            if (next >= u_end)
+            if (!Strided || next >= u_end)
                 break;
             continue;  // every claimed cell was skipped by the stride: claim again
         }
@@ -379,6 +380,20 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
     }
 }
 
+template <bool Grad>
+__global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
+{
+    relaxed_walks<Grad, false>(a);
+}
+
+// With a cell stride; held to four wavefronts per SIMD (the extra refill
+// rounds would otherwise cost the adjoint one).
+template <bool Grad>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void relaxed_strided_kernel(RelaxArgs a)
+{
+    relaxed_walks<Grad, true>(a);
+}
+
 // Quiescent types: each initial cell of a row with value >= phi records p_j
 // at generation 0 (cell.cu:104-108 gives them timer -1, always out of time).
 template <bool Grad>
@@ -393,7 +408,7 @@ __global__ void relaxed_quiescent_kernel(RelaxArgs a)
     // cells of the row walked by this pass: local ids c in [lo, hi) with
     // (c + phase) a multiple of the stride
     const uint32_t stride = a.stride_mask + 1u;
-    const uint32_t phase = (a.step ? (uint32_t)a.step[0] : a.phase) & a.stride_mask;
+    const uint32_t phase = ((a.step ? (uint32_t)a.step[0] : 0u) + a.phase) & a.stride_mask;
     const uint64_t lo = a.row_start[row] - S.cell_base + phase, hi = a.row_start[row + 1] - S.cell_base + phase;
     const double cnt = (double)((hi + stride - 1) / stride - (lo + stride - 1) / stride);
     const uint64_t base = S.h_base + (uint64_t)(row - S.row_base) * S.kdim * S.n_types;
@@ -600,9 +615,12 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     // ---- wavefronts ----
     c.lds_bytes = relaxed_lds_bytes(t_max_types);
     int occ = 0, occ_f = 0;
+    int occ_s = 0, occ_sf = 0;
     RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, relaxed_kernel<true>, 64, c.lds_bytes));
     RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, relaxed_kernel<false>, 64, c.lds_bytes));
-    occ = std::min(occ, occ_f);
+    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, relaxed_strided_kernel<true>, 64, c.lds_bytes));
+    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_sf, relaxed_strided_kernel<false>, 64, c.lds_bytes));
+    occ = std::min(std::min(occ, occ_f), std::min(occ_s, occ_sf));
     if (occ < 1)
         return PROCELL_E_HIP;
     if (ps[0].tuning.blocks_per_cu)
@@ -711,10 +729,17 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
 hipError_t core_launch(const RelaxCore &c, const RelaxArgs &a, bool grad, hipStream_t st)
 {
     if (c.any_prolif && c.n_waves) {
-        if (grad)
-            hipLaunchKernelGGL(relaxed_kernel<true>, dim3(c.n_waves), dim3(64), c.lds_bytes, st, a);
-        else
-            hipLaunchKernelGGL(relaxed_kernel<false>, dim3(c.n_waves), dim3(64), c.lds_bytes, st, a);
+        const dim3 g(c.n_waves), b(64);
+        if (a.stride_mask) {
+            if (grad)
+                hipLaunchKernelGGL(relaxed_strided_kernel<true>, g, b, c.lds_bytes, st, a);
+            else
+                hipLaunchKernelGGL(relaxed_strided_kernel<false>, g, b, c.lds_bytes, st, a);
+        } else if (grad) {
+            hipLaunchKernelGGL(relaxed_kernel<true>, g, b, c.lds_bytes, st, a);
+        } else {
+            hipLaunchKernelGGL(relaxed_kernel<false>, g, b, c.lds_bytes, st, a);
+        }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess)
             return e;
@@ -1253,6 +1278,10 @@ int procell_calib_create(const procell_params *sets, uint32_t n_sets, double bet
     c->fwd.h = d_h;
     c->fwd.expansions = d_exp;
     c->bwd = c->fwd;
+    // with a stride, the adjoint walks the next class of cells: the residual
+    // (from the forward pass) and the Jacobian then come from independent
+    // cells, so their product is an unbiased gradient of the squared loss
+    c->bwd.phase = cell_stride > 1 ? 1u : 0u;
     c->bwd.h = nullptr;
     c->bwd.g = ca.g;
     c->bwd.grad = d_grad;

@@ -231,10 +231,12 @@ int procell_relaxed_destroy(procell_relaxed *plan);
  *             (value[n_out], count[n_out]); every value must be one of the
  *             set's output values (e.g. a procell_simulate result).
  *   adam    : lr, beta1, beta2, eps (the update of Kingma & Ba, 2015).
- *   cell_stride : 1 walks every initial cell each step; a power of two k
- *             walks the cells of phase (step mod k) only (see
- *             procell_relaxed_set_stride): k times cheaper steps whose
- *             estimates cover every cell once per k steps. */
+ *   cell_stride : 1 walks every initial cell each step; a power of two
+ *             k > 1 makes the forward pass walk the cells of phase
+ *             (step mod k) and the adjoint those of phase (step + 1 mod k)
+ *             only (see procell_relaxed_set_stride): k times cheaper steps,
+ *             and a residual and Jacobian from independent cells (double
+ *             sampling), so the gradient of the squared loss is unbiased. */
 typedef struct {
     double lr;
     double beta1;

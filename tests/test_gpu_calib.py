@@ -49,6 +49,8 @@ def _torch_reference(procell_lib, prob, w, target, steps, stride=1):
         theta = calibrate.assemble_theta(lm, ls, z, quiet)
         loss = calibrate.normalized_l2(bm(calibrate.relaxed_tally(plan, theta)), tgt)
         opt.zero_grad()
+        # with a stride the adjoint walks the next class of cells (double sampling)
+        plan.set_stride(stride, (step + (1 if stride > 1 else 0)) % stride)
         loss.backward()
         opt.step()
         with torch.no_grad():
@@ -140,8 +142,9 @@ def test_edge_sets_equal_torch_composition(procell_lib):
 
 
 def test_strided_steps_equal_torch_composition(procell_lib):
-    """Cell stride 2: step s walks the cells of phase s mod 2, as a single-set
-    plan with set_stride(2, s mod 2) composed with torch does."""
+    """Cell stride 2: step s's forward pass walks the cells of phase s mod 2
+    and its adjoint those of phase s + 1 mod 2, as a single-set plan composed
+    with torch does with set_stride before each pass."""
     gen, start, targets, probs = _sets(procell_lib, 3, 4000.0)
     refs = [_torch_reference(procell_lib, p, w, t, 3, stride=2) for p, w, t in zip(probs, start, targets)]
     with procell_lib.Calibration(probs, BETA, targets, lr=LR, beta1=B1, beta2=B2, eps=EPS, cell_stride=2) as cal:
