@@ -38,6 +38,13 @@
 // go to an LDS window over the rows of the wavefront's range (the cotangent
 // window in the backward pass) and reach the tally with FP64 atomics at the
 // end.
+//
+// A plan covers one parameter set (procell_relaxed_*) or many (the batched
+// calibration, procell_calib_*): sets' rows, cells and tally blocks are
+// concatenated and every wavefront's range lies in one set.  A cell stride k
+// (a power of two) restricts a pass to the initial cells of one residue
+// class mod k.  A calibration step -- fill, forward pass, loss and cotangent,
+// adjoint pass, Adam -- is captured once into a hipGraph and replayed.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
