@@ -60,6 +60,10 @@ namespace procell {
 namespace {
 
 constexpr double kRelaxEps = 1e-15;        // smallest W d that is still expanded
+// Calibration keeps sd <= 0.8 mean (log 0.8): the expected number of
+// generation-k nodes, 2^k P(s_k <= t), stays bounded only while
+// mean^2 / (2 sd^2) > ln 2, i.e. sd / mean < 0.85.
+constexpr double kLogMaxSdRatio = -0.22314355131420976;
 // A stacked node: s, W, d (both passes), b, dlogW/dmean, dlogW/dsd (adjoint),
 // padded to one 64-byte line so that a push or a pop of a lane moves one line.
 constexpr uint32_t kFrameDoubles = 8;
@@ -912,6 +916,9 @@ __global__ __launch_bounds__(256) void calib_adam_kernel(CalibArgs c)
                 adam_update(c.x[tb + j], c.m1[tb + j], c.m2[tb + j], gr[j] * th[j], c, bc1, bc2);
                 adam_update(c.x[tb + T + j], c.m1[tb + T + j], c.m2[tb + T + j], gr[T + j] * th[T + j], c, bc1,
                             bc2);
+                // keep sd <= kMaxSdRatio mean: with untruncated timers the
+                // expected tree stays finite only while sd / mean < 0.85
+                c.x[tb + T + j] = fmin(c.x[tb + T + j], c.x[tb + j] + kLogMaxSdRatio);
             }
             const uint32_t z = tb + 2 * T + j;
             adam_update(c.x[z], c.m1[z], c.m2[z], th[2 * T + j] * (gr[2 * T + j] - dot), c, bc1, bc2);

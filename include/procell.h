@@ -185,9 +185,10 @@ int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t r
  * fluorescence and the generation cap are the plain method's.  Subtrees
  * whose weight falls below 1e-15 are not expanded.
  *   theta : device [3][n_types] = mean, sd, proportion (file order); the
- *           entries of quiescent types' mean and sd are ignored.  Means of
- *           proliferating types must stay well above 0: the tree grows
- *           until the sigmoid or the generation cap stops it.
+ *           entries of quiescent types' mean and sd are ignored.  Keep
+ *           mean > 0 and sd < 0.85 mean for proliferating types: beyond
+ *           that, untruncated timers let the expected tree grow until the
+ *           generation cap (2^cap nodes per walk).
  *   h     : device [n_rows][kdim][n_types] relaxed expected tally (rows =
  *           nonzero rows in file order, as raw_tally).
  *   g     : device, shape of h: the cotangent dL/dh.
@@ -230,7 +231,8 @@ int procell_relaxed_destroy(procell_relaxed *plan);
  *   targets : [n_sets] target histograms in the plain method's output form
  *             (value[n_out], count[n_out]); every value must be one of the
  *             set's output values (e.g. a procell_simulate result).
- *   adam    : lr, beta1, beta2, eps (the update of Kingma & Ba, 2015).
+ *   adam    : lr, beta1, beta2, eps (the update of Kingma & Ba, 2015);
+ *             each update keeps sd <= 0.8 mean (see theta above).
  *   cell_stride : 1 walks every initial cell each step; a power of two
  *             k > 1 makes the forward pass walk the cells of phase
  *             (step mod k) and the adjoint those of phase (step + 1 mod k)

@@ -158,3 +158,24 @@ def test_strided_steps_equal_torch_composition(procell_lib):
                 assert np.allclose(theta[i], th_ref, rtol=1e-9, atol=1e-12), (step, i, theta[i], th_ref)
     with pytest.raises(procell_lib.ProcellError):
         procell_lib.Calibration(probs, BETA, targets, cell_stride=3)
+
+
+def test_sd_kept_below_bound(procell_lib):
+    """A target generated with sd = 0.95 mean pulls the sd up; every update
+    keeps sd <= 0.8 mean, where the relaxed trees stay finite."""
+    from procell import workloads
+
+    vals = [10.0 ** (4.0 * i / 63.0) for i in range(64)]
+    counts = np.random.default_rng(3).poisson(40.0, 64).astype(np.uint64)
+    W = workloads.Workload
+    gen = W("wide", np.array(vals), counts, np.array([1.0]), np.array([40.0]), np.array([38.0]), 150.0, 0.0)
+    start = W("wide0", np.array(vals), counts, np.array([1.0]), np.array([40.0]), np.array([31.0]), 150.0, 0.0)
+    target = procell_lib.simulate(procell_lib.Problem.from_workload(gen, 5), want_raw=False)
+    prob = procell_lib.Problem.from_workload(start, 6)
+    with procell_lib.Calibration([prob], BETA, [target], lr=0.05) as cal:
+        ratios = []
+        for _ in range(8):
+            cal.step(5)
+            th = cal.theta()[0]
+            ratios.append(th[1, 0] / th[0, 0])
+        assert max(ratios) <= 0.8 * (1 + 1e-12), ratios
