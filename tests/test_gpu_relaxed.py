@@ -272,3 +272,23 @@ def test_strided_passes_sum_to_full(procell_lib):
     assert n_sum == n_full
     assert np.all(np.abs(sum_h - full_h) <= 1e-12 * np.abs(full_h) + 1e-12 * np.abs(full_h).max())
     assert np.all(np.abs(sum_g - full_g) <= 1e-10 * np.abs(full_g) + 1e-10 * np.abs(full_g).max())
+
+
+def test_full_size_calibration_instance_matches_oracle(procell_lib, oracle):
+    """The single-instance calibration workload at full size (256 bins,
+    ~1e6 cells, two proliferating types and a quiescent one, beta 0.5):
+    forward and adjoint against the OpenMP oracle."""
+    from procell import workloads
+
+    w = workloads.calibration_instance(0)
+    d = dict(values=w.values, counts=w.counts, proportion=w.proportion, mean=w.mean, sd=w.sd,
+             t_max=w.t_max, phi=w.phi, seed=1)
+    beta = w.meta["beta"]
+    print("full-size relaxed parity:", w.n_cells, "cells", flush=True)
+    h_ref, _ = oracle.relaxed(oracle.Problem(*_args(d)), beta)
+    g = np.random.default_rng(12).normal(size=h_ref.shape)
+    _, grad_ref = oracle.relaxed(oracle.Problem(*_args(d)), beta, g)
+    h, grad = _device_pass(procell_lib, d, beta, g)
+    assert np.all(np.abs(h - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
+    scale = max(1.0, np.abs(grad_ref).max())
+    assert np.all(np.abs(grad - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale)
