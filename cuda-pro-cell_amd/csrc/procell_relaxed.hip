@@ -200,6 +200,16 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     double acc_m = 0.0, acc_s = 0.0, acc_p = 0.0;
     uint32_t row_cursor = r_lo;  // row of the first walk not yet taken (wave-uniform)
     uint32_t n_exp = 0;
+    // The most recently pushed frame stays in registers until another push
+    // spills it: it is always the deepest pending level, i.e. the next one
+    // popped, so most push/pop pairs never touch memory (a store would also
+    // hold up the wavefront's next wait on its loads).
+    // (forward pass only: the adjoint's six-field frame would cost it a
+    // wavefront per SIMD in registers)
+    constexpr bool kCache = !Grad;
+    bool c_ok = false;
+    uint32_t c_lvl = 0;
+    double c_s = 0.0, c_W = 0.0, c_d = 0.0;
     const uint32_t smask = Strided ? a.stride_mask : 0u;
     const uint32_t sphase = Strided ? ((a.step ? (uint32_t)a.step[0] : 0u) + a.phase) & smask : 0u;
 
@@ -321,15 +331,23 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
 
         // ---- next node: left daughter (right one stacked), the only
         //      expandable daughter, or the deepest stacked node ----
-        if (expA && expB) {
+        if (kCache && expA && expB) {
+            if (c_ok) {  // spill the cached frame to its level (no tangents in the forward pass)
+                double2 *f = (double2 *)(stk + (uint64_t)c_lvl * kFrameDoubles);
+                f[0] = make_double2(c_s, c_W);
+                f[1].x = c_d;
+            }
+            c_ok = true;
+            c_lvl = kA;
+            c_s = sB;
+            c_W = WA;
+            c_d = dB;
+            pending |= 1ull << kA;
+        } else if (!kCache && expA && expB) {
             double2 *f = (double2 *)(stk + (uint64_t)kA * kFrameDoubles);
             f[0] = make_double2(sB, WA);
-            if (Grad) {  // the forward pass needs no tangents
-                f[1] = make_double2(dB, bB);
-                f[2] = make_double2(gmA, gsA);
-            } else {
-                f[1].x = dB;
-            }
+            f[1] = make_double2(dB, bB);
+            f[2] = make_double2(gmA, gsA);
             pending |= 1ull << kA;
         }
         if (expA || expB) {
@@ -347,18 +365,25 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
             pending &= ~(1ull << L);
             node = (node >> (k - L)) | 1ull;
             k = L;
-            const double2 *f = (const double2 *)(stk + (uint64_t)L * kFrameDoubles);
-            const double2 f0 = f[0];
-            s = f0.x;
-            W = f0.y;
-            if (Grad) {
-                const double2 f1 = f[1], f2 = f[2];
-                d = f1.x;
-                b = f1.y;
-                gm = f2.x;
-                gs = f2.y;
+            if (kCache && c_ok) {  // the cached frame is level L
+                s = c_s;
+                W = c_W;
+                d = c_d;
+                c_ok = false;
             } else {
-                d = f[1].x;
+                const double2 *f = (const double2 *)(stk + (uint64_t)L * kFrameDoubles);
+                const double2 f0 = f[0];
+                s = f0.x;
+                W = f0.y;
+                if (Grad) {
+                    const double2 f1 = f[1], f2 = f[2];
+                    d = f1.x;
+                    b = f1.y;
+                    gm = f2.x;
+                    gs = f2.y;
+                } else {
+                    d = f[1].x;
+                }
             }
         } else {
             if (Grad) {
