@@ -103,3 +103,26 @@ def test_autotune_keeps_results(procell_lib):
         b.run(seeds)
         for i, p in enumerate(probs):
             _same(b.result(i), _single(procell_lib, p, seeds[i]), f"set {i}")
+
+
+def test_full_size_batch_sample_equals_single_runs(procell_lib, oracle):
+    """The 512-set batch at full size (5.2e7 cells): 16 sets spread over the
+    batch equal their single runs bit for bit, and the two smallest of them
+    equal the oracle."""
+    from procell import workloads
+
+    ws = workloads.parameter_sets(0, 512)
+    probs = [procell_lib.Problem.from_workload(w, 0) for w in ws]
+    seeds = [9000 + i for i in range(len(ws))]
+    pick = list(range(0, 512, 32))
+    print("full-size batch:", sum(w.n_cells for w in ws), "cells", flush=True)
+    with procell_lib.Batch(probs) as b:
+        b.run(seeds)
+        got = {i: b.result(i) for i in pick}
+    for i in pick:
+        _same(got[i], _single(procell_lib, probs[i], seeds[i]), f"set {i}")
+    for i in sorted(pick, key=lambda i: ws[i].n_cells)[:2]:
+        tally, _, st = oracle.simulate(oracle.Problem.from_workload(ws[i], seeds[i]))
+        assert np.array_equal(got[i].raw_tally, tally), i
+        for k in STATS:
+            assert got[i].stats[k] == st[k], (i, k)

@@ -179,3 +179,26 @@ def test_sd_kept_below_bound(procell_lib):
             th = cal.theta()[0]
             ratios.append(th[1, 0] / th[0, 0])
         assert max(ratios) <= 0.8 * (1 + 1e-12), ratios
+
+
+def test_full_size_calibration_sample_equals_torch_composition(procell_lib):
+    """The 256-set calibration at full size (2e5 cells per set): after one
+    step, three sets spread over the batch equal the single-set composition
+    with torch."""
+    from procell import workloads
+
+    gen, start = workloads.calibration_batch(0, 256, 2e5)
+    with procell_lib.Batch([procell_lib.Problem.from_workload(w, 0) for w in gen]) as b:
+        b.run([1000 + i for i in range(len(gen))])
+        targets = [b.result(i, want_raw=False) for i in range(len(gen))]
+    probs = [procell_lib.Problem.from_workload(w, 50000 + i) for i, w in enumerate(start)]
+    pick = [0, 137, 255]
+    refs = {i: _torch_reference(procell_lib, probs[i], start[i], targets[i], 1) for i in pick}
+    with procell_lib.Calibration(probs, BETA, targets, lr=LR, beta1=B1, beta2=B2, eps=EPS) as cal:
+        cal.step(1)
+        loss, _ = cal.loss()
+        theta = cal.theta()
+    for i in pick:
+        l_ref, th_ref = refs[i][0]
+        assert abs(loss[i] - l_ref) <= 1e-9 * abs(l_ref), (i, loss[i], l_ref)
+        assert np.allclose(theta[i], th_ref, rtol=1e-9, atol=1e-12), (i, theta[i], th_ref)
