@@ -202,3 +202,24 @@ def test_full_size_calibration_sample_equals_torch_composition(procell_lib):
         l_ref, th_ref = refs[i][0]
         assert abs(loss[i] - l_ref) <= 1e-9 * abs(l_ref), (i, loss[i], l_ref)
         assert np.allclose(theta[i], th_ref, rtol=1e-9, atol=1e-12), (i, theta[i], th_ref)
+
+
+def test_empty_target_and_invalid_start(procell_lib):
+    """A set whose target histogram is empty (all-zero normalised target: the
+    loss is the squared norm of the normalised relaxed histogram) steps like
+    the others and stays finite; a proliferating start type with sd <= 0
+    (its log parameter is undefined) is rejected."""
+    gen, start, targets, probs = _sets(procell_lib, 2, 1000.0)
+    empty = (np.zeros(0), np.zeros(0, dtype=np.int64))
+    with procell_lib.Calibration(probs, BETA, [targets[0], empty], lr=LR) as cal:
+        cal.step(3)
+        loss, n = cal.loss()
+        assert n == 3 and np.all(np.isfinite(loss)) and loss[1] > 0
+        for th in cal.theta():
+            assert np.all(np.isfinite(th))
+    w = start[0]
+    bad_sd = np.where(w.mean >= 0, 0.0, -1.0)
+    bad = procell_lib.Problem(w.values, w.counts, w.proportion, w.mean, bad_sd, w.t_max, w.phi, 1)
+    with pytest.raises(procell_lib.ProcellError) as e:
+        procell_lib.Calibration([bad], BETA, [targets[0]])
+    assert e.value.code == procell_lib.E_INVALID
