@@ -19,6 +19,8 @@ degrees of freedom, and every |z| < 5.
 import math
 
 import numpy as np
+import pytest
+from scipy import signal
 from scipy import stats as sst
 
 
@@ -37,7 +39,7 @@ def _cdfs(mean, sd, t_max, kmax, dt=0.004, n_u=800):
     w = f_tau * dt
     w[0] *= 0.5  # trapezoid ends (f_tau(0) = 0 anyway for mean/sd >> 0)
     for _ in range(2, kmax + 2):
-        F.append(np.convolve(F[-1], w)[: len(t)])
+        F.append(signal.fftconvolve(F[-1], w)[: len(t)])
     return np.array([Fk[-1] for Fk in F])  # F_k(t_max), k = 0 .. kmax + 1
 
 
@@ -66,10 +68,21 @@ def _expected(w, phi, kdim):
     return E
 
 
-def test_expected_histogram_closed_form(oracle):
+def _workload(which):
     from procell import workloads
 
-    w = workloads.small_histogram(0)
+    if which == "hist64":
+        return workloads.small_histogram(0)
+    # the 1024-bin, five-type shape (phi 50: rows below it lost at generation 0,
+    # t_max 240) at 5 % of its cells
+    return workloads.scaled(workloads.large_histogram(0), 0.05)
+
+
+@pytest.mark.parametrize("which", ["hist64", "hist1024_5pct"])
+def test_expected_histogram_closed_form(oracle, which):
+    from procell import workloads
+
+    w = _workload(which)
     n_seeds = 20
     obs = []
     for s in range(n_seeds):
@@ -90,7 +103,7 @@ def test_expected_histogram_closed_form(oracle):
     p = 1.0 - sst.chi2.cdf(chi2, df=int(big.sum()))
     assert p > 1e-4, (chi2, int(big.sum()), p)
     # the closed form has power: a 2 % shift of one type's mean is detected
-    w2 = workloads.small_histogram(0)
+    w2 = _workload(which)
     w2.mean = w2.mean.copy()
     w2.mean[0] *= 1.02
     E2 = _expected(w2, phi, kdim)
