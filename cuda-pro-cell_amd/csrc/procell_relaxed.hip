@@ -60,6 +60,11 @@ namespace procell {
 namespace {
 
 constexpr double kRelaxEps = 1e-15;        // smallest W d that is still expanded
+// Expansions a single walk may take: a walk that needs more is cut short and
+// counted, and the call reports PROCELL_E_DEPTH (parameters far outside the
+// estimator's domain, e.g. sd >= 0.85 mean with deep rows, would otherwise
+// run for practically ever).
+constexpr uint32_t kWalkBudget = 1u << 22;
 // Calibration keeps sd <= 0.8 mean (log 0.8): the expected number of
 // generation-k nodes, 2^k P(s_k <= t), stays bounded only while
 // mean^2 / (2 sd^2) > ln 2, i.e. sd / mean < 0.85.
@@ -109,7 +114,7 @@ struct RelaxArgs {
     double *h;                 // per set [rows][kdim][n_types] relaxed tally (forward)
     const double *g;           // cotangent, layout of h (backward)
     double *grad;              // per set [3][n_types] gradient (backward)
-    unsigned long long *expansions;  // [1] expansions of the call (roots drawn + nodes expanded)
+    unsigned long long *expansions;  // [2] expansions of the call (roots drawn + nodes expanded), walks cut
     // Cell stride: a pass walks only the initial cells c (set-local id) with
     // (c + phase) & stride_mask == 0; phase = (*step + phase) & stride_mask
     // when step is set (calibration: the step counter, the adjoint pass one
@@ -199,7 +204,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     double s = 0.0, b = 0.0, W = 0.0, d = 0.0, gm = 0.0, gs = 0.0;
     double acc_m = 0.0, acc_s = 0.0, acc_p = 0.0;
     uint32_t row_cursor = r_lo;  // row of the first walk not yet taken (wave-uniform)
-    uint32_t n_exp = 0;
+    uint32_t n_exp = 0, n_cut = 0, walk_exp = 0;
     // The most recently pushed frame stays in registers until another push
     // spills it: it is always the deepest pending level, i.e. the next one
     // popped, so most push/pop pairs never touch memory (a store would also
@@ -239,6 +244,8 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
                     pj = theta[2 * T + j];
                     node = 0;  // the next step draws the root from block (cell, 0, 0)
                     pending = 0;
+                    c_ok = false;
+                    walk_exp = 0;
                     active = true;
                 }
             }
@@ -264,6 +271,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
         //      both daughters of the current node ----
         const bool pre = node == 0;
         ++n_exp;
+        ++walk_exp;
         dev::Words wd = dev::draw_block(S.seed, cell, node, 0u);
         if (pre) {
             u1 = 1.0 + dev::uniform_oc(wd.w1, 0u);
@@ -331,6 +339,8 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
 
         // ---- next node: left daughter (right one stacked), the only
         //      expandable daughter, or the deepest stacked node ----
+        // a walk past its budget is cut short (counted; the call reports it)
+        const bool over = walk_exp >= kWalkBudget;
         if (kCache && expA && expB) {
             if (c_ok) {  // spill the cached frame to its level (no tangents in the forward pass)
                 double2 *f = (double2 *)(stk + (uint64_t)c_lvl * kFrameDoubles);
@@ -350,7 +360,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
             f[2] = make_double2(gmA, gsA);
             pending |= 1ull << kA;
         }
-        if (expA || expB) {
+        if (!over && (expA || expB)) {
             const bool left = expA;
             node = pre ? 1ull : 2ull * node + (left ? 0ull : 1ull);
             k = kA;
@@ -360,7 +370,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
             W = WA;
             gm = gmA;
             gs = gsA;
-        } else if (pending) {
+        } else if (!over && pending) {
             const uint32_t L = 63u - (uint32_t)__clzll(pending);
             pending &= ~(1ull << L);
             node = (node >> (k - L)) | 1ull;
@@ -386,6 +396,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
                 }
             }
         } else {
+            n_cut += over && (expA || expB || pending) ? 1u : 0u;
             if (Grad) {
                 atomicAdd(gsum + j, acc_m);
                 atomicAdd(gsum + T + j, acc_s);
@@ -396,10 +407,15 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
         }
     }
     __syncthreads();
-    for (int o = 32; o > 0; o >>= 1)
+    for (int o = 32; o > 0; o >>= 1) {
         n_exp += __shfl_xor(n_exp, o);
-    if (lane == 0)
+        n_cut += __shfl_xor(n_cut, o);
+    }
+    if (lane == 0) {
         atomicAdd(a.expansions, (unsigned long long)n_exp);
+        if (n_cut)
+            atomicAdd(a.expansions + 1, (unsigned long long)n_cut);
+    }
     if (!Grad) {
         if (in_lds)
             for (uint32_t i = lane; i < win_n; i += 64) {
@@ -720,7 +736,7 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     const size_t o_wr = take(wave_row.size() * 4), o_rs = take(row_start.size() * 4);
     const size_t o_ri = take(row_info.size() * 4 + 4), o_rt = take(row_set.size() * 4 + 4);
     const size_t o_pr = take(prolif.size() * 4 + 4), o_qu = take(quiet.size() * 4 + 4);
-    const size_t o_ex = take(8);
+    const size_t o_ex = take(16);
     if (hipMalloc(&c.d_block, off) != hipSuccess)
         return PROCELL_E_OUT_OF_MEMORY;
     char *base = (char *)c.d_block;
@@ -1023,7 +1039,7 @@ int relaxed_call(procell_relaxed *r, bool grad, const double *theta, double *h, 
     a.h = h;
     a.g = g;
     a.grad = gr;
-    RELAX_TRY(launch_zero(a.expansions, 1, st));
+    RELAX_TRY(launch_zero(a.expansions, 2, st));
     if (!grad)
         RELAX_TRY(launch_zero((unsigned long long *)h, r->core.h_total, st));
     else
@@ -1117,9 +1133,11 @@ int procell_relaxed_expansions(procell_relaxed *r, uint64_t *n)
     RELAX_TRY(hipSetDevice(r->core.device));
     RELAX_TRY(hipDeviceSynchronize());
     unsigned long long v = 0;
-    RELAX_TRY(hipMemcpy(&v, r->core.args.expansions, sizeof v, hipMemcpyDeviceToHost));
+    unsigned long long v2[2] = {0, 0};
+    RELAX_TRY(hipMemcpy(v2, r->core.args.expansions, sizeof v2, hipMemcpyDeviceToHost));
+    v = v2[0];
     *n = v;
-    return PROCELL_OK;
+    return v2[1] ? PROCELL_E_DEPTH : PROCELL_OK;
 }
 
 int procell_relaxed_destroy(procell_relaxed *r)
@@ -1253,7 +1271,7 @@ int procell_calib_create(const procell_params *sets, uint32_t n_sets, double bet
     };
     // cleared every step: h | grad | expansions (one contiguous block)
     const size_t s_h = stake(0);
-    const size_t clear_bytes = core.h_total * 8 + P * 8 + 8;
+    const size_t clear_bytes = core.h_total * 8 + P * 8 + 16;
     so = (clear_bytes + 255) / 256 * 256;
     const size_t s_x = stake(P * 8), s_m1 = stake(P * 8), s_m2 = stake(P * 8), s_th = stake(P * 8);
     const size_t s_g = stake(core.h_total * 8 + 8), s_hb = stake(n_bins_total * 8 + 8);
@@ -1391,12 +1409,11 @@ int procell_calib_loss(procell_calib *c, double *loss, uint32_t capacity, uint64
         RELAX_TRY(hipMemcpy(&s, c->cargs.step, 8, hipMemcpyDeviceToHost));
         *steps = s;
     }
-    if (expansions) {
-        unsigned long long s = 0;
-        RELAX_TRY(hipMemcpy(&s, c->fwd.expansions, 8, hipMemcpyDeviceToHost));
-        *expansions = s;
-    }
-    return PROCELL_OK;
+    unsigned long long ex[2] = {0, 0};
+    RELAX_TRY(hipMemcpy(ex, c->fwd.expansions, sizeof ex, hipMemcpyDeviceToHost));
+    if (expansions)
+        *expansions = ex[0];
+    return ex[1] ? PROCELL_E_DEPTH : PROCELL_OK;
 }
 
 void *procell_calib_stream(procell_calib *c) { return c ? (void *)c->stream : nullptr; }

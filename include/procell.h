@@ -210,7 +210,10 @@ int procell_relaxed_backward(procell_relaxed *plan, const double *theta, const d
  * the full pass; 1 (the default) walks every cell. */
 int procell_relaxed_set_stride(procell_relaxed *plan, uint32_t stride, uint32_t phase);
 /* Expansions of the plan's last call (roots drawn plus nodes whose daughters
- * were drawn); waits for the device. */
+ * were drawn); waits for the device.  Returns PROCELL_E_DEPTH (with *n set)
+ * when a walk needed more than 2^22 expansions and was cut short: the call's
+ * h or gradient is then incomplete (parameters outside the estimator's
+ * domain, see theta above). */
 int procell_relaxed_expansions(procell_relaxed *plan, uint64_t *n);
 int procell_relaxed_destroy(procell_relaxed *plan);
 
@@ -255,7 +258,8 @@ int procell_calib_step(procell_calib *calib, uint32_t n_steps);
 int procell_calib_theta(procell_calib *calib, double *theta, uint64_t capacity);
 /* Waits; loss of each set at the last step's forward pass, the steps taken
  * and the expansions of the last step (forward and adjoint pass); steps and
- * expansions may be NULL. */
+ * expansions may be NULL.  Returns PROCELL_E_DEPTH when a walk of the last
+ * step exceeded 2^22 expansions and was cut short. */
 int procell_calib_loss(procell_calib *calib, double *loss, uint32_t capacity, uint64_t *steps,
                        uint64_t *expansions);
 void *procell_calib_stream(procell_calib *calib);                   /* hipStream_t */

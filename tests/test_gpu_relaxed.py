@@ -292,3 +292,24 @@ def test_full_size_calibration_instance_matches_oracle(procell_lib, oracle):
     assert np.all(np.abs(h - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
     scale = max(1.0, np.abs(grad_ref).max())
     assert np.all(np.abs(grad - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale)
+
+
+def test_runaway_walk_is_cut_and_reported(procell_lib):
+    """sd = 3 x mean with 29 generations allowed and a far t_max: every
+    walk would expand a full 2^29-node tree.  Walks stop at 2^22 expansions
+    and the call reports PROCELL_E_DEPTH instead of running for hours."""
+    import torch
+
+    d = dict(values=[2.0**30], counts=[4], proportion=[1.0], mean=[10.0], sd=[30.0], t_max=1000.0, phi=1.0, seed=1)
+    prob = procell_lib.Problem(*_args(d))
+    th = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
+    with procell_lib.Relaxed(prob, 1.0) as rp:
+        h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        rp.forward(th, h)
+        with pytest.raises(procell_lib.ProcellError) as e:
+            rp.expansions()
+        assert e.value.code == procell_lib.E_DEPTH
+        # a later call within the domain is clean again
+        th2 = torch.tensor([[100.0], [10.0], [1.0]], dtype=torch.float64, device="cuda")
+        rp.forward(th2, h)
+        assert rp.expansions() > 0
