@@ -298,7 +298,7 @@ def recovery(np, gen, thetas):
 
 
 def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_rounds=300, with_nm=True,
-                            with_cpu=True):
+                            with_cpu=True, lm_iters=15):
     """Batched calibration: 256 candidate parameter sets x 2e5 cells, 2-4
     types, t_max U(96,240), targets from the plain method at the generating
     parameters; 200 steps of relaxed forward + loss + adjoint + on-device
@@ -367,12 +367,48 @@ def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_roun
                       "trajectory": straj,
                       "note": "forward pass on the cells of phase step mod 4, adjoint on phase step + 1 mod 4 "
                               "(double sampling: an unbiased gradient of the squared loss)"}
+    if lm_iters:
+        out["levenberg_marquardt"] = lm_measurement(procell, calibrate, np, gen, probs, targets, lm_iters, rec0)
     if with_nm:
         out["derivative_free"] = nelder_mead_baseline(procell, calibrate, np, gen, start, targets, nm_rounds, traj,
-                                                      straj)
+                                                      straj, out.get("levenberg_marquardt", {}).get("trajectory"))
     if with_cpu:
         out["cpu_baseline"] = calibration_cpu_baseline(calibrate, np, start, targets)
     return out
+
+
+def lm_measurement(procell, calibrate, np, gen, probs, targets, iters, rec0):
+    """Levenberg-Marquardt per set over one batch relaxed plan of all sets:
+    each iteration = one Jacobian pass (h, dh/dmean, dh/dsd) of every set's
+    trial point + batched torch algebra on the device; the start point's
+    pass is counted in the first iteration.  gpu_s = wall time around the
+    iterations with a device synchronisation after each."""
+    import torch
+
+    plan = procell.RelaxedBatch(probs, 0.5)
+    lm = calibrate.BatchLM(plan, probs, targets)
+    torch.cuda.synchronize()
+    traj, gpu = [{"steps": 0, "gpu_s": 0.0, **rec0}], 0.0
+    for it in range(iters):
+        t0 = time.perf_counter()
+        lm.step(1)
+        torch.cuda.synchronize()
+        gpu += time.perf_counter() - t0
+        traj.append({"steps": it + 1, "gpu_s": gpu, "loss_median": float(lm.loss.median()),
+                     **recovery(np, gen, lm.thetas())})
+    try:
+        plan.expansions()
+        cut = False
+    except procell.ProcellError:
+        cut = True
+    plan.close()
+    log(f"levenberg-marquardt: {iters} iterations, {gpu:.2f} s ({gpu / iters * 1e3:.2f} ms/iteration); "
+        f"mean rel err median {traj[-1]['mean_rel_median']:.4f}")
+    return {"iterations": iters, "ms_per_iteration": gpu / iters * 1e3, "walks_cut": cut,
+            "method": "Levenberg-Marquardt per set on (log mean, log sd, proportion logits), damping x 0.2 on an "
+                      "accepted trial and x 4 on a rejected one, steps limited to 0.5 per parameter, "
+                      "sd <= 0.8 mean; Gauss-Newton system from one relaxed Jacobian pass per iteration",
+            "trajectory": traj}
 
 
 def calibration_cpu_baseline(calibrate, np, start, targets, n_sample=4):
@@ -394,7 +430,8 @@ def calibration_cpu_baseline(calibrate, np, start, targets, n_sample=4):
                                       f"({dt:.2f} s wall), scaled by {len(start) / n_sample:g}", "cpu": cpu_model()}
 
 
-def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, grad_traj, strided_traj=None):
+def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, grad_traj, strided_traj=None,
+                         lm_traj=None):
     """Nelder-Mead per set over the plain method: one batched plain run of
     all sets per round (each set's next simplex point, its own fixed seed),
     GPU time = the batch runs' kernel times."""
@@ -467,6 +504,7 @@ def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, gr
         "gpu_s_to_common_nelder_mead": c_nm,
         "gpu_time_ratio_at_common": (c_nm / c_grad) if (c_nm and c_grad) else None,
         **strided_ratio(first, strided_traj, traj),
+        **strided_ratio(first, lm_traj, traj, "levenberg_marquardt"),
         "wall_s_to_common_nelder_mead": next((p["wall_s"] for p in traj if p["mean_rel_median"] <= common), None),
         "wall_note": "the calibration steps run back to back on the device (their gpu_s is wall time around "
                      "the replays); each Nelder-Mead round re-creates a batch plan (host set-up and uploads), "
@@ -476,15 +514,15 @@ def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, gr
     }
 
 
-def strided_ratio(first, straj, nm_traj):
-    """GPU times of the strided calibration and of Nelder-Mead to the accuracy both reach."""
+def strided_ratio(first, straj, nm_traj, name="strided"):
+    """GPU times of a calibration trajectory and of Nelder-Mead to the accuracy both reach."""
     if not straj:
         return {}
     common = max(straj[-1]["mean_rel_median"], nm_traj[-1]["mean_rel_median"])
     a, b = first(straj, common), first(nm_traj, common)
-    return {"strided_common_mean_rel_median": common, "gpu_s_to_common_strided": a,
-            "gpu_s_to_common_nelder_mead_vs_strided": b,
-            "gpu_time_ratio_strided_at_common": (b / a) if (a and b) else None}
+    return {f"{name}_common_mean_rel_median": common, f"gpu_s_to_common_{name}": a,
+            f"gpu_s_to_common_nelder_mead_vs_{name}": b,
+            f"gpu_time_ratio_{name}_at_common": (b / a) if (a and b) else None}
 
 
 def main():
@@ -500,6 +538,7 @@ def main():
     ap.add_argument("--no-calib", action="store_true", help="skip the batched-calibration measurement")
     ap.add_argument("--calib-steps", type=int, default=200, help="steps of the batched calibration")
     ap.add_argument("--nm-rounds", type=int, default=300, help="rounds of the Nelder-Mead baseline (0: skip)")
+    ap.add_argument("--lm-iters", type=int, default=15, help="Levenberg-Marquardt iterations (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -621,7 +660,7 @@ def main():
     if not args.no_calib:
         out["calibration"] = calibration_measurement(procell, workloads, np, steps=args.calib_steps,
                                                      nm_rounds=args.nm_rounds, with_nm=args.nm_rounds > 0,
-                                                     with_cpu=not args.no_cpu)
+                                                     with_cpu=not args.no_cpu, lm_iters=args.lm_iters)
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_budget)
         out["cpu_baseline"] = cb

@@ -122,6 +122,17 @@ struct RelaxArgs {
     const unsigned long long *step;
     uint32_t stride_mask;
     uint32_t phase;
+    // Jacobian pass: [2][h_total] dh/dmean_j and dh/dsd_j of each entry's own
+    // type j (h of type j depends on type j's parameters only)
+    double *jac;
+    uint64_t h_total;
+};
+
+// What a pass of the walk computes.
+enum WalkMode : int {
+    kForward = 0,   // h
+    kAdjoint = 1,   // gradient of <g, h>
+    kJacobian = 2,  // h and its derivatives in each type's mean and sd
 };
 
 // 1 / x for x >= 1: the hardware reciprocal estimate refined by two Newton
@@ -142,9 +153,12 @@ __device__ __forceinline__ double sigmoid_scaled(double y, double ib)
     return recip_ge1(1.0 + exp(fmin(-(y * ib), 700.0)));
 }
 
-template <bool Grad, bool Strided>
+template <int Mode, bool Strided>
 __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
 {
+    constexpr bool Grad = Mode == kAdjoint;
+    constexpr bool Jac = Mode == kJacobian;
+    constexpr bool Tan = Grad || Jac;  // path tangents carried (six-field frames)
     // LDS: [kWindowDoubles] window | [3][T] gradient sums | [3][T] theta of the set |
     //      u32 [kRowCap + 1] row starts | [kRowCap] row info | [T] proliferating types
     extern __shared__ double lds[];
@@ -159,7 +173,9 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
         return;
     const uint32_t r_lo = a.wave_row[2 * w], r_hi = a.wave_row[2 * w + 1];
     const uint32_t win_n = (r_hi - r_lo + 1) * KT;
-    const bool in_lds = win_n <= kWindowDoubles;
+    // window: h (forward), g (adjoint) or h | dh/dmean | dh/dsd (Jacobian)
+    const uint32_t win_t = Jac ? 3 * win_n : win_n;
+    const bool in_lds = win_t <= kWindowDoubles;
     const uint64_t hw0 = S.h_base + (uint64_t)(r_lo - S.row_base) * KT;  // tally index of the window
     double *gsum = lds + kWindowDoubles;
     double *theta = gsum + 3 * T;
@@ -169,7 +185,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     const uint32_t nr = r_hi - r_lo + 1;
     const bool rows_lds = nr <= kRowCap;
     if (in_lds)
-        for (uint32_t i = lane; i < win_n; i += 64)
+        for (uint32_t i = lane; i < win_t; i += 64)
             lds[i] = Grad ? a.g[hw0 + i] : 0.0;
     for (uint32_t i = lane; i < 3 * T; i += 64) {
         gsum[i] = 0.0;
@@ -211,7 +227,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     // hold up the wavefront's next wait on its loads).
     // (forward pass only: the adjoint's six-field frame would cost it a
     // wavefront per SIMD in registers)
-    constexpr bool kCache = !Grad;
+    constexpr bool kCache = !Tan;
     bool c_ok = false;
     uint32_t c_lvl = 0;
     double c_s = 0.0, c_W = 0.0, c_d = 0.0;
@@ -309,7 +325,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
         const double recB = WA * (1.0 - dB);
         const bool recordA = kA > 0 || ge_phi;
         const uint32_t col = kA * T + j;
-        if (!Grad) {
+        if (Mode == kForward) {
             // both daughters land in the same (row, generation, type) entry
             const double v = (recordA ? pj * recA : 0.0) + (pre ? 0.0 : pj * recB);
             if (recordA || !pre) {
@@ -317,6 +333,35 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
                     atomicAdd(lds + (row - r_lo) * KT + col, v);
                 else
                     unsafeAtomicAdd(a.h + S.h_base + (uint64_t)(row - S.row_base) * KT + col, v);
+            }
+        } else if (Jac) {
+            // the recorded p_j W (1 - d) and its derivatives in mean_j, sd_j:
+            // p_j W ((1 - d) dlogW/dtheta + d (1 - d) / beta ds/dtheta)
+            double vh = 0.0, vm = 0.0, vs = 0.0;
+            if (recordA) {
+                const double q = dA * (1.0 - dA) * ib;
+                vh = recA;
+                vm = WA * ((1.0 - dA) * gmA + q * aA);
+                vs = WA * ((1.0 - dA) * gsA + q * bA);
+            }
+            if (!pre) {
+                const double q = dB * (1.0 - dB) * ib;
+                vh += recB;
+                vm += WA * ((1.0 - dB) * gmA + q * aA);
+                vs += WA * ((1.0 - dB) * gsA + q * bB);
+            }
+            if (recordA || !pre) {
+                if (in_lds) {
+                    double *e = lds + (row - r_lo) * KT + col;
+                    atomicAdd(e, pj * vh);
+                    atomicAdd(e + win_n, pj * vm);
+                    atomicAdd(e + 2 * win_n, pj * vs);
+                } else {
+                    const uint64_t e = S.h_base + (uint64_t)(row - S.row_base) * KT + col;
+                    unsafeAtomicAdd(a.h + e, pj * vh);
+                    unsafeAtomicAdd(a.jac + e, pj * vm);
+                    unsafeAtomicAdd(a.jac + a.h_total + e, pj * vs);
+                }
             }
         } else {
             const double gv = in_lds ? lds[(row - r_lo) * KT + col]
@@ -385,7 +430,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
                 const double2 f0 = f[0];
                 s = f0.x;
                 W = f0.y;
-                if (Grad) {
+                if (Tan) {
                     const double2 f1 = f[1], f2 = f[2];
                     d = f1.x;
                     b = f1.y;
@@ -416,12 +461,21 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
         if (n_cut)
             atomicAdd(a.expansions + 1, (unsigned long long)n_cut);
     }
-    if (!Grad) {
+    if (Mode == kForward) {
         if (in_lds)
             for (uint32_t i = lane; i < win_n; i += 64) {
                 const double v = lds[i];
                 if (v != 0.0)
                     unsafeAtomicAdd(a.h + hw0 + i, v);
+            }
+    } else if (Jac) {
+        if (in_lds)
+            for (uint32_t i = lane; i < win_t; i += 64) {
+                const double v = lds[i];
+                const uint32_t part = i / win_n, e = i - part * win_n;
+                double *dst = part == 0 ? a.h : a.jac + (uint64_t)(part - 1) * a.h_total;
+                if (v != 0.0)
+                    unsafeAtomicAdd(dst + hw0 + e, v);
             }
     } else {
         for (uint32_t i = lane; i < 3 * T; i += 64) {
@@ -435,7 +489,14 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
 template <bool Grad>
 __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
 {
-    relaxed_walks<Grad, false>(a);
+    relaxed_walks<Grad ? kAdjoint : kForward, false>(a);
+}
+
+// h, dh/dmean and dh/dsd in one pass (optionally with a cell stride).
+template <bool Strided>
+__global__ __launch_bounds__(64) void relaxed_jacobian_kernel(RelaxArgs a)
+{
+    relaxed_walks<kJacobian, Strided>(a);
 }
 
 // With a cell stride; held to four wavefronts per SIMD (the extra refill
@@ -443,7 +504,7 @@ __global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
 template <bool Grad>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void relaxed_strided_kernel(RelaxArgs a)
 {
-    relaxed_walks<Grad, true>(a);
+    relaxed_walks<Grad ? kAdjoint : kForward, true>(a);
 }
 
 // Quiescent types: each initial cell of a row with value >= phi records p_j
@@ -767,6 +828,7 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     a.prolif = (const uint32_t *)(base + o_pr);
     a.quiet = (const uint32_t *)(base + o_qu);
     a.expansions = (unsigned long long *)(base + o_ex);
+    a.h_total = c.h_total;
     if (c.n_waves) {
         const size_t stack_bytes = (size_t)c.n_waves * 64 * kdim_max * kFrameDoubles * sizeof(double);
         if (hipMalloc((void **)&c.d_stack, stack_bytes) != hipSuccess)
@@ -776,13 +838,20 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     return PROCELL_OK;
 }
 
-// Enqueues one pass: the walk kernel and the quiescent kernel (the caller
-// has cleared h, or the gradient, and the expansion counter).
-hipError_t core_launch(const RelaxCore &c, const RelaxArgs &a, bool grad, hipStream_t st)
+// Enqueues one pass of a WalkMode: the walk kernel and the quiescent kernel
+// (the caller has cleared h, the Jacobian or the gradient, and the expansion
+// counter).
+hipError_t core_launch(const RelaxCore &c, const RelaxArgs &a, int mode, hipStream_t st)
 {
+    const bool grad = mode == kAdjoint;
     if (c.any_prolif && c.n_waves) {
         const dim3 g(c.n_waves), b(64);
-        if (a.stride_mask) {
+        if (mode == kJacobian) {
+            if (a.stride_mask)
+                hipLaunchKernelGGL(relaxed_jacobian_kernel<true>, g, b, c.lds_bytes, st, a);
+            else
+                hipLaunchKernelGGL(relaxed_jacobian_kernel<false>, g, b, c.lds_bytes, st, a);
+        } else if (a.stride_mask) {
             if (grad)
                 hipLaunchKernelGGL(relaxed_strided_kernel<true>, g, b, c.lds_bytes, st, a);
             else
@@ -1029,8 +1098,8 @@ void destroy_calib(procell_calib *c)
     delete c;
 }
 
-int relaxed_call(procell_relaxed *r, bool grad, const double *theta, double *h, const double *g, double *gr,
-                 void *stream)
+int relaxed_call(procell_relaxed *r, int mode, const double *theta, double *h, const double *g, double *gr,
+                 double *jac, void *stream)
 {
     RELAX_TRY(hipSetDevice(r->core.device));
     hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
@@ -1039,12 +1108,16 @@ int relaxed_call(procell_relaxed *r, bool grad, const double *theta, double *h, 
     a.h = h;
     a.g = g;
     a.grad = gr;
+    a.jac = jac;
     RELAX_TRY(launch_zero(a.expansions, 2, st));
-    if (!grad)
-        RELAX_TRY(launch_zero((unsigned long long *)h, r->core.h_total, st));
-    else
+    if (mode == kAdjoint) {
         RELAX_TRY(launch_zero((unsigned long long *)gr, 3ull * r->core.types_total, st));
-    RELAX_TRY(core_launch(r->core, a, grad, st));
+    } else {
+        RELAX_TRY(launch_zero((unsigned long long *)h, r->core.h_total, st));
+        if (mode == kJacobian)
+            RELAX_TRY(launch_zero((unsigned long long *)jac, 2ull * r->core.h_total, st));
+    }
+    RELAX_TRY(core_launch(r->core, a, mode, st));
     return PROCELL_OK;
 }
 
@@ -1054,12 +1127,12 @@ hipError_t calib_enqueue(procell_calib *c, hipStream_t st)
     hipError_t e = launch_zero(c->d_clear, c->clear_words, st);
     if (e != hipSuccess)
         return e;
-    if ((e = core_launch(c->core, c->fwd, false, st)) != hipSuccess)
+    if ((e = core_launch(c->core, c->fwd, kForward, st)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(calib_loss_kernel, dim3(c->core.n_sets), dim3(256), 0, st, c->cargs);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    if ((e = core_launch(c->core, c->bwd, true, st)) != hipSuccess)
+    if ((e = core_launch(c->core, c->bwd, kAdjoint, st)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(calib_adam_kernel, dim3(1), dim3(256), 0, st, c->cargs);
     return hipGetLastError();
@@ -1088,6 +1161,40 @@ int procell_relaxed_create(const procell_params *p, double beta, procell_relaxed
     return PROCELL_OK;
 }
 
+int procell_relaxed_create_batch(const procell_params *sets, uint32_t n_sets, double beta, procell_relaxed **out)
+{
+    if (!sets || !out || n_sets == 0)
+        return PROCELL_E_INVALID;
+    *out = nullptr;
+    procell_relaxed *r = new (std::nothrow) procell_relaxed();
+    if (!r)
+        return PROCELL_E_OUT_OF_MEMORY;
+    const int rc = core_build(r->core, sets, n_sets, beta, false);
+    if (rc != PROCELL_OK) {
+        destroy_relaxed(r);
+        return rc;
+    }
+    *out = r;
+    return PROCELL_OK;
+}
+
+int procell_relaxed_set_shape(const procell_relaxed *r, uint32_t set, uint64_t *h_base, uint64_t *n_rows,
+                              uint32_t *kdim, uint32_t *n_types)
+{
+    if (!r || set >= r->core.n_sets)
+        return PROCELL_E_INVALID;
+    const RelaxSet &S = r->core.sets[set];
+    if (h_base)
+        *h_base = S.h_base;
+    if (n_rows)
+        *n_rows = S.n_rows;
+    if (kdim)
+        *kdim = S.kdim;
+    if (n_types)
+        *n_types = S.n_types;
+    return PROCELL_OK;
+}
+
 int procell_relaxed_shape(const procell_relaxed *r, uint64_t *n_rows, uint32_t *kdim, uint32_t *n_types)
 {
     if (!r)
@@ -1106,7 +1213,7 @@ int procell_relaxed_forward(procell_relaxed *r, const double *theta, double *h, 
 {
     if (!r || !theta || !h)
         return PROCELL_E_INVALID;
-    return relaxed_call(r, false, theta, h, nullptr, nullptr, stream);
+    return relaxed_call(r, kForward, theta, h, nullptr, nullptr, nullptr, stream);
 }
 
 int procell_relaxed_backward(procell_relaxed *r, const double *theta, const double *g, double *grad,
@@ -1114,7 +1221,14 @@ int procell_relaxed_backward(procell_relaxed *r, const double *theta, const doub
 {
     if (!r || !theta || !g || !grad)
         return PROCELL_E_INVALID;
-    return relaxed_call(r, true, theta, nullptr, g, grad, stream);
+    return relaxed_call(r, kAdjoint, theta, nullptr, g, grad, nullptr, stream);
+}
+
+int procell_relaxed_jacobian(procell_relaxed *r, const double *theta, double *h, double *jac, void *stream)
+{
+    if (!r || !theta || !h || !jac)
+        return PROCELL_E_INVALID;
+    return relaxed_call(r, kJacobian, theta, h, nullptr, nullptr, jac, stream);
 }
 
 int procell_relaxed_set_stride(procell_relaxed *r, uint32_t stride, uint32_t phase)

@@ -103,6 +103,7 @@ EXPORTS = (
     "procell_batch_create", "procell_batch_run", "procell_batch_result", "procell_batch_autotune",
     "procell_relaxed_create", "procell_relaxed_shape", "procell_relaxed_forward", "procell_relaxed_backward",
     "procell_relaxed_set_stride", "procell_relaxed_expansions", "procell_relaxed_destroy",
+    "procell_relaxed_jacobian", "procell_relaxed_create_batch", "procell_relaxed_set_shape",
     "procell_calib_create", "procell_calib_step", "procell_calib_theta", "procell_calib_loss",
     "procell_calib_stream", "procell_calib_destroy",
 )
@@ -160,6 +161,11 @@ def lib():
         L.procell_relaxed_expansions.argtypes = [ctypes.c_void_p, _u64p]
         L.procell_relaxed_set_stride.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
         L.procell_relaxed_destroy.argtypes = [ctypes.c_void_p]
+    if hasattr(L, "procell_relaxed_jacobian"):
+        L.procell_relaxed_jacobian.argtypes = [ctypes.c_void_p] * 5
+        L.procell_relaxed_create_batch.argtypes = [P, ctypes.c_uint32, ctypes.c_double,
+                                                   ctypes.POINTER(ctypes.c_void_p)]
+        L.procell_relaxed_set_shape.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _u64p, _u64p, _u32p, _u32p]
     if hasattr(L, "procell_calib_create"):
         L.procell_calib_create.argtypes = [P, ctypes.c_uint32, ctypes.c_double, R, ctypes.POINTER(AdamStruct),
                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
@@ -427,6 +433,16 @@ class Relaxed:
                                               self._stream(stream)),
                "relaxed_backward")
 
+    def jacobian(self, theta, h, jac, stream=None):
+        """h and jac [2, n_rows, kdim, n_types] = dh/dmean, dh/dsd of each
+        entry's own type, in one pass (dh/dproportion_j = h / proportion_j)."""
+        T = self.shape[2]
+        _check(lib().procell_relaxed_jacobian(self.handle, self._ptr(theta, (3, T), "theta"),
+                                              self._ptr(h, self.shape, "h"),
+                                              self._ptr(jac, (2,) + tuple(self.shape), "jac"),
+                                              self._stream(stream)),
+               "relaxed_jacobian")
+
     def set_stride(self, stride: int, phase: int = 0):
         """Later passes walk only the initial cells c with (c + phase) a
         multiple of `stride` (a power of two); 1 walks every cell."""
@@ -454,6 +470,54 @@ class Relaxed:
             self.close()
         except Exception:
             pass
+
+
+class RelaxedBatch(Relaxed):
+    """procell_relaxed_create_batch: one relaxed plan over many problems.
+    theta, h, g, grad and jac are flat device tensors, the sets' arrays
+    concatenated in order: theta [3 * types_total] (set i's [3, n_types] at
+    theta_base[i]), h and g [h_total] (set i's [n_rows, kdim, n_types] at
+    h_base[i]), grad [3 * types_total], jac [2, h_total]."""
+
+    def __init__(self, problems, beta: float):
+        self.problems = list(problems)
+        self.beta = float(beta)
+        n = len(self.problems)
+        arr = (Params * max(n, 1))(*[p.params for p in self.problems])
+        self.handle = ctypes.c_void_p()
+        _check(lib().procell_relaxed_create_batch(arr, n, self.beta, ctypes.byref(self.handle)), "relaxed_create_batch")
+        self.shapes, self.h_base, self.theta_base = [], [], []
+        tb = 0
+        for i in range(n):
+            hb, r, k, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
+            _check(lib().procell_relaxed_set_shape(self.handle, i, ctypes.byref(hb), ctypes.byref(r),
+                                                   ctypes.byref(k), ctypes.byref(t)), "relaxed_set_shape")
+            self.shapes.append((int(r.value), int(k.value), int(t.value)))
+            self.h_base.append(int(hb.value))
+            self.theta_base.append(tb)
+            tb += 3 * int(t.value)
+        self.types_total = tb // 3
+        last = self.shapes[-1]
+        self.h_total = self.h_base[-1] + last[0] * last[1] * last[2]
+        self.shape = (self.h_total,)
+
+    def forward(self, theta, h, stream=None):
+        _check(lib().procell_relaxed_forward(self.handle, self._ptr(theta, (3 * self.types_total,), "theta"),
+                                             self._ptr(h, (self.h_total,), "h"), self._stream(stream)),
+               "relaxed_forward")
+
+    def backward(self, theta, g, grad, stream=None):
+        n3 = 3 * self.types_total
+        _check(lib().procell_relaxed_backward(self.handle, self._ptr(theta, (n3,), "theta"),
+                                              self._ptr(g, (self.h_total,), "g"), self._ptr(grad, (n3,), "grad"),
+                                              self._stream(stream)),
+               "relaxed_backward")
+
+    def jacobian(self, theta, h, jac, stream=None):
+        _check(lib().procell_relaxed_jacobian(self.handle, self._ptr(theta, (3 * self.types_total,), "theta"),
+                                              self._ptr(h, (self.h_total,), "h"),
+                                              self._ptr(jac, (2, self.h_total), "jac"), self._stream(stream)),
+               "relaxed_jacobian")
 
 
 class Calibration:

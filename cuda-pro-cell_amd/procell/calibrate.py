@@ -221,3 +221,195 @@ def plain_loss(bins, value, count, target_q):
     s = h.sum()
     r = (h / s if s > 0 else h) - target_q
     return float((r * r).sum())
+
+
+# ---------------------------------------------------------------------------
+# Second order: Levenberg-Marquardt per set over a batched relaxed plan.
+# ---------------------------------------------------------------------------
+LOG_MAX_SD_RATIO = float(np.log(0.8))  # sd <= 0.8 mean keeps the relaxed tree finite
+MAX_LOG_STEP = 0.5                     # largest change of one parameter per iteration
+
+
+def spd_solve(A: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """x with A x = b for a batch of symmetric positive definite A [n, P, P]
+    and b [n, P]: Gauss-Jordan elimination without pivoting (the pivots of
+    an SPD matrix are positive), P batched torch steps, no host round trip."""
+    M = torch.cat([A, b.unsqueeze(-1)], dim=-1).clone()
+    P = A.shape[-1]
+    for k in range(P):
+        row = M[:, k:k + 1, :] / M[:, k:k + 1, k:k + 1]
+        M = M - M[:, :, k:k + 1] * row
+        M[:, k:k + 1, :] = row
+    return M[:, :, P]
+
+
+def normal_equations(Hj, MJ, SJ, mean, sd, p, q, active):
+    """Gauss-Newton terms of the normalised-L2 loss of every set.
+
+    Hj, MJ, SJ [n, B, T]: per bin and type, the binned h, dh/dmean and dh/dsd
+    (T padded with zeros); mean, sd, p [n, T]; q [n, B] the normalised target;
+    active [n, T, 3] the free parameters (log mean, log sd, proportion logit
+    of each type).  With H = sum over types, S = sum H, r = H / S - q and
+    J = dr/dphi over phi = (log mean, log sd, logit) per type (proportions =
+    softmax of the logits):
+        loss = |r|^2,  A = J^T J,  g = J^T r (half the loss gradient).
+    Returns (loss [n], A [n, 3T, 3T], g [n, 3T]) with inactive parameters'
+    rows and columns of A zero and their g zero."""
+    n, B, T = Hj.shape
+    H = Hj.sum(-1)
+    S = H.sum(-1)
+    Sd = torch.where(S > 0, S, torch.ones_like(S))
+    r = torch.where((S > 0)[:, None], H / Sd[:, None], torch.zeros_like(H)) - q
+    d_logit = Hj - p[:, None, :] * H[:, :, None]  # dH_b / dlogit_l = Hj[b, l] - p_l H_b
+    JH = torch.stack([MJ * mean[:, None, :], SJ * sd[:, None, :], d_logit], dim=-1)  # [n, B, T, 3]
+    JH = JH * active[:, None, :, :].to(JH.dtype)
+    JH = JH.reshape(n, B, 3 * T)
+    J = (JH - (H / Sd[:, None])[:, :, None] * JH.sum(1, keepdim=True)) / Sd[:, None, None]
+    loss = (r * r).sum(-1)
+    Jt = J.transpose(1, 2).contiguous()
+    A = torch.bmm(Jt, J)
+    g = torch.bmm(Jt, r.unsqueeze(-1)).squeeze(-1)
+    return loss, A, g
+
+
+def lm_step(A, g, lam, active):
+    """Levenberg-Marquardt step -(A + lam (diag A + tiny))^-1 g, inactive
+    parameters held (their system rows become the identity), each component
+    limited to MAX_LOG_STEP."""
+    n, P, _ = A.shape
+    act = active.reshape(n, P).to(A.dtype)
+    D = torch.diagonal(A, dim1=1, dim2=2)
+    scale = D.amax(-1, keepdim=True).clamp_min(1e-300)
+    damp = lam[:, None] * (D + 1e-9 * scale)
+    M = A * act[:, :, None] * act[:, None, :] + torch.diag_embed(damp * act + (1.0 - act))
+    x = -spd_solve(M, g * act)
+    return x.clamp(-MAX_LOG_STEP, MAX_LOG_STEP) * act
+
+
+class BatchLM:
+    """Levenberg-Marquardt calibration of many parameter sets at once.
+
+    Each iteration is one relaxed Jacobian pass over all sets
+    (procell_relaxed_jacobian on a batch plan) at every set's trial point,
+    then batched torch ops on the device: binning, the Gauss-Newton system
+    of the normalised-L2 loss in (log mean, log sd, proportion logits), the
+    accept / reject of each set's trial (damping / 5 on success, x 4
+    otherwise) and the next trial.  Nothing returns to the host between
+    iterations.  Keeps sd <= 0.8 mean."""
+
+    def __init__(self, plan, problems, targets, lam0=1e-3, device="cuda"):
+        """plan: a RelaxedBatch over `problems` (their parameters are the
+        start); targets: per set a plain-method output, (value, count)."""
+        self.plan = plan
+        self.dev = torch.device(device)
+        n = len(problems)
+        T = max(s[2] for s in plan.shapes)
+        binsets = []
+        for prob, s in zip(problems, plan.shapes):
+            bins, idx = output_bins(prob, s[1])
+            binsets.append((bins, idx))
+        B = max(len(b) for b, _ in binsets)
+        self.n, self.T, self.B = n, T, B
+        ent, dst = [], []
+        q = np.zeros((n, B))
+        prolif = np.zeros((n, T), dtype=bool)
+        real = np.zeros((n, T), dtype=bool)
+        th_src = []
+        phi0 = np.zeros((n, T, 3))
+        for i, (prob, (bins, idx), s, tg) in enumerate(zip(problems, binsets, plan.shapes, targets)):
+            tv, tc = (tg.value, tg.count) if hasattr(tg, "value") else tg
+            R, K, Ti = s
+            base = plan.h_base[i]
+            flat = idx.reshape(-1)
+            rk = np.nonzero(flat >= 0)[0]
+            for j in range(Ti):
+                ent.append(base + rk * Ti + j)
+                dst.append((i * B + flat[rk]) * T + j)
+            t = target_histogram(bins, np.asarray(tv), np.asarray(tc))
+            if t.sum() > 0:
+                q[i, :len(bins)] = t / t.sum()
+            m, sd, pr = prob.mean, prob.sd, prob.proportion
+            pf = np.asarray(m) >= 0
+            prolif[i, :Ti] = pf
+            real[i, :Ti] = True
+            phi0[i, :Ti, 0] = np.log(np.where(pf, m, 1.0))
+            phi0[i, :Ti, 1] = np.log(np.where(pf, sd, 1.0))
+            phi0[i, :Ti, 2] = np.log(np.maximum(pr, 1e-12))
+            for c in range(3):
+                th_src.extend(((i * 3 + c) * T + j) for j in range(Ti))
+        d = self.dev
+        self.ent = torch.tensor(np.concatenate(ent), dtype=torch.int64, device=d)
+        self.dst = torch.tensor(np.concatenate(dst), dtype=torch.int64, device=d)
+        self.q = torch.tensor(q, dtype=torch.float64, device=d)
+        self.prolif = torch.tensor(prolif, device=d)
+        self.real = torch.tensor(real, device=d)
+        self.th_src = torch.tensor(th_src, dtype=torch.int64, device=d)
+        # logits: all but each set's last type (softmax is unchanged by a
+        # common shift, so one logit is fixed; a one-type set has none free)
+        last = real.sum(1, keepdims=True) - 1
+        free_logit = real & (np.arange(T)[None, :] < last)
+        act = np.stack([prolif, prolif, free_logit], axis=-1)
+        self.active = torch.tensor(act, device=d)
+        self.phi = torch.tensor(phi0, dtype=torch.float64, device=d)
+        self.lam = torch.full((n,), float(lam0), dtype=torch.float64, device=d)
+        self.h = torch.empty(plan.h_total, dtype=torch.float64, device=d)
+        self.jac = torch.empty((2, plan.h_total), dtype=torch.float64, device=d)
+        self.iterations = 0
+        self.loss = self.A = self.g = None
+
+    def start(self):
+        """Evaluates the starting point (one Jacobian pass); step() does it
+        first when it has not been done."""
+        if self.loss is None:
+            self.loss, self.A, self.g = self._evaluate(self.phi)
+
+    def _params(self, phi):
+        neg = torch.full_like(phi[..., 0], -1.0)
+        mean = torch.where(self.prolif, torch.exp(phi[..., 0]), neg)
+        sd = torch.where(self.prolif, torch.exp(phi[..., 1]), neg)
+        logits = torch.where(self.real, phi[..., 2], torch.full_like(phi[..., 2], -torch.inf))
+        p = torch.softmax(logits, -1)
+        return mean, sd, p
+
+    def theta_flat(self, phi):
+        mean, sd, p = self._params(phi)
+        return torch.stack([mean, sd, p], 1).reshape(-1).index_select(0, self.th_src).contiguous()
+
+    def _evaluate(self, phi):
+        mean, sd, p = self._params(phi)
+        self.plan.jacobian(self.theta_flat(phi), self.h, self.jac)
+        n, B, T = self.n, self.B, self.T
+        vals = torch.stack([self.h.index_select(0, self.ent), self.jac[0].index_select(0, self.ent),
+                            self.jac[1].index_select(0, self.ent)], -1)
+        tally = torch.zeros((n * B * T, 3), dtype=torch.float64, device=self.dev).index_add_(0, self.dst, vals)
+        tally = tally.reshape(n, B, T, 3)
+        return normal_equations(tally[..., 0], tally[..., 1], tally[..., 2], mean.clamp_min(0), sd.clamp_min(0), p,
+                                self.q, self.active)
+
+    def _project(self, phi):
+        ls = torch.minimum(phi[..., 1], phi[..., 0] + LOG_MAX_SD_RATIO)
+        return torch.stack([phi[..., 0], ls, phi[..., 2]], -1)
+
+    def step(self, n_iter=1):
+        """n_iter Levenberg-Marquardt iterations of every set."""
+        self.start()
+        for _ in range(n_iter):
+            dx = lm_step(self.A, self.g, self.lam, self.active).reshape(self.n, self.T, 3)
+            trial = self._project(self.phi + dx)
+            loss, A, g = self._evaluate(trial)
+            ok = loss < self.loss
+            self.phi = torch.where(ok[:, None, None], trial, self.phi)
+            self.loss = torch.where(ok, loss, self.loss)
+            self.A = torch.where(ok[:, None, None], A, self.A)
+            self.g = torch.where(ok[:, None], g, self.g)
+            self.lam = torch.where(ok, self.lam * 0.2, self.lam * 4.0).clamp(1e-9, 1e9)
+            self.iterations += 1
+
+    def thetas(self):
+        """Per set [3, n_types] numpy (mean, sd, proportion; -1 for quiescent)."""
+        mean, sd, p = (x.cpu().numpy() for x in self._params(self.phi))
+        out = []
+        for i, s in enumerate(self.plan.shapes):
+            Ti = s[2]
+            out.append(np.stack([mean[i, :Ti], sd[i, :Ti], p[i, :Ti]]))
+        return out

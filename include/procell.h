@@ -209,6 +209,22 @@ int procell_relaxed_backward(procell_relaxed *plan, const double *theta, const d
  * at most 65536) are walked, so the passes of phases 0 .. stride-1 sum to
  * the full pass; 1 (the default) walks every cell. */
 int procell_relaxed_set_stride(procell_relaxed *plan, uint32_t stride, uint32_t phase);
+/* h and its Jacobian in one pass.  The entries of type j depend on type j's
+ * (mean, sd, proportion) only, so the Jacobian is three numbers per entry:
+ *   jac : device [2][n_rows][kdim][n_types] = dh/dmean_j, dh/dsd_j of each
+ *         entry's own type j (0 for quiescent types); dh/dproportion_j is
+ *         h / proportion_j.
+ * Enqueued like forward.  No counterpart in the reference: it serves
+ * second-order (Gauss-Newton) calibration. */
+int procell_relaxed_jacobian(procell_relaxed *plan, const double *theta, double *h, double *jac, void *stream);
+/* A plan over n_sets parameter sets walked by the same launches (sets may
+ * differ in histogram, types, t_max, phi and seed; one device).  theta, h,
+ * jac, g and grad are the sets' arrays concatenated in set order: set i's
+ * theta starts at 3 x (types of sets before i), its h at h_base (below).
+ * procell_relaxed_shape describes set 0. */
+int procell_relaxed_create_batch(const procell_params *sets, uint32_t n_sets, double beta, procell_relaxed **plan);
+int procell_relaxed_set_shape(const procell_relaxed *plan, uint32_t set, uint64_t *h_base, uint64_t *n_rows,
+                              uint32_t *kdim, uint32_t *n_types);
 /* Expansions of the plan's last call (roots drawn plus nodes whose daughters
  * were drawn); waits for the device.  Returns PROCELL_E_DEPTH (with *n set)
  * when a walk needed more than 2^22 expansions and was cut short: the call's

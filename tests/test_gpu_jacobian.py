@@ -1,0 +1,128 @@
+"""GPU: the relaxed Jacobian pass, the batch plan and Levenberg-Marquardt.
+
+procell_relaxed_jacobian tallies h and, per entry, dh/dmean_j and dh/dsd_j
+of the entry's own type; contracted with a cotangent g it must give the
+oracle's relaxed gradient (oracle_relaxed), with dh/dproportion_j = h /
+proportion_j, within the tolerances of the relaxed parity tests (FP64
+atomics in another order).  A batch plan must give every set's single-plan
+forward, Jacobian and adjoint.  Levenberg-Marquardt over a batch plan must
+never increase a set's loss and must reduce the parameter error of a
+calibration batch.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_relaxed import G_ATOL, G_RTOL, H_ATOL, H_RTOL, _args, _problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _close(x, ref, rtol, atol):
+    scale = max(1.0, float(np.abs(ref).max())) if ref.size else 1.0
+    return bool(np.all(np.abs(x - ref) <= rtol * np.abs(ref) + atol * scale))
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_jacobian_contracts_to_oracle_gradient(procell_lib, oracle, case):
+    import torch
+
+    d, beta = _problems()[case]
+    print("jacobian case", case, "beta", beta, flush=True)
+    h_ref, _ = oracle.relaxed(oracle.Problem(*_args(d)), beta)
+    g = np.random.default_rng(100 + case).normal(size=h_ref.shape)
+    _, grad_ref = oracle.relaxed(oracle.Problem(*_args(d)), beta, g)
+    prob = procell_lib.Problem(*_args(d))
+    th = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
+    with procell_lib.Relaxed(prob, beta) as rp:
+        h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        jac = torch.empty((2,) + tuple(rp.shape), dtype=torch.float64, device="cuda")
+        rp.jacobian(th, h, jac)
+        assert rp.expansions() > 0
+        h, jac = h.cpu().numpy(), jac.cpu().numpy()
+    assert _close(h, h_ref, H_RTOL, H_ATOL), np.abs(h - h_ref).max()
+    p = np.asarray(d["proportion"], dtype=np.float64)
+    grad = np.stack([(g * jac[0]).sum((0, 1)), (g * jac[1]).sum((0, 1)), (g * h).sum((0, 1)) / p])
+    assert _close(grad, grad_ref, G_RTOL, G_ATOL), (grad, grad_ref)
+    quiet = np.asarray(d["mean"]) < 0
+    assert np.all(jac[:, :, :, quiet] == 0.0)
+
+
+def test_batch_plan_equals_single_plans(procell_lib):
+    import torch
+
+    cases = [c for c, _ in _problems()]
+    beta = 0.7
+    probs = [procell_lib.Problem(*_args(d)) for d in cases]
+    rng = np.random.default_rng(5)
+    with procell_lib.RelaxedBatch(probs, beta) as bp:
+        assert len(bp.shapes) == len(cases)
+        th = torch.tensor(np.concatenate([np.stack([d["mean"], d["sd"], d["proportion"]]).reshape(-1)
+                                          for d in cases]), dtype=torch.float64, device="cuda")
+        g = torch.tensor(rng.normal(size=bp.h_total), dtype=torch.float64, device="cuda")
+        h = torch.empty(bp.h_total, dtype=torch.float64, device="cuda")
+        hj = torch.empty_like(h)
+        jac = torch.empty((2, bp.h_total), dtype=torch.float64, device="cuda")
+        grad = torch.empty(3 * bp.types_total, dtype=torch.float64, device="cuda")
+        bp.forward(th, h)
+        bp.jacobian(th, hj, jac)
+        bp.backward(th, g, grad)
+        h, hj, jac, grad, g = (x.cpu().numpy() for x in (h, hj, jac, grad, g))
+        shapes, h_base, t_base = bp.shapes, bp.h_base, bp.theta_base
+    for i, (d, prob) in enumerate(zip(cases, probs)):
+        print("batch set", i, flush=True)
+        shp = shapes[i]
+        n = int(np.prod(shp))
+        sl = slice(h_base[i], h_base[i] + n)
+        thi = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
+        with procell_lib.Relaxed(prob, beta) as rp:
+            assert tuple(rp.shape) == tuple(shp)
+            h1 = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+            j1 = torch.empty((2,) + tuple(rp.shape), dtype=torch.float64, device="cuda")
+            gr1 = torch.empty_like(thi)
+            rp.forward(thi, h1)
+            h1 = h1.cpu().numpy().reshape(-1)
+            rp.jacobian(thi, torch.empty(rp.shape, dtype=torch.float64, device="cuda"), j1)
+            rp.backward(thi, torch.tensor(g[sl].reshape(shp), device="cuda"), gr1)
+            j1, gr1 = j1.cpu().numpy().reshape(2, -1), gr1.cpu().numpy().reshape(-1)
+        assert _close(h[sl], h1, H_RTOL, H_ATOL)
+        assert _close(hj[sl], h1, H_RTOL, H_ATOL)
+        assert _close(jac[:, sl], j1, H_RTOL, H_ATOL)
+        T = shp[2]
+        assert _close(grad[t_base[i]:t_base[i] + 3 * T], gr1, G_RTOL, G_ATOL)
+
+
+def test_levenberg_marquardt_reduces_loss_and_error(procell_lib):
+    import torch
+    from procell import calibrate, workloads
+
+    gen, start = workloads.calibration_batch(3, 12, 2e4)
+    with procell_lib.Batch([procell_lib.Problem.from_workload(w, 0) for w in gen]) as b:
+        b.run([1000 + i for i in range(len(gen))])
+        targets = [b.result(i, want_raw=False) for i in range(len(gen))]
+    probs = [procell_lib.Problem.from_workload(w, 50000 + i) for i, w in enumerate(start)]
+
+    def err(thetas):
+        e = [np.abs(th[0][w.mean >= 0] - w.mean[w.mean >= 0]) / w.mean[w.mean >= 0] for w, th in zip(gen, thetas)]
+        return float(np.median(np.concatenate(e)))
+
+    with procell_lib.RelaxedBatch(probs, 0.5) as plan:
+        lm = calibrate.BatchLM(plan, probs, targets)
+        lm.start()
+        e0 = err(lm.thetas())
+        losses = [lm.loss.clone()]
+        for it in range(8):
+            lm.step(1)
+            losses.append(lm.loss.clone())
+        torch.cuda.synchronize()
+        plan.expansions()  # no walk was cut
+        th = lm.thetas()
+    L = torch.stack(losses).cpu().numpy()
+    print("median loss", np.median(L, axis=1), "error", e0, "->", err(th), flush=True)
+    assert np.all(np.isfinite(L))
+    assert np.all(L[1:] <= L[:-1])
+    assert np.median(L[-1]) < 0.5 * np.median(L[0])
+    assert err(th) < 0.8 * e0
+    for w, t in zip(start, th):
+        pr = w.mean >= 0
+        assert np.all(t[1][pr] <= 0.8 * t[0][pr] * (1 + 1e-12))
+        assert abs(t[2].sum() - 1.0) < 1e-12

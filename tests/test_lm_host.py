@@ -1,0 +1,131 @@
+"""Levenberg-Marquardt host algebra on the CPU: the batched SPD solve, the
+Gauss-Newton terms of the normalised-L2 loss against torch autograd on an
+analytic stand-in for the relaxed tally, and convergence of the damped
+steps on it."""
+import numpy as np
+import pytest
+import torch
+
+from procell import calibrate as cal
+
+
+def test_spd_solve_matches_numpy():
+    rng = np.random.default_rng(3)
+    n, P = 7, 9
+    X = rng.normal(size=(n, P, P))
+    A = X @ X.transpose(0, 2, 1) + 0.1 * np.eye(P)
+    b = rng.normal(size=(n, P))
+    x = cal.spd_solve(torch.tensor(A), torch.tensor(b)).numpy()
+    np.testing.assert_allclose(x, np.linalg.solve(A, b[..., None])[..., 0], rtol=1e-9, atol=1e-12)
+
+
+# Analytic model: per set i, bin b, type j the tally is
+#   h = p_j w[i,b,j] exp(-(c_b - mean_j)^2 / (2 sd_j^2))        (proliferating)
+#   h = p_j w[i,b,j]                                             (quiescent)
+# so h of type j depends on type j's parameters only, as the relaxed tally.
+B, T = 12, 3
+CENTERS = torch.linspace(20.0, 90.0, B, dtype=torch.float64)
+W = torch.tensor(np.random.default_rng(9).uniform(0.5, 2.0, size=(2, B, T)))
+PROLIF = torch.tensor([[True, True, False], [True, False, False]])
+REAL = torch.tensor([[True, True, True], [True, True, False]])
+
+
+def _tally(mean, sd, p):
+    z = (CENTERS[None, :, None] - mean[:, None, :]) / sd[:, None, :]
+    f = torch.where(PROLIF[:, None, :], torch.exp(-0.5 * z * z), torch.ones_like(z)) * W
+    Hj = p[:, None, :] * f
+    MJ = torch.where(PROLIF[:, None, :], Hj * z / sd[:, None, :], torch.zeros_like(Hj))
+    SJ = torch.where(PROLIF[:, None, :], Hj * z * z / sd[:, None, :], torch.zeros_like(Hj))
+    return Hj, MJ, SJ
+
+
+def _params(phi):
+    mean = torch.where(PROLIF, torch.exp(phi[..., 0]), torch.ones_like(phi[..., 0]))
+    sd = torch.where(PROLIF, torch.exp(phi[..., 1]), torch.ones_like(phi[..., 1]))
+    logits = torch.where(REAL, phi[..., 2], torch.full_like(phi[..., 2], -torch.inf))
+    return mean, sd, torch.softmax(logits, -1)
+
+
+def _active():
+    last = REAL.sum(1, keepdim=True) - 1
+    free = REAL & (torch.arange(T)[None, :] < last)
+    return torch.stack([PROLIF, PROLIF, free], -1)
+
+
+def _residual(phi, q):
+    Hj, _, _ = _tally(*_params(phi))
+    H = Hj.sum(-1)
+    return H / H.sum(-1, keepdim=True) - q
+
+
+def _phi(mean, sd, p):
+    return torch.stack([torch.log(mean), torch.log(sd), torch.log(p)], -1)
+
+
+PHI_TRUE = _phi(torch.tensor([[40.0, 70.0, 1.0], [55.0, 1.0, 1.0]], dtype=torch.float64),
+                torch.tensor([[8.0, 15.0, 1.0], [12.0, 1.0, 1.0]], dtype=torch.float64),
+                torch.tensor([[0.5, 0.3, 0.2], [0.6, 0.4, 1e-300]], dtype=torch.float64))
+
+
+def test_normal_equations_match_autograd():
+    q = _residual(PHI_TRUE, 0.0).detach() + 0.01  # any target
+    phi = PHI_TRUE + 0.1 * torch.tensor(np.random.default_rng(1).normal(size=PHI_TRUE.shape))
+    act = _active()
+    mean, sd, p = _params(phi)
+    loss, A, g = cal.normal_equations(*_tally(mean, sd, p), mean, sd, p, q, act)
+    r = _residual(phi, q)
+    np.testing.assert_allclose(loss.numpy(), (r * r).sum(-1).numpy(), rtol=1e-12)
+    for i in range(2):
+        # J of r_i in the active parameters of set i, by autograd
+        def ri(x, i=i):
+            full = phi.clone()
+            full[i] = x.reshape(T, 3)
+            return _residual(full, q)[i]
+
+        J = torch.autograd.functional.jacobian(ri, phi[i].reshape(-1).clone())
+        mask = act[i].reshape(-1).to(torch.float64)
+        J = J * mask[None, :]
+        np.testing.assert_allclose(g[i].numpy(), (J.T @ r[i]).numpy(), rtol=1e-9, atol=1e-15)
+        np.testing.assert_allclose(A[i].numpy(), (J.T @ J).numpy(), rtol=1e-9, atol=1e-15)
+
+
+def test_damped_steps_converge_on_exact_target():
+    q = _residual(PHI_TRUE, 0.0).detach()
+    act = _active()
+    start = PHI_TRUE + torch.tensor([[[0.12, -0.1, 0.3], [-0.1, 0.1, -0.2], [0.0, 0.0, 0.0]],
+                                     [[-0.12, 0.12, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]]], dtype=torch.float64)
+    phi = start.clone()
+    lam = torch.full((2,), 1e-3, dtype=torch.float64)
+
+    def ev(x):
+        mean, sd, p = _params(x)
+        return cal.normal_equations(*_tally(mean, sd, p), mean, sd, p, q, act)
+
+    loss, A, g = ev(phi)
+    losses = [loss]
+    for _ in range(40):
+        trial = phi + cal.lm_step(A, g, lam, act).reshape(2, T, 3)
+        l2, A2, g2 = ev(trial)
+        ok = l2 < loss
+        phi = torch.where(ok[:, None, None], trial, phi)
+        loss = torch.where(ok, l2, loss)
+        A = torch.where(ok[:, None, None], A2, A)
+        g = torch.where(ok[:, None], g2, g)
+        lam = torch.where(ok, lam * 0.2, lam * 4.0).clamp(1e-9, 1e9)
+        losses.append(loss)
+    assert all((b <= a).all() for a, b in zip(losses, losses[1:]))
+    assert float(loss.max()) < 1e-20
+    m, s, p = _params(phi)
+    mt, st, pt = _params(PHI_TRUE)
+    np.testing.assert_allclose(m[PROLIF].numpy(), mt[PROLIF].numpy(), rtol=1e-6)
+    np.testing.assert_allclose(s[PROLIF].numpy(), st[PROLIF].numpy(), rtol=1e-6)
+    np.testing.assert_allclose(p[REAL].numpy(), pt[REAL].numpy(), atol=1e-8)
+
+
+def test_step_holds_inactive_parameters_and_limits_size():
+    act = _active()
+    A = torch.eye(3 * T, dtype=torch.float64).expand(2, -1, -1).clone() * 1e-6
+    g = torch.ones((2, 3 * T), dtype=torch.float64)
+    x = cal.lm_step(A, g, torch.full((2,), 1e-3, dtype=torch.float64), act).reshape(2, T, 3)
+    assert (x[~act] == 0).all()
+    assert (x[act].abs() <= cal.MAX_LOG_STEP).all() and (x[act] == -cal.MAX_LOG_STEP).all()
