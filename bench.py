@@ -368,7 +368,8 @@ def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_roun
                       "note": "forward pass on the cells of phase step mod 4, adjoint on phase step + 1 mod 4 "
                               "(double sampling: an unbiased gradient of the squared loss)"}
     if lm_iters:
-        out["levenberg_marquardt"] = lm_measurement(procell, calibrate, np, gen, probs, targets, lm_iters, rec0)
+        out["levenberg_marquardt"] = lm_measurement(procell, calibrate, np, gen, probs, targets, lm_iters, rec0,
+                                                    traj)
     if with_nm:
         out["derivative_free"] = nelder_mead_baseline(procell, calibrate, np, gen, start, targets, nm_rounds, traj,
                                                       straj, out.get("levenberg_marquardt", {}).get("trajectory"))
@@ -377,7 +378,7 @@ def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_roun
     return out
 
 
-def lm_measurement(procell, calibrate, np, gen, probs, targets, iters, rec0):
+def lm_measurement(procell, calibrate, np, gen, probs, targets, iters, rec0, adam_traj):
     """Levenberg-Marquardt per set over one batch relaxed plan of all sets:
     each iteration = one Jacobian pass (h, dh/dmean, dh/dsd) of every set's
     trial point + batched torch algebra on the device; the start point's
@@ -402,9 +403,16 @@ def lm_measurement(procell, calibrate, np, gen, probs, targets, iters, rec0):
     except procell.ProcellError:
         cut = True
     plan.close()
+    # GPU time to the median loss Adam ends at (same loss, same relaxed stream seeds)
+    goal = adam_traj[-1]["loss_median"]
+    t_lm = next((p["gpu_s"] for p in traj[1:] if p["loss_median"] <= goal), None)
+    t_adam = next((p["gpu_s"] for p in adam_traj[1:] if p["loss_median"] <= goal), None)
     log(f"levenberg-marquardt: {iters} iterations, {gpu:.2f} s ({gpu / iters * 1e3:.2f} ms/iteration); "
-        f"mean rel err median {traj[-1]['mean_rel_median']:.4f}")
+        f"mean rel err median {traj[-1]['mean_rel_median']:.4f}; Adam's final median loss {goal:.3e} "
+        f"after {t_lm} s (Adam {t_adam} s)")
     return {"iterations": iters, "ms_per_iteration": gpu / iters * 1e3, "walks_cut": cut,
+            "adam_final_loss_median": goal, "gpu_s_to_adam_final_loss": t_lm, "gpu_s_adam_to_it": t_adam,
+            "gpu_time_ratio_adam_over_lm_at_loss": (t_adam / t_lm) if (t_lm and t_adam) else None,
             "method": "Levenberg-Marquardt per set on (log mean, log sd, proportion logits), damping x 0.2 on an "
                       "accepted trial and x 4 on a rejected one, steps limited to 0.5 per parameter, "
                       "sd <= 0.8 mean; Gauss-Newton system from one relaxed Jacobian pass per iteration",
