@@ -274,11 +274,12 @@ def normal_equations(Hj, MJ, SJ, mean, sd, p, q, active):
 
 def lm_step(A, g, lam, active):
     """Levenberg-Marquardt step -(A + lam (diag A + tiny))^-1 g, inactive
-    parameters held (their system rows become the identity), each component
-    limited to MAX_LOG_STEP."""
+    parameters and those the loss does not depend on (a zero Jacobian
+    column) held -- their system rows become the identity -- and each
+    component limited to MAX_LOG_STEP."""
     n, P, _ = A.shape
-    act = active.reshape(n, P).to(A.dtype)
     D = torch.diagonal(A, dim1=1, dim2=2)
+    act = (active.reshape(n, P) & (D > 0)).to(A.dtype)
     scale = D.amax(-1, keepdim=True).clamp_min(1e-300)
     damp = lam[:, None] * (D + 1e-9 * scale)
     M = A * act[:, :, None] * act[:, None, :] + torch.diag_embed(damp * act + (1.0 - act))

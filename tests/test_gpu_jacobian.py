@@ -126,3 +126,40 @@ def test_levenberg_marquardt_reduces_loss_and_error(procell_lib):
         pr = w.mean >= 0
         assert np.all(t[1][pr] <= 0.8 * t[0][pr] * (1 + 1e-12))
         assert abs(t[2].sum() - 1.0) < 1e-12
+
+
+def test_levenberg_marquardt_edge_sets(procell_lib):
+    """An all-quiescent set (no walks; its normalised histogram does not
+    depend on its proportions, so nothing moves), a one-type set (no free
+    logit) and a set with an empty target step with the others: every loss
+    stays finite and never increases, the all-quiescent set keeps its
+    proportions and the one-type set's proportion stays 1."""
+    import torch
+    from procell import calibrate, workloads
+
+    gen, start = workloads.calibration_batch(4, 2, 5000)
+    with procell_lib.Batch([procell_lib.Problem.from_workload(w, 0) for w in gen]) as b:
+        b.run([1000, 1001])
+        targets = [b.result(i, want_raw=False) for i in range(2)]
+    w = start[0]
+    quiet = procell_lib.Problem(w.values, w.counts, [0.3, 0.7], [-1.0, -1.0], [-1.0, -1.0], w.t_max, w.phi, 5)
+    one = procell_lib.Problem(w.values, w.counts, [1.0], [w.mean[0]], [w.sd[0]], w.t_max, w.phi, 6)
+    probs = [procell_lib.Problem.from_workload(s, 50 + i) for i, s in enumerate(start)] + [quiet, one]
+    empty = (np.zeros(0), np.zeros(0, dtype=np.int64))
+    tg = [targets[0], empty, targets[0], targets[0]]
+    with procell_lib.RelaxedBatch(probs, 0.5) as plan:
+        lm = calibrate.BatchLM(plan, probs, tg)
+        lm.start()
+        losses = [lm.loss.clone()]
+        for _ in range(4):
+            lm.step(1)
+            losses.append(lm.loss.clone())
+        th = lm.thetas()
+    L = torch.stack(losses).cpu().numpy()
+    print("losses", L, flush=True)
+    assert np.all(np.isfinite(L)) and np.all(L[1:] <= L[:-1])
+    assert L[-1][2] == L[0][2]
+    assert np.allclose(th[2][2], [0.3, 0.7], rtol=1e-12)
+    assert L[-1][3] < L[0][3] and L[-1][0] < L[0][0]
+    assert th[3][2].tolist() == [1.0]
+    assert np.all(th[2][0] == -1.0) and abs(th[2][2].sum() - 1.0) < 1e-12

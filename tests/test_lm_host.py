@@ -129,3 +129,16 @@ def test_step_holds_inactive_parameters_and_limits_size():
     x = cal.lm_step(A, g, torch.full((2,), 1e-3, dtype=torch.float64), act).reshape(2, T, 3)
     assert (x[~act] == 0).all()
     assert (x[act].abs() <= cal.MAX_LOG_STEP).all() and (x[act] == -cal.MAX_LOG_STEP).all()
+
+
+def test_step_holds_parameters_the_loss_ignores():
+    """A zero Jacobian column (e.g. the proportions of an all-quiescent set,
+    which the normalised histogram does not see) gives a zero step, not 0/0."""
+    act = _active()
+    A = torch.zeros((2, 3 * T, 3 * T), dtype=torch.float64)
+    A[0, 0, 0] = 2.0
+    g = torch.zeros((2, 3 * T), dtype=torch.float64)
+    g[0, 0] = 1.0
+    x = cal.lm_step(A, g, torch.full((2,), 1e-9, dtype=torch.float64), act)
+    assert torch.isfinite(x).all()
+    assert x[0, 0] < 0 and (x[0, 1:] == 0).all() and (x[1] == 0).all()
