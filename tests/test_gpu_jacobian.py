@@ -47,6 +47,32 @@ def test_jacobian_contracts_to_oracle_gradient(procell_lib, oracle, case):
     assert np.all(jac[:, :, :, quiet] == 0.0)
 
 
+def test_strided_jacobian_passes_sum_to_full(procell_lib):
+    """With cell stride 4 the Jacobian passes of phases 0..3 (the strided
+    instantiation) sum to the full pass, h and both derivative tallies."""
+    import torch
+
+    d, beta = _problems()[0]
+    prob = procell_lib.Problem(*_args(d))
+    th = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
+    with procell_lib.Relaxed(prob, beta) as rp:
+        h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        jac = torch.empty((2,) + tuple(rp.shape), dtype=torch.float64, device="cuda")
+        rp.jacobian(th, h, jac)
+        full = np.concatenate([h.cpu().numpy()[None], jac.cpu().numpy()])
+        n_full = rp.expansions()
+        total, n_sum = np.zeros_like(full), 0
+        for phase in range(4):
+            rp.set_stride(4, phase)
+            rp.jacobian(th, h, jac)
+            n_sum += rp.expansions()
+            total += np.concatenate([h.cpu().numpy()[None], jac.cpu().numpy()])
+        rp.set_stride(1)
+    assert n_sum == n_full
+    for a, b in zip(total, full):
+        assert np.all(np.abs(a - b) <= 1e-12 * np.abs(b) + 1e-12 * np.abs(b).max())
+
+
 def test_batch_plan_equals_single_plans(procell_lib):
     import torch
 
