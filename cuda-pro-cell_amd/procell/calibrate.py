@@ -298,12 +298,22 @@ class BatchLM:
     otherwise) and the next trial.  Nothing returns to the host between
     iterations.  Keeps sd <= 0.8 mean."""
 
-    def __init__(self, plan, problems, targets, lam0=1e-3, device="cuda"):
+    def __init__(self, plan, problems, targets, lam0=1e-3, device="cuda", replicas=1):
         """plan: a RelaxedBatch over `problems` (their parameters are the
-        start); targets: per set a plain-method output, (value, count)."""
+        start), or over `replicas` copies of them in order (copy r of set i
+        is the plan's set r * len(problems) + i, same histogram and types,
+        another relaxed-stream seed): the copies' tallies are summed, which
+        for the normalised loss averages the estimator over seeds.
+        targets: per set a plain-method output, (value, count)."""
         self.plan = plan
         self.dev = torch.device(device)
         n = len(problems)
+        if len(plan.shapes) != n * replicas:
+            raise ValueError(f"the plan has {len(plan.shapes)} sets, expected {n} x {replicas}")
+        for r in range(1, replicas):
+            for i in range(n):
+                if plan.shapes[r * n + i] != plan.shapes[i]:
+                    raise ValueError(f"copy {r} of set {i} has another tally shape")
         T = max(s[2] for s in plan.shapes)
         binsets = []
         for prob, s in zip(problems, plan.shapes):
@@ -320,12 +330,13 @@ class BatchLM:
         for i, (prob, (bins, idx), s, tg) in enumerate(zip(problems, binsets, plan.shapes, targets)):
             tv, tc = (tg.value, tg.count) if hasattr(tg, "value") else tg
             R, K, Ti = s
-            base = plan.h_base[i]
             flat = idx.reshape(-1)
             rk = np.nonzero(flat >= 0)[0]
-            for j in range(Ti):
-                ent.append(base + rk * Ti + j)
-                dst.append((i * B + flat[rk]) * T + j)
+            for r in range(replicas):
+                base = plan.h_base[r * n + i]
+                for j in range(Ti):
+                    ent.append(base + rk * Ti + j)
+                    dst.append((i * B + flat[rk]) * T + j)
             t = target_histogram(bins, np.asarray(tv), np.asarray(tc))
             if t.sum() > 0:
                 q[i, :len(bins)] = t / t.sum()
@@ -336,8 +347,10 @@ class BatchLM:
             phi0[i, :Ti, 0] = np.log(np.where(pf, m, 1.0))
             phi0[i, :Ti, 1] = np.log(np.where(pf, sd, 1.0))
             phi0[i, :Ti, 2] = np.log(np.maximum(pr, 1e-12))
-            for c in range(3):
-                th_src.extend(((i * 3 + c) * T + j) for j in range(Ti))
+        for r in range(replicas):  # theta of the plan's sets, in plan order
+            for i, s in enumerate(plan.shapes[:n]):
+                for c in range(3):
+                    th_src.extend(((i * 3 + c) * T + j) for j in range(s[2]))
         d = self.dev
         self.ent = torch.tensor(np.concatenate(ent), dtype=torch.int64, device=d)
         self.dst = torch.tensor(np.concatenate(dst), dtype=torch.int64, device=d)
@@ -410,7 +423,7 @@ class BatchLM:
         """Per set [3, n_types] numpy (mean, sd, proportion; -1 for quiescent)."""
         mean, sd, p = (x.cpu().numpy() for x in self._params(self.phi))
         out = []
-        for i, s in enumerate(self.plan.shapes):
+        for i, s in enumerate(self.plan.shapes[:self.n]):
             Ti = s[2]
             out.append(np.stack([mean[i, :Ti], sd[i, :Ti], p[i, :Ti]]))
         return out

@@ -189,3 +189,34 @@ def test_levenberg_marquardt_edge_sets(procell_lib):
     assert L[-1][3] < L[0][3] and L[-1][0] < L[0][0]
     assert th[3][2].tolist() == [1.0]
     assert np.all(th[2][0] == -1.0) and abs(th[2][2].sum() - 1.0) < 1e-12
+
+
+def test_levenberg_marquardt_replicas_map_onto_their_sets(procell_lib):
+    """Two copies of every set with the same seed sum to twice the tally,
+    which the normalised loss does not see: the run equals the one-copy run
+    (to the rounding of the sums).  A mismatched plan is rejected."""
+    import torch
+    from procell import calibrate, workloads
+
+    gen, start = workloads.calibration_batch(6, 5, 1e4)
+    with procell_lib.Batch([procell_lib.Problem.from_workload(w, 0) for w in gen]) as b:
+        b.run([1000 + i for i in range(len(gen))])
+        targets = [b.result(i, want_raw=False) for i in range(len(gen))]
+    probs = [procell_lib.Problem.from_workload(w, 50 + i) for i, w in enumerate(start)]
+    runs = []
+    for K in (1, 2):
+        with procell_lib.RelaxedBatch(probs * K, 0.5) as plan:
+            lm = calibrate.BatchLM(plan, probs, targets, replicas=K)
+            lm.start()
+            losses = [lm.loss.clone()]
+            for _ in range(3):
+                lm.step(1)
+                losses.append(lm.loss.clone())
+            runs.append((torch.stack(losses).cpu().numpy(), lm.thetas()))
+        print("replicas", K, "loss", runs[-1][0][-1], flush=True)
+    np.testing.assert_allclose(runs[1][0], runs[0][0], rtol=1e-9)
+    for a, b in zip(runs[0][1], runs[1][1]):
+        np.testing.assert_allclose(a, b, rtol=1e-7, atol=1e-10)
+    with procell_lib.RelaxedBatch(probs, 0.5) as plan:
+        with pytest.raises(ValueError):
+            calibrate.BatchLM(plan, probs, targets, replicas=2)
