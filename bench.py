@@ -191,8 +191,10 @@ def relaxed_measurement(procell, workloads, np, torch, with_cpu=True, reps=50, f
     h = torch.empty(plan.shape, dtype=torch.float64, device="cuda")
     g = torch.randn(plan.shape, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(0))
     grad = torch.empty_like(th)
+    jac = torch.empty((2,) + tuple(plan.shape), dtype=torch.float64, device="cuda")
     times = {}
-    for name, fn in (("forward", lambda: plan.forward(th, h)), ("backward", lambda: plan.backward(th, g, grad))):
+    for name, fn in (("forward", lambda: plan.forward(th, h)), ("backward", lambda: plan.backward(th, g, grad)),
+                     ("jacobian", lambda: plan.jacobian(th, h, jac))):
         for _ in range(5):
             fn()
         torch.cuda.synchronize()
@@ -226,6 +228,7 @@ def relaxed_measurement(procell, workloads, np, torch, with_cpu=True, reps=50, f
     exp_pass = times["forward_expansions"]
     pass_ms = times["forward"] + times["backward"]
     log(f"relaxed: forward {times['forward']:.4f} ms, backward {times['backward']:.4f} ms, "
+        f"jacobian {times['jacobian']:.4f} ms, "
         f"{exp_pass} expansions per pass; fit {fit_steps} steps {fit_s:.2f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     out = {
         "workload": "hist256_1e6_types3_calibration: 256 log-spaced bins 1..1e4, lognormal Poisson counts "
@@ -235,6 +238,7 @@ def relaxed_measurement(procell, workloads, np, torch, with_cpu=True, reps=50, f
         "beta": beta,
         "forward_ms": times["forward"],
         "backward_ms": times["backward"],
+        "jacobian_ms": times["jacobian"],
         "expansions_per_pass": exp_pass,
         "expansions_per_s": 2 * exp_pass / (pass_ms * 1e-3),
         "roofline": {
