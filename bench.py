@@ -387,10 +387,15 @@ def lm_measurement(procell, calibrate, np, gen, probs, targets, iters, rec0, ada
     each iteration = one Jacobian pass (h, dh/dmean, dh/dsd) of every set's
     trial point + batched torch algebra on the device; the start point's
     pass is counted in the first iteration.  gpu_s = wall time around the
-    iterations with a device synchronisation after each."""
+    iterations with a device synchronisation after each, after one untimed
+    iteration of a throw-away instance (torch's first-call set-up of its
+    BLAS and index kernels)."""
     import torch
 
     plan = procell.RelaxedBatch(probs, 0.5)
+    warm = calibrate.BatchLM(plan, probs, targets)
+    warm.step(1)
+    del warm
     lm = calibrate.BatchLM(plan, probs, targets)
     torch.cuda.synchronize()
     traj, gpu = [{"steps": 0, "gpu_s": 0.0, **rec0}], 0.0
