@@ -142,3 +142,80 @@ def test_step_holds_parameters_the_loss_ignores():
     x = cal.lm_step(A, g, torch.full((2,), 1e-9, dtype=torch.float64), act)
     assert torch.isfinite(x).all()
     assert x[0, 0] < 0 and (x[0, 1:] == 0).all() and (x[1] == 0).all()
+
+
+class _StandInPlan:
+    """CPU stand-in for a batch relaxed plan: the sets' shapes and offsets
+    as the library lays them out, and a 'Jacobian pass' that writes known
+    values (h = p_j c_e, dh/dmean = c_e / 7, dh/dsd = -c_e / 11 for each
+    entry e of type j) and records the theta it was given."""
+
+    def __init__(self, problems, replicas=1):
+        self.shapes, self.h_base = [], []
+        base = 0
+        for _ in range(replicas):
+            for p in problems:
+                r, k, t = p.tally_shape()
+                self.shapes.append((int(r), int(k), int(t)))
+                self.h_base.append(base)
+                base += int(r) * int(k) * int(t)
+        self.h_total = base
+        self.c = torch.tensor(np.random.default_rng(2).uniform(0.5, 1.5, base), dtype=torch.float64)
+        self.thetas = []
+
+    def jacobian(self, theta, h, jac):
+        self.thetas.append(theta.clone())
+        k = 0
+        for (r, kd, t), b in zip(self.shapes, self.h_base):
+            th = theta[k:k + 3 * t].reshape(3, t)
+            c = self.c[b:b + r * kd * t].reshape(r, kd, t)
+            h[b:b + r * kd * t] = (th[2] * c).reshape(-1)
+            jac[0, b:b + r * kd * t] = (c / 7).reshape(-1)
+            jac[1, b:b + r * kd * t] = (-c / 11).reshape(-1)
+            k += 3 * t
+
+
+@pytest.mark.parametrize("replicas", [1, 2])
+def test_batch_lm_index_maps(replicas):
+    """BatchLM's entry -> (set, bin, type) and parameter -> theta maps, on
+    the CPU with the stand-in plan, against a loop over the output bins:
+    the start theta handed to the pass, the loss and the gradient J^T r."""
+    from procell import Problem, workloads
+
+    gen, start = workloads.calibration_batch(7, 3, 800)
+    probs = [Problem.from_workload(w, 10 + i) for i, w in enumerate(start)]
+    targets = [(w.values[w.counts > 0], w.counts[w.counts > 0]) for w in gen]
+    plan = _StandInPlan(probs, replicas)
+    lm = cal.BatchLM(plan, probs, targets, device="cpu", replicas=replicas)
+    lm.start()
+    th = plan.thetas[0].numpy()
+    want = np.concatenate([np.concatenate([w.mean, w.sd, w.proportion]) for w in start] * replicas)
+    np.testing.assert_allclose(th, want, rtol=1e-12)
+    c = plan.c.numpy()
+    for i, (p, w, (tv, tc)) in enumerate(zip(probs, start, targets)):
+        r, kd, t = plan.shapes[i]
+        bins, idx = cal.output_bins(p, kd)
+        H = np.zeros(len(bins))
+        M = np.zeros((len(bins), t))
+        Sg = np.zeros((len(bins), t))
+        for rep in range(replicas):
+            b0 = plan.h_base[rep * len(probs) + i]
+            cc = c[b0:b0 + r * kd * t].reshape(r, kd, t)
+            for row in range(r):
+                for k in range(kd):
+                    if idx[row, k] >= 0:
+                        H[idx[row, k]] += (w.proportion * cc[row, k]).sum()
+                        M[idx[row, k]] += cc[row, k] / 7
+                        Sg[idx[row, k]] += -cc[row, k] / 11
+        q = cal.target_histogram(bins, tv, tc)
+        q = q / q.sum()
+        res = H / H.sum() - q
+        assert abs(float(lm.loss[i]) - (res * res).sum()) <= 1e-12 * (res * res).sum()
+        # d loss / d log mean_j = 2 sum_b r_b (M_bj mean_j - H_b/S sum_b' M_b'j mean_j) / S
+        S = H.sum()
+        pr = w.mean >= 0
+        for j in np.nonzero(pr)[0]:
+            dm = (M[:, j] * w.mean[j] - H / S * (M[:, j] * w.mean[j]).sum()) / S
+            ds = (Sg[:, j] * w.sd[j] - H / S * (Sg[:, j] * w.sd[j]).sum()) / S
+            np.testing.assert_allclose(float(lm.g[i, 3 * j]), (res * dm).sum(), rtol=1e-9, atol=1e-18)
+            np.testing.assert_allclose(float(lm.g[i, 3 * j + 1]), (res * ds).sum(), rtol=1e-9, atol=1e-18)
