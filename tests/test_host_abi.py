@@ -269,11 +269,22 @@ def test_relaxed_validation_and_no_device(procell_lib):
     assert L.procell_relaxed_create(None, 0.5, ctypes.byref(h)) == procell_lib.E_INVALID
     for null_call in (L.procell_relaxed_forward(None, None, None, None),
                       L.procell_relaxed_backward(None, None, None, None, None),
-                      L.procell_relaxed_shape(None, None, None, None)):
+                      L.procell_relaxed_shape(None, None, None, None),
+                      L.procell_relaxed_jacobian(None, None, None, None, None),
+                      L.procell_relaxed_set_shape(None, 0, None, None, None, None)):
         assert null_call == procell_lib.E_INVALID
+    # batch plans: no sets, no array, bad beta
+    assert L.procell_relaxed_create_batch(None, 1, 0.5, ctypes.byref(h)) == procell_lib.E_INVALID
+    assert L.procell_relaxed_create_batch(ctypes.byref(prob.params), 0, 0.5, ctypes.byref(h)) == procell_lib.E_INVALID
+    for beta in (0.0, float("nan")):
+        assert L.procell_relaxed_create_batch(ctypes.byref(prob.params), 1, beta, ctypes.byref(h)) == \
+            procell_lib.E_INVALID
     if not _has_gpu():
         with pytest.raises(procell_lib.ProcellError) as e:
             procell_lib.Relaxed(prob, 0.5)
+        assert e.value.code == procell_lib.E_NO_DEVICE
+        with pytest.raises(procell_lib.ProcellError) as e:
+            procell_lib.RelaxedBatch([prob, prob], 0.5)
         assert e.value.code == procell_lib.E_NO_DEVICE
 
 
