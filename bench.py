@@ -8,7 +8,10 @@ and per-type tracking.  Inputs are uploaded once before timing (resident in
 HBM); every step uses a different seed.  value = divisions summed over the
 timed steps / wall time between the two synchronisations.
 
-Prints one JSON line on stdout; everything else goes to stderr.
+Prints one compact JSON line on stdout (headline first: metric, value, unit,
+roofline, cpu_baseline, then one short summary object per further workload);
+the full record with every trajectory goes to the --detail file and progress
+lines to stderr.
 """
 from __future__ import annotations
 
@@ -57,15 +60,21 @@ def traffic_from_profile():
         return None
 
 
-def issue_frac_from_profile():
-    """VALU issue utilisation (SQ_INSTS_VALU / (CUs x cycles)) from the
-    committed rocprofv3 counter summary."""
+def counter_fracs_from_profile():
+    """VALU fractions from the committed rocprofv3 counter summary:
+    all VALU instructions / (CUs x cycles), and the INT32 + FP64 + CVT
+    instructions alone / (CUs x cycles)."""
+    out = {"valu_issue_all": None, "valu_frac_int32_f64_cvt": None}
     try:
         with open(COUNTER_FILE) as f:
-            m = re.search(r"VALU issue utilisation.*=\s*([0-9.]+)", f.read())
-        return float(m.group(1)) if m else None
+            text = f.read()
     except OSError:
-        return None
+        return out
+    for key, pat in (("valu_issue_all", r"VALU issue utilisation.*=\s*([0-9.]+)"),
+                     ("valu_frac_int32_f64_cvt", r"INT32 \+ F64 \+ CVT fraction.*=\s*([0-9.]+)")):
+        m = re.search(pat, text)
+        out[key] = float(m.group(1)) if m else None
+    return out
 
 
 def cpu_model():
@@ -79,19 +88,19 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(workload, budget_s: float):
+def cpu_baseline(workload, budget_s: float, min_runs: int = 5):
     """The CPU oracle (OpenMP, all host cores of this box) on the same
-    workload, seeds 0, 1, ... until about budget_s of wall time."""
+    workload: seeds 0, 1, ... until at least min_runs runs and about
+    budget_s of wall time; value = the median of the runs' rates."""
     from oracle import binding as ob
 
     threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
-    divs, seeds, t0 = 0, 0, time.perf_counter()
-    while True:
-        _, _, st = ob.simulate(ob.Problem.from_workload(workload, seeds), threads)
+    rates, divs, t0 = [], 0, time.perf_counter()
+    while len(rates) < min_runs or time.perf_counter() - t0 < budget_s:
+        t1 = time.perf_counter()
+        _, _, st = ob.simulate(ob.Problem.from_workload(workload, len(rates)), threads)
+        rates.append(st["divisions"] / (time.perf_counter() - t1))
         divs += st["divisions"]
-        seeds += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
     dt = time.perf_counter() - t0
     # one core on a 1/20 sample of the cells
     from procell import workloads
@@ -100,12 +109,14 @@ def cpu_baseline(workload, budget_s: float):
     t1 = time.perf_counter()
     _, _, st1 = ob.simulate(ob.Problem.from_workload(small, 0), 1)
     dt1 = time.perf_counter() - t1
+    rates.sort()
     return {
-        "value": divs / dt,
+        "value": rates[len(rates) // 2],
         "unit": "divisions/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"full workload, seeds 0..{seeds - 1} ({divs} divisions, {dt:.2f} s wall), OpenMP oracle",
+        "sample": f"median of {len(rates)} runs of the full workload (seeds 0..{len(rates) - 1}, {divs} divisions, "
+                  f"{dt:.2f} s wall), OpenMP oracle",
         "cpu": cpu_model(),
         "one_core_value": st1["divisions"] / dt1,
         "one_core_sample": f"5% of the cells, seed 0 ({st1['divisions']} divisions, {dt1:.2f} s)",
@@ -542,6 +553,82 @@ def strided_ratio(first, straj, nm_traj, name="strided"):
             f"gpu_time_ratio_{name}_at_common": (b / a) if (a and b) else None}
 
 
+def _sig(x, digits=5):
+    """Rounds every float of a JSON value to `digits` significant digits."""
+    if isinstance(x, float):
+        return float(f"{x:.{digits}g}") if x == x and abs(x) != float("inf") else None
+    if isinstance(x, dict):
+        return {k: _sig(v, digits) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_sig(v, digits) for v in x]
+    return x
+
+
+def _pick(d, keys):
+    return {k: d[k] for k in keys if d and k in d}
+
+
+RECOVERY_KEYS = ("mean_rel_median", "sd_rel_median", "proportion_abs_median", "mean_within_2pct",
+                 "sd_within_2pct", "proportion_within_0.02")
+HEADLINE_KEYS = ("metric", "value", "unit", "ms_per_step", "steps", "warmup", "n_gpus", "higher_is_better",
+                 "scaling", "vs_baseline", "dtype", "data")
+LINE_LIMIT = 4096  # bytes of the printed line (the driver keeps the tail of stdout)
+
+
+def compact_line(full: dict) -> dict:
+    """The printed line: the headline fields first, then the config,
+    roofline and CPU baseline, then one short summary per further workload
+    (every trajectory stays in the detail file)."""
+    out = _pick(full, HEADLINE_KEYS)
+    out["config"] = _pick(full.get("config"), ("workload", "cells", "bins", "types", "divisions_per_step",
+                                               "abi_end_to_end_ms", "workgroups", "knobs"))
+    out["roofline"] = _pick(full.get("roofline"), ("bound", "achieved", "peak", "unit", "frac", "traffic",
+                                                   "kernel_ms", "work_per_unit", "valu_issue_all",
+                                                   "valu_frac_int32_f64_cvt", "hbm_gbs"))
+    out["cpu_baseline"] = _pick(full.get("cpu_baseline"), ("value", "unit", "cores", "kind", "sample",
+                                                           "one_core_value"))
+    b = full.get("batched")
+    if b:
+        out["batched"] = _pick(b, ("value", "ms_per_batch_run", "divisions_per_run", "sequential_plans_value",
+                                   "autotuned"))
+    r = full.get("relaxed")
+    if r:
+        rs = _pick(r, ("forward_ms", "backward_ms", "jacobian_ms", "expansions_per_s", "grad_steps_per_s_kernels"))
+        rs.update(_pick(r.get("roofline"), ("forward_frac", "adjoint_frac")))
+        fit = r.get("fit") or {}
+        rs["fit_loss"] = [fit.get("loss_start"), fit.get("loss_end"), fit.get("plain_noise_floor")]
+        if r.get("cpu_baseline"):
+            rs["cpu_baseline"] = _pick(r["cpu_baseline"], ("value", "unit", "cores"))
+        out["relaxed"] = rs
+    c = full.get("calibration")
+    if c:
+        cs = _pick(c, ("ms_per_step", "grad_steps_per_s", "roofline_frac_nominal"))
+        if c.get("trajectory"):
+            cs["adam_final"] = _pick(c["trajectory"][-1], RECOVERY_KEYS + ("steps", "gpu_s"))
+        if c.get("strided"):
+            cs["strided"] = {"ms_per_step": c["strided"]["ms_per_step"],
+                             **_pick(c["strided"]["trajectory"][-1], ("mean_rel_median", "gpu_s"))}
+        lm = c.get("levenberg_marquardt")
+        if lm:
+            cs["lm"] = {**_pick(lm, ("ms_per_iteration", "walks_cut", "gpu_time_ratio_adam_over_lm_at_loss")),
+                        **_pick(lm["trajectory"][-1], ("mean_rel_median", "sd_rel_median", "gpu_s"))}
+        nm = c.get("derivative_free")
+        if nm:
+            cs["nelder_mead"] = {**_pick(nm, ("gpu_s", "gpu_time_ratio_at_common",
+                                              "gpu_time_ratio_strided_at_common",
+                                              "gpu_time_ratio_levenberg_marquardt_at_common")),
+                                 **_pick(nm["trajectory"][-1], ("mean_rel_median",))}
+        rich = c.get("rich_target")
+        if rich:
+            cs["rich_target"] = rich.get("summary", {})
+        if c.get("cpu_baseline"):
+            cs["cpu_baseline"] = _pick(c["cpu_baseline"], ("value", "unit", "cores"))
+        out["calibration"] = cs
+    if full.get("detail_file"):
+        out["detail_file"] = full["detail_file"]
+    return _sig(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -556,6 +643,8 @@ def main():
     ap.add_argument("--calib-steps", type=int, default=200, help="steps of the batched calibration")
     ap.add_argument("--nm-rounds", type=int, default=300, help="rounds of the Nelder-Mead baseline (0: skip)")
     ap.add_argument("--lm-iters", type=int, default=15, help="Levenberg-Marquardt iterations (0: skip)")
+    ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
+                    help="file for the full record (every trajectory); '' to skip")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -627,6 +716,7 @@ def main():
     log(f"timed {K} steps: {ev_ms:.2f} ms events, {wall * 1e3:.2f} ms wall, kernel {kernel_ms:.4f} ms avg, "
         f"{total_div} divisions, {value:.4g} div/s")
     traffic = traffic_from_profile()
+    fracs = counter_fracs_from_profile()
     out = {
         "metric": METRIC,
         "value": value,
@@ -649,9 +739,8 @@ def main():
             "types": int(len(w.proportion)),
             "divisions_per_step": div_per_run,
             "seeds": f"{seeds[0]}..{seeds[-1]}",
-            "workgroups": stats["workgroups"],
-            "block_size": stats["block_size"],
-            "lds_bytes": stats["lds_bytes"],
+            "knobs": {"workgroups": stats["workgroups"], "block_size": stats["block_size"],
+                      "lds_bytes": stats["lds_bytes"], "window_rows": stats["window_rows"]},
             "abi_end_to_end_ms": float(np.median(e2e)),
         },
         "roofline": {
@@ -666,7 +755,10 @@ def main():
             "work_per_unit": f"{LANE_OPS_PER_DIVISION} lane-ops per division",
             "peak_source": "MI355X FP64 vector peak 78.6 TFLOP/s / 2 = one wave64 VALU instruction "
                            "per CU per clock (256 CUs x 64 lanes x 2.4 GHz)",
-            "issue_frac_counters": issue_frac_from_profile(),
+            # from the committed counter summary: all VALU instructions, and the
+            # INT32 + FP64 + CVT ones alone, per CU and cycle
+            **fracs,
+            "hbm_gbs": traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None,
         },
     }
     if not args.no_batch:
@@ -682,7 +774,18 @@ def main():
         cb = cpu_baseline(w, args.cpu_budget)
         out["cpu_baseline"] = cb
         log(f"cpu baseline: {cb['value']:.4g} div/s on {cb['cores']} cores; 1 core {cb['one_core_value']:.4g}")
-    print(json.dumps(out), flush=True)
+    if args.detail:
+        try:
+            os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
+            with open(args.detail, "w") as f:
+                json.dump(out, f, indent=1)
+            out["detail_file"] = args.detail
+        except OSError as e:
+            log(f"detail file not written: {e}")
+    line = json.dumps(compact_line(out))
+    if len(line) > LINE_LIMIT:
+        log(f"warning: result line is {len(line)} bytes")
+    print(line, flush=True)
 
 
 if __name__ == "__main__":
