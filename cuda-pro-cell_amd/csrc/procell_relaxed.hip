@@ -122,6 +122,7 @@ struct RelaxArgs {
     const double *g;           // cotangent, layout of h (backward)
     double *grad;              // per set [3][n_types] gradient (backward)
     unsigned long long *expansions;  // [2] expansions of the call (roots drawn + nodes expanded), walks cut
+    unsigned long long *set_cuts;    // [n_sets] walks cut per set (NULL: not kept)
     // Cell stride: a pass walks only the initial cells c (set-local id) with
     // (c + phase) & stride_mask == 0; phase = (*step + phase) & stride_mask
     // when step is set (calibration: the step counter, the adjoint pass one
@@ -171,7 +172,8 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     extern __shared__ double lds[];
     const uint32_t lane = threadIdx.x;
     const uint32_t w = blockIdx.x;
-    const RelaxSet S = a.sets[a.wave_set[w]];
+    const uint32_t set_id = a.wave_set[w];
+    const RelaxSet S = a.sets[set_id];
     const uint32_t T = S.n_types;
     const uint32_t KT = S.kdim * T;
     uint64_t next = a.wave_unit[2 * w];
@@ -465,8 +467,11 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     }
     if (lane == 0) {
         atomicAdd(a.expansions, (unsigned long long)n_exp);
-        if (n_cut)
+        if (n_cut) {
             atomicAdd(a.expansions + 1, (unsigned long long)n_cut);
+            if (a.set_cuts)
+                atomicAdd(a.set_cuts + set_id, (unsigned long long)n_cut);
+        }
     }
     if (Mode == kForward) {
         if (in_lds)
@@ -805,7 +810,7 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     const size_t o_wr = take(wave_row.size() * 4), o_rs = take(row_start.size() * 4);
     const size_t o_ri = take(row_info.size() * 4 + 4), o_rt = take(row_set.size() * 4 + 4);
     const size_t o_pr = take(prolif.size() * 4 + 4), o_qu = take(quiet.size() * 4 + 4);
-    const size_t o_ex = take(16);
+    const size_t o_ex = take(16 + (size_t)n * 8);  // expansions, walks cut, walks cut per set
     if (hipMalloc(&c.d_block, off) != hipSuccess)
         return PROCELL_E_OUT_OF_MEMORY;
     char *base = (char *)c.d_block;
@@ -836,6 +841,7 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     a.prolif = (const uint32_t *)(base + o_pr);
     a.quiet = (const uint32_t *)(base + o_qu);
     a.expansions = (unsigned long long *)(base + o_ex);
+    a.set_cuts = a.expansions + 2;
     a.h_total = c.h_total;
     if (c.n_waves) {
         const size_t stack_bytes = (size_t)c.n_waves * 64 * kdim_max * kFrameDoubles * sizeof(double);
@@ -1117,7 +1123,7 @@ int relaxed_call(procell_relaxed *r, int mode, const double *theta, double *h, c
     a.g = g;
     a.grad = gr;
     a.jac = jac;
-    RELAX_TRY(launch_zero(a.expansions, 2, st));
+    RELAX_TRY(launch_zero(a.expansions, 2ull + a.n_sets, st));
     if (mode == kAdjoint) {
         RELAX_TRY(launch_zero((unsigned long long *)gr, 3ull * r->core.types_total, st));
     } else {
@@ -1260,6 +1266,16 @@ int procell_relaxed_expansions(procell_relaxed *r, uint64_t *n)
     v = v2[0];
     *n = v;
     return v2[1] ? PROCELL_E_DEPTH : PROCELL_OK;
+}
+
+int procell_relaxed_set_cuts(procell_relaxed *r, uint64_t *cuts, void *stream)
+{
+    if (!r || !cuts)
+        return PROCELL_E_INVALID;
+    RELAX_TRY(hipSetDevice(r->core.device));
+    RELAX_TRY(hipMemcpyAsync(cuts, r->core.args.set_cuts, (size_t)r->core.args.n_sets * 8, hipMemcpyDeviceToDevice,
+                             (hipStream_t)stream));
+    return PROCELL_OK;
 }
 
 int procell_relaxed_destroy(procell_relaxed *r)
@@ -1456,6 +1472,7 @@ int procell_calib_create(const procell_params *sets, uint32_t n_sets, double bet
     c->fwd.theta = ca.theta;
     c->fwd.h = d_h;
     c->fwd.expansions = d_exp;
+    c->fwd.set_cuts = nullptr;  // a calibration reports the step's total only
     c->bwd = c->fwd;
     // with a stride, the adjoint walks the next class of cells: the residual
     // (from the forward pass) and the Jacobian then come from independent

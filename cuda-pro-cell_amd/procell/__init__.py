@@ -102,7 +102,7 @@ EXPORTS = (
     "procell_read_types", "procell_write_histogram", "procell_free", "procell_debug_draws",
     "procell_batch_create", "procell_batch_run", "procell_batch_result", "procell_batch_autotune",
     "procell_relaxed_create", "procell_relaxed_shape", "procell_relaxed_forward", "procell_relaxed_backward",
-    "procell_relaxed_set_stride", "procell_relaxed_expansions", "procell_relaxed_destroy",
+    "procell_relaxed_set_stride", "procell_relaxed_expansions", "procell_relaxed_set_cuts", "procell_relaxed_destroy",
     "procell_relaxed_jacobian", "procell_relaxed_create_batch", "procell_relaxed_set_shape",
     "procell_calib_create", "procell_calib_step", "procell_calib_theta", "procell_calib_loss",
     "procell_calib_stream", "procell_calib_destroy",
@@ -159,6 +159,7 @@ def lib():
         L.procell_relaxed_backward.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p]
         L.procell_relaxed_expansions.argtypes = [ctypes.c_void_p, _u64p]
+        L.procell_relaxed_set_cuts.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.procell_relaxed_set_stride.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
         L.procell_relaxed_destroy.argtypes = [ctypes.c_void_p]
     if hasattr(L, "procell_relaxed_jacobian"):
@@ -447,6 +448,17 @@ class Relaxed:
         """Later passes walk only the initial cells c with (c + phase) a
         multiple of `stride` (a power of two); 1 walks every cell."""
         _check(lib().procell_relaxed_set_stride(self.handle, int(stride), int(phase)), "relaxed_set_stride")
+
+    def set_cuts(self, cuts, stream=None):
+        """Enqueues a copy of the last call's walks cut short, per set, into
+        `cuts` (an int64 CUDA tensor with one entry per set); no wait."""
+        import torch
+
+        n = len(getattr(self, "shapes", [None]))
+        if cuts.dtype != torch.int64 or not cuts.is_cuda or not cuts.is_contiguous() or cuts.numel() != n:
+            raise ValueError(f"cuts: a contiguous int64 CUDA tensor of {n} entries is required")
+        _check(lib().procell_relaxed_set_cuts(self.handle, ctypes.c_void_p(cuts.data_ptr()), self._stream(stream)),
+               "relaxed_set_cuts")
 
     def expansions(self) -> int:
         """Expansions of the last call (roots drawn plus nodes expanded); waits."""

@@ -296,7 +296,11 @@ class BatchLM:
     of the normalised-L2 loss in (log mean, log sd, proportion logits), the
     accept / reject of each set's trial (damping / 5 on success, x 4
     otherwise) and the next trial.  Nothing returns to the host between
-    iterations.  Keeps sd <= 0.8 mean."""
+    iterations.  Keeps sd <= 0.8 mean (the start point is projected onto
+    it too).  A trial whose pass cut a walk of its set short (over the
+    relaxed walks' expansion budget: h and the Jacobian are then incomplete)
+    is rejected on the device; thetas() raises PROCELL_E_DEPTH when a set's
+    current point itself came from a cut pass (only the start can)."""
 
     def __init__(self, plan, problems, targets, lam0=1e-3, device="cuda", replicas=1):
         """plan: a RelaxedBatch over `problems` (their parameters are the
@@ -364,10 +368,14 @@ class BatchLM:
         free_logit = real & (np.arange(T)[None, :] < last)
         act = np.stack([prolif, prolif, free_logit], axis=-1)
         self.active = torch.tensor(act, device=d)
-        self.phi = torch.tensor(phi0, dtype=torch.float64, device=d)
+        self.phi = self._project(torch.tensor(phi0, dtype=torch.float64, device=d))
         self.lam = torch.full((n,), float(lam0), dtype=torch.float64, device=d)
         self.h = torch.empty(plan.h_total, dtype=torch.float64, device=d)
         self.jac = torch.empty((2, plan.h_total), dtype=torch.float64, device=d)
+        self.replicas = replicas
+        self.set_cut = torch.zeros(len(plan.shapes), dtype=torch.int64, device=d)  # last pass, per plan set
+        self.cut = torch.zeros(n, dtype=torch.bool, device=d)       # current point from a cut pass
+        self.cut_trials = torch.zeros(n, dtype=torch.int64, device=d)  # trials rejected for cuts
         self.iterations = 0
         self.loss = self.A = self.g = None
 
@@ -375,7 +383,8 @@ class BatchLM:
         """Evaluates the starting point (one Jacobian pass); step() does it
         first when it has not been done."""
         if self.loss is None:
-            self.loss, self.A, self.g = self._evaluate(self.phi)
+            loss, self.A, self.g, self.cut = self._evaluate(self.phi)
+            self.loss = torch.where(self.cut, torch.full_like(loss, torch.inf), loss)
 
     def _params(self, phi):
         neg = torch.full_like(phi[..., 0], -1.0)
@@ -392,13 +401,16 @@ class BatchLM:
     def _evaluate(self, phi):
         mean, sd, p = self._params(phi)
         self.plan.jacobian(self.theta_flat(phi), self.h, self.jac)
+        self.plan.set_cuts(self.set_cut)
         n, B, T = self.n, self.B, self.T
+        cut = self.set_cut.reshape(self.replicas, n).sum(0) > 0
         vals = torch.stack([self.h.index_select(0, self.ent), self.jac[0].index_select(0, self.ent),
                             self.jac[1].index_select(0, self.ent)], -1)
         tally = torch.zeros((n * B * T, 3), dtype=torch.float64, device=self.dev).index_add_(0, self.dst, vals)
         tally = tally.reshape(n, B, T, 3)
-        return normal_equations(tally[..., 0], tally[..., 1], tally[..., 2], mean.clamp_min(0), sd.clamp_min(0), p,
-                                self.q, self.active)
+        loss, A, g = normal_equations(tally[..., 0], tally[..., 1], tally[..., 2], mean.clamp_min(0),
+                                      sd.clamp_min(0), p, self.q, self.active)
+        return loss, A, g, cut
 
     def _project(self, phi):
         ls = torch.minimum(phi[..., 1], phi[..., 0] + LOG_MAX_SD_RATIO)
@@ -410,8 +422,10 @@ class BatchLM:
         for _ in range(n_iter):
             dx = lm_step(self.A, self.g, self.lam, self.active).reshape(self.n, self.T, 3)
             trial = self._project(self.phi + dx)
-            loss, A, g = self._evaluate(trial)
-            ok = loss < self.loss
+            loss, A, g, cut = self._evaluate(trial)
+            ok = (loss < self.loss) & ~cut
+            self.cut_trials += cut.to(torch.int64)
+            self.cut = self.cut & ~ok
             self.phi = torch.where(ok[:, None, None], trial, self.phi)
             self.loss = torch.where(ok, loss, self.loss)
             self.A = torch.where(ok[:, None, None], A, self.A)
@@ -420,7 +434,14 @@ class BatchLM:
             self.iterations += 1
 
     def thetas(self):
-        """Per set [3, n_types] numpy (mean, sd, proportion; -1 for quiescent)."""
+        """Per set [3, n_types] numpy (mean, sd, proportion; -1 for quiescent).
+        Raises PROCELL_E_DEPTH when a set's current point came from a pass
+        that cut a walk short."""
+        if self.cut.any():
+            from procell import E_DEPTH, ProcellError
+
+            bad = torch.nonzero(self.cut).flatten().tolist()
+            raise ProcellError(E_DEPTH, f"BatchLM: walks cut short at the current point of sets {bad[:8]}")
         mean, sd, p = (x.cpu().numpy() for x in self._params(self.phi))
         out = []
         for i, s in enumerate(self.plan.shapes[:self.n]):

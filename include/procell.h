@@ -231,6 +231,12 @@ int procell_relaxed_set_shape(const procell_relaxed *plan, uint32_t set, uint64_
  * h or gradient is then incomplete (parameters outside the estimator's
  * domain, see theta above). */
 int procell_relaxed_expansions(procell_relaxed *plan, uint64_t *n);
+/* Walks cut short (over the 2^22-expansion budget) in the plan's last call,
+ * per set: enqueues on `stream` a copy of the plan's counters into the
+ * caller's device array cuts[n_sets] and returns without waiting, so a
+ * caller can reject a set's cut pass without a host round trip.  No
+ * reference counterpart. */
+int procell_relaxed_set_cuts(procell_relaxed *plan, uint64_t *cuts, void *stream);
 int procell_relaxed_destroy(procell_relaxed *plan);
 
 /* ---- batched calibration ----------------------------------------------

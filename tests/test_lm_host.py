@@ -162,6 +162,14 @@ class _StandInPlan:
         self.h_total = base
         self.c = torch.tensor(np.random.default_rng(2).uniform(0.5, 1.5, base), dtype=torch.float64)
         self.thetas = []
+        self.cut_sets = set()   # plan sets whose passes report a cut walk
+        self.cut_after = 0      # ... from this call on (0: every call)
+
+    def set_cuts(self, cuts):
+        cuts.zero_()
+        if len(self.thetas) > self.cut_after:
+            for i in self.cut_sets:
+                cuts[i] = 3
 
     def jacobian(self, theta, h, jac):
         self.thetas.append(theta.clone())
@@ -219,3 +227,67 @@ def test_batch_lm_index_maps(replicas):
             ds = (Sg[:, j] * w.sd[j] - H / S * (Sg[:, j] * w.sd[j]).sum()) / S
             np.testing.assert_allclose(float(lm.g[i, 3 * j]), (res * dm).sum(), rtol=1e-9, atol=1e-18)
             np.testing.assert_allclose(float(lm.g[i, 3 * j + 1]), (res * ds).sum(), rtol=1e-9, atol=1e-18)
+
+
+def _lm_instance(replicas=1, n=3):
+    from procell import Problem, workloads
+
+    gen, start = workloads.calibration_batch(7, n, 800)
+    probs = [Problem.from_workload(w, 10 + i) for i, w in enumerate(start)]
+    targets = [(w.values[w.counts > 0], w.counts[w.counts > 0]) for w in gen]
+    return probs, targets, _StandInPlan(probs, replicas)
+
+
+def test_batch_lm_rejects_trials_with_cut_walks():
+    """A set whose trial passes cut a walk short keeps its point (the trial
+    is rejected on the device and counted); the other sets move on."""
+    probs, targets, plan = _lm_instance(replicas=2)
+    plan.cut_sets = {len(probs) + 1}  # copy 1 of set 1
+    plan.cut_after = 1                # the start pass is whole
+    lm = cal.BatchLM(plan, probs, targets, device="cpu", replicas=2)
+    lm.start()
+    phi0 = lm.phi.clone()
+    lm.step(3)
+    assert torch.equal(lm.phi[1], phi0[1])
+    assert lm.cut_trials.tolist() == [0, 3, 0]
+    assert not torch.equal(lm.phi[0], phi0[0]) and not torch.equal(lm.phi[2], phi0[2])
+    lm.thetas()  # the current points come from whole passes
+
+
+def test_batch_lm_cut_start_raises_until_replaced():
+    """A start point whose pass was cut counts as infinitely bad: the first
+    whole trial replaces it; while none has, thetas() raises E_DEPTH."""
+    from procell import E_DEPTH, ProcellError
+
+    probs, targets, plan = _lm_instance()
+    plan.cut_sets = {2}
+    lm = cal.BatchLM(plan, probs, targets, device="cpu")
+    lm.start()
+    assert torch.isinf(lm.loss[2]) and bool(lm.cut[2])
+    with pytest.raises(ProcellError) as e:
+        lm.thetas()
+    assert e.value.code == E_DEPTH
+    lm.step(1)  # still cut: rejected, still raises
+    with pytest.raises(ProcellError):
+        lm.thetas()
+    plan.cut_sets = set()
+    lm.step(1)
+    assert not lm.cut.any() and torch.isfinite(lm.loss).all()
+    lm.thetas()
+
+
+def test_batch_lm_projects_the_start():
+    """A start with sd > 0.8 mean is handed to the pass at sd = 0.8 mean."""
+    probs, targets, plan = _lm_instance()
+    w = probs[0]
+    j = int(np.nonzero(np.asarray(w.mean) >= 0)[0][0])
+    sd = np.array(w.sd, dtype=float)
+    sd[j] = 2.0 * w.mean[j]
+    from procell import Problem
+
+    probs[0] = Problem(w.values, w.counts, w.proportion, w.mean, sd, w.params.t_max, w.params.phi, w.params.seed)
+    lm = cal.BatchLM(plan, probs, targets, device="cpu")
+    lm.start()
+    T = plan.shapes[0][2]
+    th = plan.thetas[0][:3 * T].reshape(3, T)
+    np.testing.assert_allclose(float(th[1, j]), 0.8 * w.mean[j], rtol=1e-12)
