@@ -216,10 +216,62 @@ def test_global_fallbacks_match_oracle(procell_lib, oracle):
         gt = torch.tensor(g, dtype=torch.float64, device="cuda")
         grad = torch.empty_like(th)
         rp.backward(th, gt, grad)
-        h, grad = h.cpu().numpy(), grad.cpu().numpy()
+        # the Jacobian pass records its three tallies with global atomics too
+        hj = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        jac = torch.empty((2,) + tuple(rp.shape), dtype=torch.float64, device="cuda")
+        rp.jacobian(th, hj, jac)
+        h, grad, hj, jac = h.cpu().numpy(), grad.cpu().numpy(), hj.cpu().numpy(), jac.cpu().numpy()
     assert np.all(np.abs(h - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
+    assert np.all(np.abs(hj - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max()))
     scale = max(1.0, np.abs(grad_ref).max())
     assert np.all(np.abs(grad - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale), (grad, grad_ref)
+    p = np.asarray(d["proportion"])
+    gj = np.stack([(g * jac[0]).sum((0, 1)), (g * jac[1]).sum((0, 1)), (g * hj).sum((0, 1)) / p])
+    assert np.all(np.abs(gj - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale), (gj, grad_ref)
+
+
+def test_global_fallbacks_of_a_later_batch_set(procell_lib, oracle):
+    """A batch plan whose second set takes the global fallbacks (rows and
+    tallies beyond the LDS stage and window, blocks_per_cu = 1): its h,
+    Jacobian and gradient sit at an offset into the concatenated arrays."""
+    import torch
+
+    small = _problems()[1][0]
+    n = 30000
+    values = [10.0 ** (5.0 * i / (n - 1)) for i in range(n)]
+    big = dict(values=values, counts=[1] * n, proportion=[0.6, 0.4], mean=[30.0, 45.0], sd=[8.0, 9.0],
+               t_max=100.0, phi=1.0, seed=22)
+    cases = [small, big]
+    beta = 0.5
+    probs = [procell_lib.Problem(*_args(d), blocks_per_cu=1) for d in cases]
+    rng = np.random.default_rng(12)
+    with procell_lib.RelaxedBatch(probs, beta) as bp:
+        th = torch.tensor(np.concatenate([np.stack([d["mean"], d["sd"], d["proportion"]]).reshape(-1)
+                                          for d in cases]), dtype=torch.float64, device="cuda")
+        g_np = rng.normal(size=bp.h_total)
+        h = torch.empty(bp.h_total, dtype=torch.float64, device="cuda")
+        jac = torch.empty((2, bp.h_total), dtype=torch.float64, device="cuda")
+        grad = torch.empty(3 * bp.types_total, dtype=torch.float64, device="cuda")
+        bp.jacobian(th, h, jac)
+        bp.backward(th, torch.tensor(g_np, dtype=torch.float64, device="cuda"), grad)
+        h, jac, grad = h.cpu().numpy(), jac.cpu().numpy(), grad.cpu().numpy()
+        shapes, h_base, t_base = bp.shapes, bp.h_base, bp.theta_base
+    for i, d in enumerate(cases):
+        op = oracle.Problem(*_args(d))
+        h_ref, _ = oracle.relaxed(op, beta)
+        sl = slice(h_base[i], h_base[i] + h_ref.size)
+        gi = g_np[sl].reshape(h_ref.shape)
+        _, grad_ref = oracle.relaxed(op, beta, gi)
+        hi = h[sl].reshape(h_ref.shape)
+        assert np.all(np.abs(hi - h_ref) <= H_RTOL * np.abs(h_ref) + H_ATOL * max(1.0, np.abs(h_ref).max())), i
+        T = shapes[i][2]
+        scale = max(1.0, np.abs(grad_ref).max())
+        gr = grad[t_base[i]:t_base[i] + 3 * T].reshape(3, T)
+        assert np.all(np.abs(gr - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale), (i, gr, grad_ref)
+        p = np.asarray(d["proportion"])
+        gj = np.stack([(gi * jac[0, sl].reshape(h_ref.shape)).sum((0, 1)),
+                       (gi * jac[1, sl].reshape(h_ref.shape)).sum((0, 1)), (gi * hi).sum((0, 1)) / p])
+        assert np.all(np.abs(gj - grad_ref) <= G_RTOL * np.abs(grad_ref) + G_ATOL * scale), (i, gj, grad_ref)
 
 
 def test_many_types_match_oracle(procell_lib, oracle):
