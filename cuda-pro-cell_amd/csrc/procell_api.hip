@@ -152,7 +152,12 @@ using namespace procell;
 // across runs.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSeedSlots = 4;  // pinned host staging slots for batch seeds
-constexpr double kBatchSegments = 8.0;  // batch segments per resident tree workgroup (measured: 1x 6.5 ms, 2x 4.4, 4x 3.3, 8x 3.1, 16x 3.4 on 512 sets)
+// Segments per resident workgroup unless tuned.  1e7-cell histogram, 3
+// workgroups of 256 per CU: 1 -> 0.321 ms per run, 2 -> 0.307, 4 -> 0.305,
+// 8 -> 0.338, 16 -> 0.401 (more segments: more set-up and window flushes).
+// A batch of many small sets balances better with finer segments.
+constexpr uint32_t kSegmentsPerWorkgroup = 4;
+constexpr uint32_t kBatchSegmentsPerWorkgroup = 8;
 
 struct procell_plan {
     int device = 0;
@@ -165,9 +170,8 @@ struct procell_plan {
     LaunchShape shape{};
     RunArgs args{};
     void *d_block = nullptr;  // one allocation holding every table
-    void *d_queue = nullptr;  // queue of dividing initial cells (16 B per initial cell)
     size_t tally_total = 0;
-    size_t zero_words = 0;    // counters, tally and segment cursors, cleared per run
+    size_t zero_words = 0;    // counters and tally, cleared per run
     uint64_t *d_seeds = nullptr;       // [n_sets] seeds of the current batch run (in d_block)
     uint64_t *h_seeds = nullptr;       // pinned [kSeedSlots][n_sets]
     hipEvent_t seed_ev[kSeedSlots] = {};
@@ -235,23 +239,21 @@ void destroy_plan(procell_plan *pl)
         (void)hipHostFree(pl->h_seeds);
     if (pl->d_block)
         (void)hipFree(pl->d_block);
-    if (pl->d_queue)
-        (void)hipFree(pl->d_queue);
     if (pl->own_stream && pl->stream)
         (void)hipStreamDestroy(pl->stream);
     delete pl;
 }
 
-// Expected tree-kernel work of one initial cell of a row, in units of one
-// division: the expected number of divisions, with division k+1 completing
-// at D = tau0 (1 + U) + tau_1 + ... + tau_k approximated as normal
-// (truncation ignored), plus a small constant per cell.  The root kernel's
-// work is balanced separately (equal chunks).  Only used to balance the
-// tree kernel's segments.
+// Lane-steps of one initial cell of a row: its root step (one lane of a
+// full-width step, about what a division costs) plus the expected number of
+// divisions, with division k+1 completing at D = tau0 (1 + U) + tau_1 + ... +
+// tau_k approximated as normal (truncation ignored).  Only used to balance
+// the workgroups' segments; claims and stealing even out its errors.
+constexpr double kRootCost = 1.0;
 double row_cost(const Setup &s, uint32_t row)
 {
     const int kcap = (int)(s.row_info[row] & 0xFFu);
-    double w = 0.02;
+    double w = kRootCost;
     double p_prev = 0.0;
     for (uint32_t j = 0; j < s.n_types; ++j) {
         const double m = s.type_mean[j];
@@ -311,7 +313,7 @@ void split_segments(const Setup &s, uint32_t g, std::vector<uint64_t> &begin, st
     }
 }
 
-// Expected tree-kernel work of a set (sum of row costs over its cells).
+// Expected work of a set (sum of row costs over its cells).
 double set_cost(const Setup &s)
 {
     double w = 0.0;
@@ -405,9 +407,9 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     int bs = tu.block_size ? (int)tu.block_size : 256;
     int occ = 0;
     for (;;) {
-        const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
+        const size_t base = sim_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
         int occ_v = 0;
-        if (tree_occupancy(bs, base, lds_rows != 0, kdim, &occ_v) != hipSuccess)
+        if (sim_occupancy(bs, base, lds_rows != 0, kdim, &occ_v) != hipSuccess)
             return fail(PROCELL_E_HIP);
         if (occ_v >= 1) {
             occ = occ_v;
@@ -418,25 +420,28 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
         bs -= 64;  // deep trees: a smaller workgroup keeps the stack within LDS
     }
     // by default at most 12 resident wavefronts per CU (3 per SIMD): more
-    // shrink every workgroup's tally window and add claim traffic without
-    // adding issue rate (measured on the 1e7-cell histogram workload)
+    // shrink every workgroup's tally window without adding issue rate
+    // (measured on the 1e7-cell histogram workload)
     occ = std::min<int>(occ, tu.blocks_per_cu ? (int)tu.blocks_per_cu : std::max(1, 12 * 64 / bs));
-    ls.tree_block_size = bs;
-    const size_t base = tree_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
+    ls.block_size = bs;
+    const size_t base = sim_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
     const size_t budget = std::min(lds_per_cu / (size_t)occ, lds_per_block);
     const size_t row_bytes = (size_t)kdim * T * 4;
     uint32_t win_rows = budget > base + 16 ? (uint32_t)((budget - base - 16) / row_bytes) : 0;
 
-    // ---- segments: one per tree-kernel workgroup, each within one set ----
-    // A single set gets one segment per resident workgroup.  In a batch each
-    // set gets segments in proportion to its expected work (at least one),
-    // kBatchSegments per resident workgroup in total: stealing stays within
-    // a set, so finer segments let the workgroup dispatcher balance the
-    // sets (workgroups beyond the resident ones start as others finish).
-    const uint32_t resident = (uint32_t)std::min<int64_t>((int64_t)n_cu * occ, kMaxSegments);
+    // ---- segments: one per workgroup, each within one set ----
+    // `per_wg` segments per resident workgroup in total, of about equal
+    // expected work: workgroups beyond the resident ones start as others
+    // finish, so the dispatcher balances the segments over the CUs.  In a
+    // batch each set gets segments in proportion to its expected work (at
+    // least one).
+    const uint32_t resident = (uint32_t)((int64_t)n_cu * occ);
+    const uint32_t per_wg = tu.segments ? std::min<uint32_t>(tu.segments, 64)
+                                        : (n_sets == 1 ? kSegmentsPerWorkgroup : kBatchSegmentsPerWorkgroup);
     std::vector<uint32_t> g_set(n_sets, 1);
     if (n_sets == 1) {
-        g_set[0] = resident;
+        g_set[0] = (uint32_t)std::min<uint64_t>((uint64_t)resident * per_wg,
+                                                std::max<uint64_t>(1, (pl->setups[0].n_cells + 63) / 64));
     } else {
         std::vector<double> w(n_sets);
         double wt = 0.0;
@@ -444,12 +449,12 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
             w[i] = set_cost(pl->setups[i]);
             wt += w[i];
         }
-        const double target = kBatchSegments * (double)resident;
+        const double target = (double)per_wg * (double)resident;
         for (uint32_t i = 0; i < n_sets; ++i)
             g_set[i] = wt > 0.0 ? (uint32_t)std::max(1.0, std::floor(w[i] / wt * target + 0.5)) : 1u;
     }
     std::vector<uint64_t> seg_begin(1, 0);
-    std::vector<uint32_t> seg_row, seg_set;
+    std::vector<uint32_t> seg_row, seg_row_end, seg_set;
     for (uint32_t i = 0; i < n_sets; ++i) {
         const Setup &s = pl->setups[i];
         std::vector<uint64_t> b;
@@ -459,6 +464,11 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
         for (uint32_t k = 0; k < g_set[i]; ++k) {
             seg_begin.push_back(info[i].cell_base + b[k + 1]);
             seg_row.push_back(info[i].row_base + r[k]);
+            // one past the row of the segment's last cell (an empty segment: its first row + 1)
+            uint32_t r_end = r[k] + 1;
+            if (b[k + 1] > b[k])
+                r_end = (uint32_t)(std::upper_bound(s.start.begin(), s.start.end() - 1, b[k + 1] - 1) - s.start.begin());
+            seg_row_end.push_back(info[i].row_base + r_end);
             seg_set.push_back(i);
         }
         info[i].seg_hi = (uint32_t)seg_row.size();
@@ -492,53 +502,26 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     }
     // the window must not cost a resident workgroup (LDS allocation granularity)
     for (;;) {
-        ls.tree_lds_bytes = tree_lds_bytes(win_rows, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
+        ls.lds_bytes = sim_lds_bytes(win_rows, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
         int occ_w = 0;
-        if (tree_occupancy(bs, ls.tree_lds_bytes, lds_rows != 0, kdim, &occ_w) != hipSuccess)
+        if (sim_occupancy(bs, ls.lds_bytes, lds_rows != 0, kdim, &occ_w) != hipSuccess)
             return fail(PROCELL_E_HIP);
         // keep a margin for the per-workgroup LDS allocation granularity
-        const bool fits = (size_t)occ * align_up(ls.tree_lds_bytes, 2048) <= lds_per_cu;
+        const bool fits = (size_t)occ * align_up(ls.lds_bytes, 2048) <= lds_per_cu;
         if ((occ_w >= occ && fits) || win_rows == 0)
             break;
         win_rows = win_rows > 8 ? win_rows - win_rows / 8 : 0;
     }
-    ls.tree_blocks = (int)g;
-    ls.root_block_size = 256;
-    ls.root_lds_bytes = root_lds_bytes(T, 256, lds_rows);
+    ls.blocks = (int)g;
     a.win_rows = win_rows;
-    // queue entries per claim: a share of what the segment has left, at
-    // most `claim` (a few prefetch sets) and at least half a set
-    a.claim = tu.chunk ? std::min<uint32_t>(tu.chunk, 1u << 20) : 1024;
-    a.claim_min = std::min<uint32_t>(a.claim, 32);
-    a.claim_steal = std::min<uint32_t>(a.claim, 64);
     a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 16;
 
-    // ---- one device allocation: counters | tally | segment cursors | tables ----
+    // ---- one device allocation: counters | tally | tables ----
     size_t off = 0;
     const size_t o_cnt = off; off += (size_t)n_sets * kCounterCopies * kNumCounters * 8;
     const size_t o_tally = off; off += pl->tally_total * 8;
-    off = align_up(off, 256);
-    const size_t o_cur = off; off += (size_t)g * kCursorWords * 8;
     pl->zero_words = off / 8;
     off = align_up(off, 256);
-    // root-kernel units: kUnit cells each, none spanning two segments
-    std::vector<uint32_t> unit_first(g + 1, 0), unit_seg;
-    for (uint32_t i = 0; i < g; ++i) {
-        const uint64_t len = seg_begin[i + 1] - seg_begin[i];
-        const uint64_t nc = (len + kUnit - 1) / kUnit;
-        unit_first[i + 1] = unit_first[i] + (uint32_t)nc;
-        unit_seg.insert(unit_seg.end(), (size_t)nc, i);
-    }
-    const size_t o_useg = off; off = align_up(off + unit_seg.size() * 4 + 4, 256);
-    const size_t o_ufirst = off; off = align_up(off + (size_t)(g + 1) * 4, 256);
-    a.n_units = (uint32_t)unit_seg.size();
-    {
-        int occ_r = 0;
-        if (root_occupancy(256, ls.root_lds_bytes, lds_rows != 0, &occ_r) != hipSuccess)
-            return fail(PROCELL_E_HIP);
-        const uint64_t res_r = (uint64_t)n_cu * (uint64_t)std::max(occ_r, 1);
-        ls.root_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>((a.n_units + 3) / 4, res_r));
-    }
     const size_t o_rs = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
     const size_t o_ri = off; off = align_up(off + (size_t)(n_rows + 1) * 4, 256);
     const size_t o_cum = off; off = align_up(off + (size_t)n_types_tot * 8, 256);
@@ -547,6 +530,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     const size_t o_tf = off; off = align_up(off + (size_t)n_types_tot * 4, 256);
     const size_t o_sb = off; off = align_up(off + (size_t)(g + 1) * 8, 256);
     const size_t o_sr = off; off = align_up(off + (size_t)g * 4, 256);
+    const size_t o_sre = off; off = align_up(off + (size_t)g * 4, 256);
     const size_t o_ss = off; off = align_up(off + (size_t)g * 4, 256);
     const size_t o_info = off; off = align_up(off + (size_t)n_sets * sizeof(SetInfo), 256);
     const size_t o_seed = off; off = align_up(off + (size_t)n_sets * 8, 256);
@@ -559,8 +543,6 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     }
     if (hipMalloc(&pl->d_block, off) != hipSuccess)
         return fail(PROCELL_E_OUT_OF_MEMORY);
-    if (n_cells && hipMalloc(&pl->d_queue, (size_t)n_cells * sizeof(QueueEntry)) != hipSuccess)
-        return fail(PROCELL_E_OUT_OF_MEMORY);
     char *dbase = (char *)pl->d_block;
     a.n_sets = n_sets;
     a.n_cells = n_cells;
@@ -569,10 +551,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     a.kdim = kdim;
     a.counters = (unsigned long long *)(dbase + o_cnt);
     a.tally = (unsigned long long *)(dbase + o_tally);
-    a.seg_cur = (unsigned long long *)(dbase + o_cur);
-    a.unit_seg = (const uint32_t *)(dbase + o_useg);
-    a.unit_first = (const uint32_t *)(dbase + o_ufirst);
-    a.queue = (QueueEntry *)pl->d_queue;
+
     a.row_start = (const uint32_t *)(dbase + o_rs);
     a.row_info = (const uint32_t *)(dbase + o_ri);
     a.type_cum = (const double *)(dbase + o_cum);
@@ -581,6 +560,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     a.type_file = (const uint32_t *)(dbase + o_tf);
     a.seg_begin = (const uint64_t *)(dbase + o_sb);
     a.seg_row = (const uint32_t *)(dbase + o_sr);
+    a.seg_row_end = (const uint32_t *)(dbase + o_sre);
     a.seg_set = (const uint32_t *)(dbase + o_ss);
     a.sets = (const SetInfo *)(dbase + o_info);
     a.set_seed = (const uint64_t *)(dbase + o_seed);
@@ -613,8 +593,8 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
         !up(o_cum, cum.data(), cum.size() * 8) || !up(o_mean, mean.data(), mean.size() * 8) ||
         !up(o_sd, sd.data(), sd.size() * 8) || !up(o_tf, tfile.data(), tfile.size() * 4) ||
         !up(o_sb, seg_begin.data(), seg_begin.size() * 8) || !up(o_sr, seg_row.data(), seg_row.size() * 4) ||
-        !up(o_ss, seg_set.data(), seg_set.size() * 4) || !up(o_info, info.data(), info.size() * sizeof(SetInfo)) ||
-        !up(o_useg, unit_seg.data(), unit_seg.size() * 4) || !up(o_ufirst, unit_first.data(), unit_first.size() * 4))
+        !up(o_sre, seg_row_end.data(), seg_row_end.size() * 4) ||
+        !up(o_ss, seg_set.data(), seg_set.size() * 4) || !up(o_info, info.data(), info.size() * sizeof(SetInfo)))
         return fail(PROCELL_E_HIP);
     if (hipStreamSynchronize(pl->stream) != hipSuccess)
         return fail(PROCELL_E_HIP);
@@ -635,14 +615,12 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     return PROCELL_OK;
 }
 
-// The run's kernels: clearing, roots, trees.
+// The run's kernels: clearing, then the simulation.
 hipError_t launch_run(procell_plan *pl)
 {
     hipError_t e = launch_zero(pl->args.counters, pl->zero_words, pl->stream);
     if (e == hipSuccess)
-        e = launch_root(pl->args, pl->shape, pl->stream);
-    if (e == hipSuccess)
-        e = launch_tree(pl->args, pl->shape, pl->stream);
+        e = launch_sim(pl->args, pl->shape, pl->stream);
     return e;
 }
 
@@ -762,9 +740,9 @@ int plan_result(procell_plan *pl, uint32_t set, procell_result *r)
     st.rejected = cnt[kRejected];
     st.cells = cnt[kCells];
     st.pool = cnt[kPoolCount];
-    st.workgroups = (uint32_t)pl->shape.tree_blocks;
-    st.block_size = (uint32_t)pl->shape.tree_block_size;
-    st.lds_bytes = (uint32_t)pl->shape.tree_lds_bytes;
+    st.workgroups = (uint32_t)pl->shape.blocks;
+    st.block_size = (uint32_t)pl->shape.block_size;
+    st.lds_bytes = (uint32_t)pl->shape.lds_bytes;
     st.window_rows = pl->args.win_rows;
     if (pl->timing && pl->timed) {
         float ms = 0.f;
@@ -918,7 +896,6 @@ int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t r
     if (!plan || !seeds || !chosen || reps == 0)
         return PROCELL_E_INVALID;
     HIP_TRY(hipSetDevice(plan->device));
-    static const uint32_t kClaims[] = {256, 512, 1024, 2048};
     static const uint32_t kIdle[] = {8, 16, 24};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
@@ -929,39 +906,31 @@ int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t r
     RunArgs &a = plan->args;
     const RunArgs keep = a;
     float best = -1.f;
-    uint32_t best_claim = a.claim, best_idle = a.idle_limit;
+    uint32_t best_idle = a.idle_limit;
     int rc = PROCELL_OK;
     const bool timing = plan->timing;
     plan->timing = false;
     drop_graph(plan);
-    for (uint32_t c : kClaims) {
-        for (uint32_t idle : kIdle) {
-            a.claim = c;
-            a.claim_min = std::min<uint32_t>(c, 32);
-            a.claim_steal = std::min<uint32_t>(c, 64);
-            a.idle_limit = idle;
-            rc = run_plan(plan, seeds, 0, false);  // warm-up
-            for (uint32_t i = 0; rc == PROCELL_OK && i < reps; ++i) {
-                if (i == 0 && hipEventRecord(e0, plan->stream) != hipSuccess)
-                    rc = PROCELL_E_HIP;
-                if (rc == PROCELL_OK)
-                    rc = run_plan(plan, seeds, 0, false);
-            }
-            float ms = 0.f;
-            if (rc == PROCELL_OK &&
-                (hipEventRecord(e1, plan->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    for (uint32_t idle : kIdle) {
+        a.idle_limit = idle;
+        rc = run_plan(plan, seeds, 0, false);  // warm-up
+        for (uint32_t i = 0; rc == PROCELL_OK && i < reps; ++i) {
+            if (i == 0 && hipEventRecord(e0, plan->stream) != hipSuccess)
                 rc = PROCELL_E_HIP;
-            if (rc != PROCELL_OK)
-                break;
-            if (best < 0.f || ms < best) {
-                best = ms;
-                best_claim = c;
-                best_idle = idle;
-            }
+            if (rc == PROCELL_OK)
+                rc = run_plan(plan, seeds, 0, false);
         }
+        float ms = 0.f;
+        if (rc == PROCELL_OK &&
+            (hipEventRecord(e1, plan->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = PROCELL_E_HIP;
         if (rc != PROCELL_OK)
             break;
+        if (best < 0.f || ms < best) {
+            best = ms;
+            best_idle = idle;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -969,14 +938,10 @@ int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t r
     a = keep;
     if (rc != PROCELL_OK)
         return rc;
-    a.claim = best_claim;
-    a.claim_min = std::min<uint32_t>(best_claim, 32);
-    a.claim_steal = std::min<uint32_t>(best_claim, 64);
     a.idle_limit = best_idle;
     plan->runs = 0;
     memset(chosen, 0, sizeof *chosen);
-    chosen->block_size = (uint32_t)plan->shape.tree_block_size;
-    chosen->chunk = best_claim;
+    chosen->block_size = (uint32_t)plan->shape.block_size;
     chosen->idle_lanes = best_idle;
     return PROCELL_OK;
 }

@@ -17,20 +17,6 @@ constexpr uint64_t kMaxCells = 0xFFFFFFFFull;
 constexpr uint64_t kMaxRows = 1ull << 24;
 constexpr uint32_t kMaxTypes = 256;
 constexpr int kMaxGeneration = 62;  // heap index of a node must fit 64 bits
-constexpr uint32_t kUnit = 256;     // initial cells per root-kernel work unit (one wavefront's)
-constexpr uint32_t kMaxSegments = 2048;  // tree-kernel workgroups (= segments) at most
-
-// Rows of a root-kernel wavefront's generation-0 tally window (LDS, at most
-// 2 KB; a unit of kUnit cells mostly lies in one or two rows), and the
-// window capacity that covers every type count up to n_types.
-__host__ __device__ constexpr uint32_t root_win_rows(uint32_t n_types)
-{
-    return n_types <= 64 ? 8u : 512u / n_types;
-}
-__host__ __device__ constexpr uint32_t root_win_cap(uint32_t max_types)
-{
-    return max_types <= 64 ? 8u * max_types : 512u;
-}
 
 // Per-parameter-set values of a plan: one set for a plain run, many for a
 // batch.  Rows, initial cells, type-table entries, tally entries and
@@ -43,11 +29,12 @@ struct SetInfo {
     uint32_t n_types;
     uint32_t type_base;   // first entry of the set in the type tables
     uint32_t row_base;    // first nonzero row of the set
-    uint32_t seg_lo, seg_hi;  // the set's segments (tree-kernel workgroups)
+    uint32_t seg_lo, seg_hi;  // the set's segments (workgroups)
     uint32_t pad[2];
 };
 
-// A dividing initial cell, written by the root kernel for the tree kernel.
+// A dividing initial cell, buffered in LDS between a root step and the
+// hand-out to an idle lane.
 struct QueueEntry {
     double s0;          // completion time of its first division
     uint32_t cell;      // initial cell id
@@ -67,11 +54,6 @@ enum Counter : int {
 };
 // Copies of the counters block: workgroup b adds into copy b % kCounterCopies,
 // so the device atomics on any one counter stay few; the host sums the copies.
-// Words per segment cursor block.  Claims are device-scope atomics on these
-// cursors; a block of its own per segment spreads them over many cache lines
-// and memory channels instead of queueing them on a few.
-constexpr uint32_t kCursorWords = 32;
-
 constexpr uint32_t kCounterCopies = 64;
 
 // Problem after validation: nonzero rows in file order, types sorted by
@@ -102,12 +84,8 @@ struct RunArgs {
     uint32_t kdim;          // most generations + 1 of any set
     uint32_t stack_levels;  // per-lane LDS stack depth (levels 1 .. kdim-1)
     uint32_t win_rows;      // rows of the LDS tally window (0: none)
-    uint32_t claim;         // largest claim of queue entries
-    uint32_t claim_min;     // smallest claim (guided claims shrink down to it)
-    uint32_t claim_steal;   // largest claim from another workgroup's segment
     uint32_t idle_limit;    // idle lanes tolerated before a wavefront refills
     uint32_t rows_in_lds;   // nonzero: the row tables are copied to LDS
-    uint32_t n_units;       // root-kernel work units
     const SetInfo *sets;         // [n_sets]
     const uint64_t *set_seed;    // [n_sets] seeds of a batch run (n_sets > 1)
     const uint32_t *seg_set;     // [segments] set of each segment
@@ -119,35 +97,24 @@ struct RunArgs {
     const uint32_t *type_file;   // sorted position -> file index
     const uint64_t *seg_begin;   // [segments + 1] first initial cell of each workgroup's segment
     const uint32_t *seg_row;     // [segments] row of each segment's first cell (window base)
-    unsigned long long *seg_cur;   // [segments][kCursorWords]: word 0 queue entries claimed so
-                                   // far, word 1 queue entries of the segment
-    const uint32_t *unit_seg;    // [units] segment of each root-kernel unit
-    const uint32_t *unit_first;  // [segments + 1] first unit of each segment
-    QueueEntry *queue;           // [n_cells] dividing initial cells; segment g's entries
-                                 // start at seg_begin[g]
+    const uint32_t *seg_row_end; // [segments] one past the row of each segment's last cell
     unsigned long long *tally;   // per set [rows][kdim][n_types] of the set, file type order
     unsigned long long *counters;  // [n_sets][kCounterCopies][kNumCounters]
 };
 
 struct LaunchShape {
-    int tree_blocks;        // = segments
-    int root_blocks;        // persistent root workgroups (<= chunks)
-    int tree_block_size;
-    size_t tree_lds_bytes;
-    int root_block_size;
-    size_t root_lds_bytes;
+    int blocks;             // = segments
+    int block_size;
+    size_t lds_bytes;
 };
 
-size_t tree_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32_t stack_levels,
-                      uint32_t block_size, uint32_t lds_rows);
-size_t root_lds_bytes(uint32_t n_types, uint32_t block_size, uint32_t lds_rows);
+size_t sim_lds_bytes(uint32_t win_rows, uint32_t kdim, uint32_t n_types, uint32_t stack_levels, uint32_t block_size,
+                     uint32_t lds_rows);
 hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t st);
-hipError_t launch_root(const RunArgs &a, const LaunchShape &ls, hipStream_t st);
-hipError_t launch_tree(const RunArgs &a, const LaunchShape &ls, hipStream_t st);
+hipError_t launch_sim(const RunArgs &a, const LaunchShape &ls, hipStream_t st);
 hipError_t launch_debug_draws(uint64_t seed, const uint64_t *cell, const uint64_t *node,
                               const uint32_t *attempt, uint64_t n, uint32_t *words, double *z,
                               hipStream_t st);
-hipError_t tree_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, uint32_t kdim, int *blocks_per_cu);
-hipError_t root_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, int *blocks_per_cu);
+hipError_t sim_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, uint32_t kdim, int *blocks_per_cu);
 
 }  // namespace procell

@@ -123,19 +123,23 @@ struct Tables {
     const double (*sincos)[2];  // [256] {sin, cos}(2 pi j/256)
 };
 
-// Natural log for positive normal x.
-__device__ __forceinline__ double log_pos(double x, const Coeffs &K, const Tables &tb)
+// Natural log for positive normal x, in two parts: the table index, then
+// the arithmetic given that entry (so a caller can issue the table read
+// early and overlap its latency with other work).
+__device__ __forceinline__ uint32_t log_index(double x)
+{
+    return (uint32_t)(__builtin_bit_cast(uint64_t, x) >> 45) & 127u;
+}
+
+__device__ __forceinline__ double log_with_entry(double x, uint32_t j, double invc, double logc, const Coeffs &K)
 {
     const uint64_t bits = __builtin_bit_cast(uint64_t, x);
     int expo = (int)(bits >> 52) - 1023;
-    const uint32_t j = (uint32_t)(bits >> 45) & 127u;
     double m = __builtin_bit_cast(double, (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
     if (j >= 64u) {
         m = m * 0.5;
         expo = expo + 1;
     }
-    const double invc = tb.log[j][0];
-    const double logc = tb.log[j][1];
     const double r = fma(m, invc, -1.0);
     const double r2 = r * r;
     double p = K.lp[5];
@@ -147,6 +151,12 @@ __device__ __forceinline__ double log_pos(double x, const Coeffs &K, const Table
     const double lo = de * K.ln2_lo;
     const double t = fma(r2, p, lo);
     return hi + (r + t);
+}
+
+__device__ __forceinline__ double log_pos(double x, const Coeffs &K, const Tables &tb)
+{
+    const uint32_t j = log_index(x);
+    return log_with_entry(x, j, tb.log[j][0], tb.log[j][1], K);
 }
 
 // Square root for finite a >= 0.
@@ -165,12 +175,17 @@ __device__ __forceinline__ double sqrt_nonneg(double a)
     return a > 0.0 ? out : 0.0;
 }
 
-// sin and cos of 2 pi u with u = uniform_co(hi, lo).
-__device__ __forceinline__ void sincos_turns(uint32_t hi, uint32_t lo, double &sn_out, double &cs_out,
-                                             const Coeffs &K, const Tables &tb)
+// sin and cos of 2 pi u with u = uniform_co(hi, lo): the table index (the
+// nearest 1/256 turn), then the arithmetic given that entry.
+__device__ __forceinline__ uint32_t sincos_index(uint32_t hi)
+{
+    return ((hi >> 23) + 1u) >> 1;  // 0..256
+}
+
+__device__ __forceinline__ void sincos_with_entry(uint32_t hi, uint32_t lo, uint32_t j, double sa, double ca,
+                                                  double &sn_out, double &cs_out, const Coeffs &K)
 {
     const double u = uniform_co(hi, lo);
-    const uint32_t j = ((hi >> 23) + 1u) >> 1;     // nearest 1/256 turn, 0..256
     const double d = u - (double)j * 0x1p-8;       // exact, |d| <= 1/512
     const double t = fma(d, K.two_pi_hi, d * K.two_pi_lo);
     const double t2 = t * t;
@@ -178,20 +193,31 @@ __device__ __forceinline__ void sincos_turns(uint32_t hi, uint32_t lo, double &s
     ps = fma(t2, ps, K.s1);
     const double st = fma(t * t2, ps, t);
     const double cm = t2 * fma(t2, fma(t2, K.c3, K.c2), K.c1);
-    const double sa = tb.sincos[j & 255u][0];
-    const double ca = tb.sincos[j & 255u][1];
     sn_out = sa + fma(sa, cm, ca * st);
     cs_out = ca + fma(ca, cm, -(sa * st));
 }
 
-// Box-Muller (Box & Muller 1958) pair from one block.
+__device__ __forceinline__ void sincos_turns(uint32_t hi, uint32_t lo, double &sn_out, double &cs_out,
+                                             const Coeffs &K, const Tables &tb)
+{
+    const uint32_t j = sincos_index(hi);
+    sincos_with_entry(hi, lo, j, tb.sincos[j & 255u][0], tb.sincos[j & 255u][1], sn_out, cs_out, K);
+}
+
+// Box-Muller (Box & Muller 1958) pair from one block.  Both table entries
+// are read before any arithmetic that needs them, so the two LDS round trips
+// overlap each other and the uniforms' conversions.
 __device__ __forceinline__ void normal_pair(const Words &w, double &z0, double &z1, const Coeffs &K,
                                             const Tables &tb)
 {
     const double u1 = uniform_oc(w.w0, w.w1);
-    const double rho = sqrt_nonneg(-2.0 * log_pos(u1, K, tb));
+    const uint32_t jl = log_index(u1);
+    const uint32_t js = sincos_index(w.w2);
+    const double invc = tb.log[jl][0], logc = tb.log[jl][1];
+    const double sa = tb.sincos[js & 255u][0], ca = tb.sincos[js & 255u][1];
     double sn, cs;
-    sincos_turns(w.w2, w.w3, sn, cs, K, tb);
+    sincos_with_entry(w.w2, w.w3, js, sa, ca, sn, cs, K);
+    const double rho = sqrt_nonneg(-2.0 * log_with_entry(u1, jl, invc, logc, K));
     z0 = rho * cs;
     z1 = rho * sn;
 }

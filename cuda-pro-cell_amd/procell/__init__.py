@@ -51,7 +51,7 @@ class CellType(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
-                ("chunk", ctypes.c_uint32), ("idle_lanes", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+                ("segments", ctypes.c_uint32), ("idle_lanes", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
 class Params(ctypes.Structure):
@@ -193,7 +193,7 @@ class Problem:
     """Inputs of one simulation, kept alive for the C structs that point at them."""
 
     def __init__(self, values, counts, proportion, mean, sd, t_max, phi=0.0, seed=0,
-                 track_ratio=True, device=0, block_size=0, blocks_per_cu=0, chunk=0, idle_lanes=0, flags=0):
+                 track_ratio=True, device=0, block_size=0, blocks_per_cu=0, segments=0, idle_lanes=0, flags=0):
         values = np.ascontiguousarray(values, dtype=np.float64)
         counts = np.ascontiguousarray(counts, dtype=np.uint64)
         self.rows = (Row * max(len(values), 1))()
@@ -221,7 +221,7 @@ class Problem:
         self.params.seed = int(seed) & (2**64 - 1)
         self.params.track_ratio = int(bool(track_ratio))
         self.params.device = int(device)
-        self.params.tuning = Tuning(block_size, blocks_per_cu, chunk, idle_lanes, flags)
+        self.params.tuning = Tuning(block_size, blocks_per_cu, segments, idle_lanes, flags)
         self.params.stream = None
 
     @classmethod
@@ -380,7 +380,8 @@ class Batch(Plan):
         t = Tuning()
         _check(lib().procell_batch_autotune(self.handle, s.ctypes.data_as(_u64p), int(reps), ctypes.byref(t)),
                "batch_autotune")
-        return {"chunk": t.chunk, "idle_lanes": t.idle_lanes, "block_size": t.block_size}
+        return {"segments": t.segments, "idle_lanes": t.idle_lanes, "block_size": t.block_size,
+                "blocks_per_cu": t.blocks_per_cu}
 
 
 class Relaxed:

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PROCELL_ABI_VERSION 1
+#define PROCELL_ABI_VERSION 2
 
 /* ---- status codes (never exit(); every entry point returns one) ---- */
 #define PROCELL_OK 0
@@ -65,9 +65,10 @@ typedef struct {
 /* Launch knobs.  0 means "choose automatically".  Results never depend on
  * them; only speed does. */
 typedef struct {
-    uint32_t block_size;    /* threads per workgroup of the tree kernel: 64..256, multiple of 64 */
-    uint32_t blocks_per_cu; /* resident workgroups per CU of the persistent tree kernel */
-    uint32_t chunk;         /* most dividing initial cells a wavefront claims from the work queue at once */
+    uint32_t block_size;    /* threads per workgroup: 64..256, multiple of 64 */
+    uint32_t blocks_per_cu; /* resident workgroups per CU */
+    uint32_t segments;      /* segments of initial cells per resident workgroup (1..64); the
+                               workgroups of later segments start as earlier ones finish */
     uint32_t idle_lanes;    /* idle lanes a wavefront tolerates before refilling them (0..63) */
     uint32_t flags;         /* PROCELL_FLAG_* */
 } procell_tuning;

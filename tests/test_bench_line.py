@@ -31,7 +31,7 @@ def _full_record():
                      "hbm_gbs": 453.1},
         "batched": {"value": 8.4e10, "unit": "divisions/s", "workload": "w" * 200, "cells": 52000000,
                     "divisions_per_run": 2.58e8, "ms_per_batch_run": 3.05, "runs": 20,
-                    "autotuned": {"chunk": 1024, "idle_lanes": 16, "blocks_per_cu": 3},
+                    "autotuned": {"segments": 8, "idle_lanes": 16, "blocks_per_cu": 3},
                     "sequential_plans_value": 2.3e9, "sequential_plans_sample": "s" * 100},
         "relaxed": {"workload": "w" * 250, "cells": 999517, "beta": 0.5, "forward_ms": 0.149, "backward_ms": 0.163,
                     "jacobian_ms": 0.16, "expansions_per_pass": 8200000, "expansions_per_s": 5.3e10,
@@ -83,7 +83,7 @@ def test_line_keeps_the_summaries():
     assert cal["lm"]["gpu_time_ratio_adam_over_lm_at_loss"] == 27.0
     assert cal["nelder_mead"]["gpu_time_ratio_at_common"] == 0.3
     assert cal["rich_target"]["gpu_time_ratio_at_2pct"] == 11.0
-    assert obj["batched"]["autotuned"]["chunk"] == 1024
+    assert obj["batched"]["autotuned"]["segments"] == 8
     assert obj["relaxed"]["forward_frac"] == 0.33
 
 
