@@ -153,10 +153,11 @@ using namespace procell;
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSeedSlots = 4;  // pinned host staging slots for batch seeds
 // Segments per resident workgroup unless tuned.  1e7-cell histogram, 3
-// workgroups of 256 per CU: 1 -> 0.321 ms per run, 2 -> 0.307, 4 -> 0.305,
-// 8 -> 0.338, 16 -> 0.401 (more segments: more set-up and window flushes).
-// A batch of many small sets balances better with finer segments.
-constexpr uint32_t kSegmentsPerWorkgroup = 4;
+// workgroups of 256 per CU, segment sizes decreasing geometrically: 1 ->
+// 0.321 ms per run, 2 -> 0.289, 3 -> 0.294, 4 -> 0.309, 8 -> 0.340 (more
+// segments: more set-up and window flushes).  A batch of many small sets
+// balances better with finer segments.
+constexpr uint32_t kSegmentsPerWorkgroup = 2;
 constexpr uint32_t kBatchSegmentsPerWorkgroup = 8;
 
 struct procell_plan {
@@ -278,9 +279,12 @@ double row_cost(const Setup &s, uint32_t row)
     return w;
 }
 
-// Split the initial cells into `g` contiguous segments of about equal cost.
-void split_segments(const Setup &s, uint32_t g, std::vector<uint64_t> &begin, std::vector<uint32_t> &row_of)
+// Split the initial cells into contiguous segments whose cumulative costs end
+// at the fractions cum[0] < cum[1] < ... < cum[g-1] = 1 of the total.
+void split_segments(const Setup &s, const std::vector<double> &cum, std::vector<uint64_t> &begin,
+                    std::vector<uint32_t> &row_of)
 {
+    const uint32_t g = (uint32_t)cum.size();
     const uint32_t n_rows = (uint32_t)s.v.size();
     begin.assign(g + 1, s.n_cells);
     row_of.assign(g, n_rows ? n_rows - 1 : 0);
@@ -296,8 +300,8 @@ void split_segments(const Setup &s, uint32_t g, std::vector<uint64_t> &begin, st
     for (uint32_t r = 0; r < n_rows && next < g; ++r) {
         const uint64_t n = s.start[r + 1] - s.start[r];
         const double row_total = w[r] * (double)n;
-        while (next < g && acc + row_total >= total * next / g) {
-            const double want = total * next / g - acc;
+        while (next < g && acc + row_total >= total * cum[next - 1]) {
+            const double want = total * cum[next - 1] - acc;
             uint64_t cut = s.start[r] + (uint64_t)(want / w[r]);
             if (cut > s.start[r + 1])
                 cut = s.start[r + 1];
@@ -453,25 +457,51 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
         for (uint32_t i = 0; i < n_sets; ++i)
             g_set[i] = wt > 0.0 ? (uint32_t)std::max(1.0, std::floor(w[i] / wt * target + 0.5)) : 1u;
     }
-    std::vector<uint64_t> seg_begin(1, 0);
-    std::vector<uint32_t> seg_row, seg_row_end, seg_set;
+    // Segment sizes: a single set's decrease geometrically in launch order
+    // (rounds of `resident` segments, each round's segments a factor q
+    // smaller, the last ones 1/8 of the first), so the last workgroups
+    // dispatched are short and the CUs finish together; a batch's sets are
+    // split evenly and all segments are launched largest first.
+    struct Seg {
+        double cost;
+        uint32_t set;
+        uint64_t b0, b1;  // cells, relative to the set
+        uint32_t row;
+    };
+    std::vector<Seg> segs;
     for (uint32_t i = 0; i < n_sets; ++i) {
         const Setup &s = pl->setups[i];
+        const uint32_t gi = g_set[i];
+        std::vector<double> wseg(gi, 1.0);
+        if (n_sets == 1 && per_wg > 1 && gi == resident * per_wg) {
+            const double q = pow(0.125, 1.0 / (double)(per_wg - 1));
+            for (uint32_t k = 0; k < gi; ++k)
+                wseg[k] = pow(q, (double)(k / resident));
+        }
+        std::vector<double> cum(gi);
+        double acc = 0.0, tot = 0.0;
+        for (double x : wseg)
+            tot += x;
+        for (uint32_t k = 0; k < gi; ++k) {
+            acc += wseg[k];
+            cum[k] = k + 1 == gi ? 1.0 : acc / tot;
+        }
         std::vector<uint64_t> b;
         std::vector<uint32_t> r;
-        split_segments(s, g_set[i], b, r);
-        info[i].seg_lo = (uint32_t)seg_row.size();
-        for (uint32_t k = 0; k < g_set[i]; ++k) {
-            seg_begin.push_back(info[i].cell_base + b[k + 1]);
-            seg_row.push_back(info[i].row_base + r[k]);
-            // one past the row of the segment's last cell (an empty segment: its first row + 1)
-            uint32_t r_end = r[k] + 1;
-            if (b[k + 1] > b[k])
-                r_end = (uint32_t)(std::upper_bound(s.start.begin(), s.start.end() - 1, b[k + 1] - 1) - s.start.begin());
-            seg_row_end.push_back(info[i].row_base + r_end);
-            seg_set.push_back(i);
-        }
-        info[i].seg_hi = (uint32_t)seg_row.size();
+        split_segments(s, cum, b, r);
+        const double sc = set_cost(s);
+        for (uint32_t k = 0; k < gi; ++k)
+            segs.push_back(Seg{sc * wseg[k] / tot, i, b[k], b[k + 1], r[k]});
+    }
+    std::stable_sort(segs.begin(), segs.end(), [](const Seg &x, const Seg &y) { return x.cost > y.cost; });
+    std::vector<uint64_t> seg_begin, seg_end;
+    std::vector<uint32_t> seg_row, seg_set;
+    for (const Seg &sg : segs) {
+        const SetInfo &si = info[sg.set];
+        seg_begin.push_back(si.cell_base + sg.b0);
+        seg_end.push_back(si.cell_base + sg.b1);
+        seg_row.push_back(si.row_base + sg.row);
+        seg_set.push_back(sg.set);
     }
     const uint32_t g = (uint32_t)seg_row.size();
     // no more window rows than a segment spans
@@ -482,7 +512,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
         if (!s.n_cells)
             continue;
         const uint64_t first = seg_begin[k] - si.cell_base;
-        const uint64_t end = seg_begin[k + 1] - si.cell_base;
+        const uint64_t end = seg_end[k] - si.cell_base;
         const uint64_t lc = end > first ? end - 1 : first;
         const uint64_t lcc = lc < s.n_cells ? lc : s.n_cells - 1;
         const uint32_t lr = (uint32_t)(std::upper_bound(s.start.begin(), s.start.end() - 1, lcc) - s.start.begin()) - 1;
@@ -528,9 +558,9 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     const size_t o_mean = off; off = align_up(off + (size_t)n_types_tot * 8, 256);
     const size_t o_sd = off; off = align_up(off + (size_t)n_types_tot * 8, 256);
     const size_t o_tf = off; off = align_up(off + (size_t)n_types_tot * 4, 256);
-    const size_t o_sb = off; off = align_up(off + (size_t)(g + 1) * 8, 256);
+    const size_t o_sb = off; off = align_up(off + (size_t)g * 8, 256);
+    const size_t o_se = off; off = align_up(off + (size_t)g * 8, 256);
     const size_t o_sr = off; off = align_up(off + (size_t)g * 4, 256);
-    const size_t o_sre = off; off = align_up(off + (size_t)g * 4, 256);
     const size_t o_ss = off; off = align_up(off + (size_t)g * 4, 256);
     const size_t o_info = off; off = align_up(off + (size_t)n_sets * sizeof(SetInfo), 256);
     const size_t o_seed = off; off = align_up(off + (size_t)n_sets * 8, 256);
@@ -559,8 +589,8 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     a.type_sd = (const double *)(dbase + o_sd);
     a.type_file = (const uint32_t *)(dbase + o_tf);
     a.seg_begin = (const uint64_t *)(dbase + o_sb);
+    a.seg_end = (const uint64_t *)(dbase + o_se);
     a.seg_row = (const uint32_t *)(dbase + o_sr);
-    a.seg_row_end = (const uint32_t *)(dbase + o_sre);
     a.seg_set = (const uint32_t *)(dbase + o_ss);
     a.sets = (const SetInfo *)(dbase + o_info);
     a.set_seed = (const uint64_t *)(dbase + o_seed);
@@ -592,8 +622,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     if (!up(o_rs, start32.data(), start32.size() * 4) || !up(o_ri, info32.data(), info32.size() * 4) ||
         !up(o_cum, cum.data(), cum.size() * 8) || !up(o_mean, mean.data(), mean.size() * 8) ||
         !up(o_sd, sd.data(), sd.size() * 8) || !up(o_tf, tfile.data(), tfile.size() * 4) ||
-        !up(o_sb, seg_begin.data(), seg_begin.size() * 8) || !up(o_sr, seg_row.data(), seg_row.size() * 4) ||
-        !up(o_sre, seg_row_end.data(), seg_row_end.size() * 4) ||
+        !up(o_sb, seg_begin.data(), seg_begin.size() * 8) || !up(o_se, seg_end.data(), seg_end.size() * 8) || !up(o_sr, seg_row.data(), seg_row.size() * 4) ||
         !up(o_ss, seg_set.data(), seg_set.size() * 4) || !up(o_info, info.data(), info.size() * sizeof(SetInfo)))
         return fail(PROCELL_E_HIP);
     if (hipStreamSynchronize(pl->stream) != hipSuccess)

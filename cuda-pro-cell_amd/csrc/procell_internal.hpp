@@ -29,8 +29,7 @@ struct SetInfo {
     uint32_t n_types;
     uint32_t type_base;   // first entry of the set in the type tables
     uint32_t row_base;    // first nonzero row of the set
-    uint32_t seg_lo, seg_hi;  // the set's segments (workgroups)
-    uint32_t pad[2];
+    uint32_t pad[4];
 };
 
 // A dividing initial cell, buffered in LDS between a root step and the
@@ -95,9 +94,9 @@ struct RunArgs {
     const double *type_mean;
     const double *type_sd;
     const uint32_t *type_file;   // sorted position -> file index
-    const uint64_t *seg_begin;   // [segments + 1] first initial cell of each workgroup's segment
+    const uint64_t *seg_begin;   // [segments] first initial cell of each workgroup's segment
+    const uint64_t *seg_end;     // [segments] one past its last initial cell
     const uint32_t *seg_row;     // [segments] row of each segment's first cell (window base)
-    const uint32_t *seg_row_end; // [segments] one past the row of each segment's last cell
     unsigned long long *tally;   // per set [rows][kdim][n_types] of the set, file type order
     unsigned long long *counters;  // [n_sets][kCounterCopies][kNumCounters]
 };

@@ -82,49 +82,6 @@ __global__ void zero_kernel(unsigned long long *p, uint64_t n)
         p[i] = 0ull;
 }
 
-// First row of a range: 64-ary search with one load per lane per round over
-// rows [lo, hi) (row_start[lo] <= cell < row_start[hi]).
-__device__ __forceinline__ uint32_t find_row_wave(const uint32_t *row_start, uint32_t lo, uint32_t hi,
-                                                  uint64_t cell, uint32_t lane)
-{
-    while (hi - lo > 1) {
-        const uint32_t step = (hi - lo + kWave - 1) / kWave;
-        const uint32_t cand = lo + lane * step;
-        const bool le = cand < hi && row_start[cand] <= cell;
-        const uint64_t m = __ballot(le);
-        const uint32_t top = 63u - (uint32_t)__builtin_clzll(m);  // lane 0 always qualifies
-        const uint32_t new_lo = lo + top * step;
-        const uint32_t new_hi = new_lo + step < hi ? new_lo + step : hi;
-        lo = new_lo;
-        hi = new_hi;
-    }
-    return lo;
-}
-
-// Row of the cell c_cur + lane of a root step (lanes < n_valid), given the row
-// row_cur of c_cur.  Either the whole step lies in row_cur, or the rows that
-// start inside the step are marked in the wavefront's 64 flags (one load per
-// lane) and each lane counts the starts at or below it.
-__device__ __forceinline__ uint32_t step_row(const uint32_t *row_start, uint32_t n_rows, uint32_t row_cur,
-                                             uint64_t c_cur, uint32_t n_valid, uint32_t lane, uint32_t *flags)
-{
-    const uint64_t next = row_cur + 1 < n_rows ? (uint64_t)row_start[row_cur + 1] : ~0ull;
-    if (__builtin_amdgcn_readfirstlane((uint32_t)(next > c_cur + n_valid - 1)))
-        return row_cur;
-    flags[lane] = 0u;
-    __builtin_amdgcn_wave_barrier();
-    const uint32_t ri = row_cur + 1 + lane;
-    if (ri < n_rows) {
-        const uint64_t t = (uint64_t)row_start[ri] - c_cur;  // >= 0: row_cur holds c_cur
-        if (t < n_valid)
-            flags[t] = 1u;
-    }
-    __builtin_amdgcn_wave_barrier();
-    const uint64_t m = __ballot(flags[lane] != 0u);
-    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-    return row_cur + (uint32_t)__popcll(m & upto);
-}
-
 // Per-lane event counts.  Recorded and dropped cells are not counted on the
 // device: the host takes recorded = the sum of the tally and dropped = cells
 // + divisions - recorded - lost (every division adds one cell, and every
@@ -223,17 +180,16 @@ struct Lane {
 //    dropped on the spot, so the lane's next cell is always a dividing one
 //    (the left daughter, else the right, else the deepest pending right
 //    daughter from the lane's column of the LDS stack).
-// LDS: log and sin/cos tables | stream coefficients | type tables (cum,
-// mean, sd: double [T]; file: u32 [T]) | row tables when they fit (first
+// LDS: log and sin/cos tables | stream coefficients | type tables ((mean,
+// sd): double2 [T]; cum: double [T]; file: u32 [T]) | row tables when they fit (first
 // cell: u32 [n_rows + 1]; info: u32 [n_rows]) | counter sums u32 [8] | root
-// step cursor u32 [4] | per wavefront: buffered entries QueueEntry [64],
-// row-start flags u32 [64] |
+// step cursor u32 [4] | per wavefront: buffered entries QueueEntry [64] |
 // window tally u32 [win_rows * kdim * T + 1 spare] | stack double
 // [stack_levels * blockDim]
 // ======================================================================
 __host__ __device__ constexpr size_t sim_wave_lds_bytes()
 {
-    return (size_t)kWave * sizeof(QueueEntry) + kWave * 4;
+    return (size_t)kWave * sizeof(QueueEntry);
 }
 
 constexpr size_t kSimSharedBytes = 48;  // counter sums u32 [8], root step cursor u32 [4]
@@ -260,30 +216,26 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
     double(*m_log)[2] = (double(*)[2])lds;            // [128] log table
     double(*m_sc)[2] = (double(*)[2])(lds + 128 * 16);  // [256] sin/cos table
     dev::Coeffs *k_lds = (dev::Coeffs *)(lds + kMathTableBytes);
-    double *t_cum = (double *)(lds + kMathTableBytes + (sizeof(dev::Coeffs) + 15) / 16 * 16);
-    double *t_mean = t_cum + T;
-    double *t_sd = t_mean + T;
-    uint32_t *t_file = (uint32_t *)(t_sd + T);
-    uint32_t *r_start = (uint32_t *)((unsigned char *)t_cum + ((size_t)T * 28 + 15) / 16 * 16);
+    double2 *t_msd = (double2 *)(lds + kMathTableBytes + (sizeof(dev::Coeffs) + 15) / 16 * 16);  // mean, sd
+    double *t_cum = (double *)(t_msd + T);
+    uint32_t *t_file = (uint32_t *)(t_cum + T);
+    uint32_t *r_start = (uint32_t *)((unsigned char *)t_msd + ((size_t)T * 28 + 15) / 16 * 16);
     uint32_t *r_info = r_start + lds_rows + 1;
     uint32_t *red = (uint32_t *)((unsigned char *)r_start + (RowsLds ? ((size_t)(2 * lds_rows + 1) * 4 + 15) / 16 * 16 : 0));
     uint32_t *cursor = red + kNumCounters;  // root steps of the segment claimed so far
     QueueEntry *ent = (QueueEntry *)((unsigned char *)red + kSimSharedBytes + (size_t)wave * sim_wave_lds_bytes());
-    uint32_t *flags = (uint32_t *)(ent + kWave);
     unsigned int *win = (unsigned int *)((unsigned char *)red + kSimSharedBytes + (size_t)n_waves * sim_wave_lds_bytes());
     double *stack = (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
 
     const uint64_t seg_begin = uni64(a.seg_begin[seg0]);
-    const uint64_t seg_cells = uni64(a.seg_begin[seg0 + 1]) - seg_begin;
+    const uint64_t seg_cells = uni64(a.seg_end[seg0]) - seg_begin;
     const uint32_t row_lo = uni(a.seg_row[seg0]);  // first row of the segment and of the tally window
-    const uint32_t row_hi = uni(a.seg_row_end[seg0]);  // one past its last row
     if (tid == 0)
         *k_lds = dev::coeffs();
     copy_tables(m_log, m_sc, tid, bdim);
     for (uint32_t i = tid; i < T; i += bdim) {
         t_cum[i] = a.type_cum[S.type_base + i];
-        t_mean[i] = a.type_mean[S.type_base + i];
-        t_sd[i] = a.type_sd[S.type_base + i];
+        t_msd[i] = make_double2(a.type_mean[S.type_base + i], a.type_sd[S.type_base + i]);
         t_file[i] = a.type_file[S.type_base + i];
     }
     for (uint32_t i = tid; i <= lds_rows && RowsLds; i += bdim)
@@ -306,6 +258,23 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
 
     uint32_t rpos = 0, rn = 0;      // buffered dividing roots: entries [rpos, rn) of ent
     uint32_t roots_done = 0;
+    // the first eight cumulative proportions of the type search, in
+    // registers (-inf for types not searched: the last one and those beyond T)
+    double cum8[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+        cum8[k] = k + 1 < T ? t_cum[k] : -__builtin_inf();
+    // Row boundaries of the wavefront's root steps, in registers: lane j holds
+    // the first cell of row rb + 1 + j (~0 past the last row) and the info of
+    // row rb + j.  A wavefront's root steps only move forward in its segment,
+    // so rb only grows.
+    uint32_t rb = row_lo, bnd = 0, rinfo = 0;
+    auto load_rows = [&]() {
+        const uint32_t r1 = rb + 1 + lane;
+        bnd = r1 <= a.n_rows ? rs[r1] : 0xFFFFFFFFu;
+        rinfo = rb + lane < a.n_rows ? ri[rb + lane] : 0u;
+    };
+    load_rows();
 
     Lane<N> L;
     L.cell = L.tb_glob = 0;
@@ -334,11 +303,22 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
             if (!roots_done) {
                 const uint64_t c_cur = seg_begin + c_off;
                 const uint32_t n_valid = uni((uint32_t)min<uint64_t>(kWave, seg_cells - c_off));
-                const uint32_t row_cur = uni(find_row_wave(rs, row_lo, row_hi, c_cur, lane));
                 const uint64_t c = c_cur + lane;
                 const bool valid = lane < n_valid;
-                const uint32_t row = step_row(rs, a.n_rows, row_cur, c_cur, n_valid, lane, flags);
-                const uint32_t info = ri[valid ? row : row_cur];
+                // rows of the step's cells (cell ids < 2^32): move rb to the
+                // row of c_cur, then count the boundaries inside the step
+                const uint32_t c32 = (uint32_t)c_cur;
+                for (uint64_t le = __ballot(bnd <= c32); le; le = __ballot(bnd <= c32)) {
+                    rb = uni(rb + (uint32_t)__popcll(le));
+                    load_rows();
+                }
+                const uint32_t nb = uni((uint32_t)__popcll(__ballot((uint64_t)bnd < c_cur + n_valid)));
+                uint32_t rrel = 0;  // boundaries at or below this lane's cell
+                for (uint32_t j = 0; j < nb; ++j)
+                    rrel += lane >= (uint32_t)__builtin_amdgcn_readlane((int)bnd, (int)j) - c32 ? 1u : 0u;
+                const uint32_t row = rb + rrel;
+                const uint32_t info = nb ? (uint32_t)__shfl(rinfo, (int)rrel)
+                                         : (uint32_t)__builtin_amdgcn_readfirstlane((int)rinfo);
                 const uint32_t kc = info & 0xFFu;
                 const bool ge_phi = (info >> 8) & 1u;
                 // one block per initial cell: type uniform (word 0), phase
@@ -350,10 +330,16 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 // scanning from the highest index down, the last hit is the
                 // first; past the last proportion, the last type
                 uint32_t jt = T - 1;
-                for (uint32_t k = T - 1; k-- > 0;)
-                    jt = ut < t_cum[k] ? k : jt;
-                const double m = t_mean[jt];
-                const double sdv = t_sd[jt];
+                if (T > 9) {
+                    for (uint32_t k = T - 2; k >= 8; --k)
+                        jt = ut < t_cum[k] ? k : jt;
+                }
+#pragma unroll
+                for (int k = 7; k >= 0; --k)
+                    jt = ut < cum8[k] ? (uint32_t)k : jt;
+                const double2 msd = t_msd[jt];
+                const double m = msd.x;
+                const double sdv = msd.y;
                 const bool quiescent = m < 0.0;
                 // first timer (cell.cu:106-122): two candidates from the root
                 // block, then node 1's redraw stream
@@ -397,8 +383,9 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 const uint32_t tf = t_file[jt];
                 const uint32_t woff = row - row_lo;
                 const bool in_win = woff < a.win_rows;
-                if (rec_ok && in_win)
+                if (rec_ok && in_win) {
                     atomicAdd(&win[woff * kdim * T + tf], 1u);
+                }
                 uint64_t pend = __ballot(rec_ok && !in_win);
                 while (pend) {
                     const uint32_t leader = (uint32_t)__builtin_ctzll(pend);
@@ -440,8 +427,9 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 L.pend = (N)0;
                 L.redo = 0u;
                 L.kcap = ri[row] & 0xFFu;
-                L.mean = t_mean[jt];
-                L.sd = t_sd[jt];
+                const double2 msd = t_msd[jt];
+                L.mean = msd.x;
+                L.sd = msd.y;
                 L.tb_lds = off < a.win_rows ? off * kdim * T + tf : kNoWindow;
                 L.tb_glob = S.tally_base + (uint64_t)(row - S.row_base) * kdim * T + tf;
                 L.active = true;

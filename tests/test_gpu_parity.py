@@ -211,3 +211,23 @@ def test_sparse_rows_many_row_starts_per_step(procell_lib, oracle):
     gp, op = _problem_pair(procell_lib, oracle, values, counts, [0.5, 0.5], [12.0, 30.0], [3.0, 9.0],
                            70.0, 2.0, 17)
     assert_same(procell_lib, oracle, gp, op)
+
+
+def test_cell_ids_beyond_two_to_the_31(procell_lib):
+    """More than 2^31 initial cells (quiescent, so the run is root steps
+    only): cell ids and segment bounds past 2^31 must stay exact in the
+    kernel's wave-uniform 64-bit values.  The output is the input restricted
+    to values >= phi (closed form): every quiescent cell is tallied at its
+    own value, or lost below phi."""
+    big = 2**31 + 12345
+    values = [60.0, 55.0, 10.0]
+    counts = [big - 9000, 9000, 7]
+    prob = procell_lib.Problem(values, counts, [1.0], [-1.0], [-1.0], 100.0, 50.0, 3)
+    print("cells", sum(counts), flush=True)
+    res = procell_lib.simulate(prob)
+    st = res.stats
+    assert st["cells"] == sum(counts) and st["lost"] == 7 and st["divisions"] == 0
+    assert res.value.tolist() == [55.0, 60.0]
+    assert res.count.tolist() == [9000, big - 9000]
+    assert res.type_count.reshape(-1).tolist() == [9000, big - 9000]
+    assert res.raw_tally[:, 0, 0].tolist() == [big - 9000, 9000, 0]
