@@ -1,12 +1,14 @@
 """Benchmark: cell divisions per second of the plain proliferation method on
 one MI355X.
 
-A step is one complete simulation (procell_plan_run: clear the tallies, the
-root kernel, the persistent division-tree kernel) of the 1024-bin, ~1e7-cell histogram
+A step is one complete simulation (procell_plan_run: clear the tallies, then
+the simulation kernel: initial cells and their division trees) of the
+1024-bin, ~1e7-cell histogram
 with five cell types (four proliferating, one quiescent), t_max 240, phi 50
 and per-type tracking.  Inputs are uploaded once before timing (resident in
-HBM); every step uses a different seed.  value = divisions summed over the
-timed steps / wall time between the two synchronisations.
+HBM; the launch knobs come from the library's autotuner, run untimed on the
+same plan); every step uses a different seed.  value = divisions summed over
+the timed steps / event time between the two synchronisations.
 
 Prints one compact JSON line on stdout (headline first: metric, value, unit,
 roofline, cpu_baseline, then one short summary object per further workload);
@@ -41,8 +43,8 @@ LANE_OPS_PER_EXPANSION_FWD = 234
 LANE_OPS_PER_EXPANSION_ADJ = 268
 # MI355X VALU issue peak: 256 CUs x 64 lanes/clk x 2.4 GHz (lane-ops/s).
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "simulation_kernels_hbm_traffic.txt")
-COUNTER_FILE = os.path.join(REPO, "profiles", "simulation_kernels_valu_counters.txt")
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "sim_kernel_hist1024_1e7_hbm_traffic.txt")
+COUNTER_FILE = os.path.join(REPO, "profiles", "sim_kernel_hist1024_1e7_valu_counters.txt")
 
 
 def log(*a):
@@ -50,8 +52,8 @@ def log(*a):
 
 
 def traffic_from_profile():
-    """HBM bytes per run (root + tree kernel) from the committed rocprofv3
-    --pmc summary."""
+    """HBM bytes per run of the simulation kernel from the committed
+    rocprofv3 --pmc summary."""
     try:
         with open(TRAFFIC_FILE) as f:
             m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
@@ -637,6 +639,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=1.5, help="seconds of CPU-baseline wall time")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-method measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch knobs for the headline plan")
+    ap.add_argument("--knobs", default="", metavar="BPC,SEG,IDLE",
+                    help="fixed launch knobs for the headline plan (workgroups per CU, segments per workgroup, "
+                         "idle lanes) instead of the autotuner")
     ap.add_argument("--no-relaxed", action="store_true", help="skip the relaxed-estimator measurement")
     ap.add_argument("--fit-steps", type=int, default=300, help="Adam steps of the relaxed-estimator fit")
     ap.add_argument("--no-calib", action="store_true", help="skip the batched-calibration measurement")
@@ -662,19 +668,30 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("no GPU visible")
     if args.copy_calib:
+        # a device-to-device copy of known size whose source has been pushed
+        # out of the 256 MB Infinity Cache (another 512 MiB written after it),
+        # so its reads come from HBM
         n = args.copy_calib << 20
         src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        evict = torch.zeros((512 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"copy calibration: {n} bytes read, {n} bytes written")
-        del src, dst
+        log(f"copy calibration: {n} bytes read, {n} bytes written (the last elementwise kernel before the plan)")
+        del src, dst, evict
     w = workloads.large_histogram(0)
     log(f"workload {w.name}: {w.n_cells} cells, {len(w.values)} bins, {len(w.proportion)} types")
     stream = torch.cuda.Stream()
-    prob = procell.Problem.from_workload(w, 0, flags=procell.FLAG_TIMING)
+    fixed = [int(x) for x in args.knobs.split(",")] if args.knobs else None
+    kw = dict(blocks_per_cu=fixed[0], segments=fixed[1], idle_lanes=fixed[2]) if fixed else {}
+    prob = procell.Problem.from_workload(w, 0, flags=procell.FLAG_TIMING, **kw)
     plan = procell.Plan(prob, stream=stream.cuda_stream)
+    # launch knobs from the library's autotuner, on this workload (results do
+    # not depend on them); untimed
+    t_tune = time.perf_counter()
+    tuned = plan.autotune([0], reps=3) if not (args.no_autotune or fixed) else None
+    log(f"autotuned {tuned} in {time.perf_counter() - t_tune:.2f} s")
     K, W = args.steps, args.warmup
     seeds = [1000 + i for i in range(K)]
     for i in range(W):
@@ -740,7 +757,8 @@ def main():
             "divisions_per_step": div_per_run,
             "seeds": f"{seeds[0]}..{seeds[-1]}",
             "knobs": {"workgroups": stats["workgroups"], "block_size": stats["block_size"],
-                      "lds_bytes": stats["lds_bytes"], "window_rows": stats["window_rows"]},
+                      "lds_bytes": stats["lds_bytes"], "window_rows": stats["window_rows"],
+                      **({"autotuned": tuned} if tuned else {})},
             "abi_end_to_end_ms": float(np.median(e2e)),
         },
         "roofline": {
@@ -750,7 +768,7 @@ def main():
             "unit": "Tlane-op/s",
             "frac": achieved / VALU_PEAK_LANE_OPS,
             "traffic": traffic,
-            "kernel": "zero_kernel + root_kernel + tree_kernel (one run, back to back, timed together)",
+            "kernel": "zero_kernel + sim_kernel (one run, timed together)",
             "kernel_ms": kernel_ms,
             "work_per_unit": f"{LANE_OPS_PER_DIVISION} lane-ops per division",
             "peak_source": "MI355X FP64 vector peak 78.6 TFLOP/s / 2 = one wave64 VALU instruction "

@@ -184,6 +184,10 @@ struct procell_plan {
     bool timed = false;            // a run has recorded its events
     uint64_t batch_runs = 0;
     bool ran = false;
+    // copies of the inputs (the autotuner re-creates the layout from them)
+    std::vector<procell_params> params;
+    std::vector<std::vector<procell_row>> rows;
+    std::vector<std::vector<procell_type>> types;
 };
 
 constexpr uint32_t kTimedRuns = 256;
@@ -352,6 +356,15 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
             return rc;
         }
         pl->track_ratio[i] = ps[i].track_ratio;
+    }
+    pl->params.assign(ps, ps + n_sets);
+    pl->rows.resize(n_sets);
+    pl->types.resize(n_sets);
+    for (uint32_t i = 0; i < n_sets; ++i) {
+        pl->rows[i].assign(ps[i].rows, ps[i].rows + ps[i].n_rows);
+        pl->types[i].assign(ps[i].types, ps[i].types + ps[i].n_types);
+        pl->params[i].rows = pl->rows[i].data();
+        pl->params[i].types = pl->types[i].data();
     }
     int n_cu = 0;
     size_t lds_per_cu = 0, lds_per_block = 0;
@@ -922,56 +935,118 @@ int procell_batch_result(procell_plan *plan, uint32_t set, procell_result *r)
 
 int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t reps, procell_tuning *chosen)
 {
+    return procell_plan_autotune(plan, seeds, reps, chosen);
+}
+
+// Device time of warm-up + reps runs (eager launches) of a plan.
+static int time_plan(procell_plan *pl, const uint64_t *seeds, uint32_t reps, hipEvent_t e0, hipEvent_t e1, float *ms)
+{
+    const bool timing = pl->timing;
+    pl->timing = false;
+    int rc = run_plan(pl, seeds, 0, false);  // warm-up
+    if (rc == PROCELL_OK && hipEventRecord(e0, pl->stream) != hipSuccess)
+        rc = PROCELL_E_HIP;
+    for (uint32_t i = 0; rc == PROCELL_OK && i < reps; ++i)
+        rc = run_plan(pl, seeds, 0, false);
+    if (rc == PROCELL_OK &&
+        (hipEventRecord(e1, pl->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+         hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+        rc = PROCELL_E_HIP;
+    pl->timing = timing;
+    return rc;
+}
+
+int procell_plan_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t reps, procell_tuning *chosen)
+{
     if (!plan || !seeds || !chosen || reps == 0)
         return PROCELL_E_INVALID;
     HIP_TRY(hipSetDevice(plan->device));
-    static const uint32_t kIdle[] = {8, 16, 24};
+    // layouts: resident workgroups per CU (sets the LDS tally window's
+    // width) x segments per resident workgroup; for each, the idle-lane
+    // threshold, which needs no new layout
+    static const uint32_t kBlocks[] = {2, 3};
+    static const uint32_t kSegments[] = {1, 2, 3, 4, 8};
+    static const uint32_t kIdle[] = {8, 12, 16, 24};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
         (void)hipEventDestroy(e0);
         return PROCELL_E_HIP;
     }
-    RunArgs &a = plan->args;
-    const RunArgs keep = a;
+    const uint32_t n_sets = plan->args.n_sets;
+    std::vector<procell_params> ps = plan->params;
     float best = -1.f;
-    uint32_t best_idle = a.idle_limit;
+    procell_plan *best_plan = nullptr;
+    procell_tuning best_tu{};
     int rc = PROCELL_OK;
-    const bool timing = plan->timing;
-    plan->timing = false;
-    drop_graph(plan);
-    for (uint32_t idle : kIdle) {
-        a.idle_limit = idle;
-        rc = run_plan(plan, seeds, 0, false);  // warm-up
-        for (uint32_t i = 0; rc == PROCELL_OK && i < reps; ++i) {
-            if (i == 0 && hipEventRecord(e0, plan->stream) != hipSuccess)
-                rc = PROCELL_E_HIP;
-            if (rc == PROCELL_OK)
-                rc = run_plan(plan, seeds, 0, false);
+    for (uint32_t bpc : kBlocks) {
+        for (uint32_t seg : kSegments) {
+            for (procell_params &p : ps) {
+                p.tuning.blocks_per_cu = bpc;
+                p.tuning.segments = seg;
+                p.tuning.idle_lanes = 0;
+                p.tuning.flags &= ~PROCELL_FLAG_TIMING;  // candidates need no timing ring
+                p.stream = (void *)plan->stream;         // and run on the plan's stream
+            }
+            procell_plan *cand = nullptr;
+            rc = create_plan(ps.data(), n_sets, &cand);
+            if (rc != PROCELL_OK)
+                break;
+            cand->own_stream = false;
+            bool kept = false;
+            for (uint32_t idle : kIdle) {
+                cand->args.idle_limit = idle;
+                float ms = 0.f;
+                rc = time_plan(cand, seeds, reps, e0, e1, &ms);
+                if (rc != PROCELL_OK)
+                    break;
+                if (best < 0.f || ms < best) {
+                    best = ms;
+                    best_tu = ps[0].tuning;
+                    best_tu.idle_lanes = idle;
+                    best_tu.block_size = (uint32_t)cand->shape.block_size;
+                    if (best_plan && best_plan != cand)
+                        destroy_plan(best_plan);
+                    best_plan = cand;
+                    kept = true;
+                }
+            }
+            if (best_plan == cand)
+                best_plan->args.idle_limit = best_tu.idle_lanes;
+            if (!kept)
+                destroy_plan(cand);
+            if (rc != PROCELL_OK)
+                break;
         }
-        float ms = 0.f;
-        if (rc == PROCELL_OK &&
-            (hipEventRecord(e1, plan->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            rc = PROCELL_E_HIP;
         if (rc != PROCELL_OK)
             break;
-        if (best < 0.f || ms < best) {
-            best = ms;
-            best_idle = idle;
-        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    plan->timing = timing;
-    a = keep;
-    if (rc != PROCELL_OK)
-        return rc;
-    a.idle_limit = best_idle;
+    if (rc != PROCELL_OK || !best_plan) {
+        if (best_plan)
+            destroy_plan(best_plan);
+        return rc != PROCELL_OK ? rc : PROCELL_E_INVALID;
+    }
+    // the plan takes the fastest candidate's layout; its own stream, events
+    // and timing ring stay
+    drop_graph(plan);
+    std::swap(plan->shape, best_plan->shape);
+    std::swap(plan->args, best_plan->args);
+    std::swap(plan->d_block, best_plan->d_block);
+    std::swap(plan->tally_total, best_plan->tally_total);
+    std::swap(plan->zero_words, best_plan->zero_words);
+    std::swap(plan->d_seeds, best_plan->d_seeds);
+    std::swap(plan->tally_base, best_plan->tally_base);
+    std::swap(plan->tally_len, best_plan->tally_len);
+    std::swap(plan->setups, best_plan->setups);
+    best_tu.flags = plan->params[0].tuning.flags;
+    for (uint32_t i = 0; i < n_sets; ++i)
+        plan->params[i].tuning = best_tu;
+    destroy_plan(best_plan);
     plan->runs = 0;
-    memset(chosen, 0, sizeof *chosen);
-    chosen->block_size = (uint32_t)plan->shape.block_size;
-    chosen->idle_lanes = best_idle;
+    plan->ran = false;
+    *chosen = best_tu;
     return PROCELL_OK;
 }
 

@@ -101,6 +101,7 @@ EXPORTS = (
     "procell_read_histogram",
     "procell_read_types", "procell_write_histogram", "procell_free", "procell_debug_draws",
     "procell_batch_create", "procell_batch_run", "procell_batch_result", "procell_batch_autotune",
+    "procell_plan_autotune",
     "procell_relaxed_create", "procell_relaxed_shape", "procell_relaxed_forward", "procell_relaxed_backward",
     "procell_relaxed_set_stride", "procell_relaxed_expansions", "procell_relaxed_set_cuts", "procell_relaxed_destroy",
     "procell_relaxed_jacobian", "procell_relaxed_create_batch", "procell_relaxed_set_shape",
@@ -152,6 +153,7 @@ def lib():
         L.procell_batch_run.argtypes = [ctypes.c_void_p, _u64p]
         L.procell_batch_result.argtypes = [ctypes.c_void_p, ctypes.c_uint32, R]
         L.procell_batch_autotune.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint32, ctypes.POINTER(Tuning)]
+        L.procell_plan_autotune.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint32, ctypes.POINTER(Tuning)]
     if hasattr(L, "procell_relaxed_create"):
         L.procell_relaxed_create.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]
         L.procell_relaxed_shape.argtypes = [ctypes.c_void_p, _u64p, _u32p, _u32p]
@@ -321,6 +323,20 @@ class Plan:
                "plan_kernel_times")
         return ms[: n.value].copy()
 
+    def autotune(self, seeds, reps: int = 2) -> dict:
+        """Times every candidate layout (resident workgroups per CU x segments
+        per workgroup) and idle-lane threshold on this plan's inputs (seeds:
+        one per set), keeps the fastest and returns its knobs."""
+        s = np.ascontiguousarray(np.atleast_1d(np.asarray(seeds, dtype=np.uint64)))
+        n_sets = len(getattr(self, "problems", [None]))
+        if s.size != n_sets:
+            raise ValueError(f"{s.size} seeds for {n_sets} sets")
+        t = Tuning()
+        _check(lib().procell_plan_autotune(self.handle, s.ctypes.data_as(_u64p), int(reps), ctypes.byref(t)),
+               "plan_autotune")
+        return {"blocks_per_cu": t.blocks_per_cu, "segments": t.segments, "idle_lanes": t.idle_lanes,
+                "block_size": t.block_size}
+
     @property
     def stream(self) -> int:
         return lib().procell_plan_stream(self.handle) or 0
@@ -374,14 +390,8 @@ class Batch(Plan):
         return buf.result()
 
     def autotune(self, seeds, reps: int = 2) -> dict:
-        """Times the candidate claim sizes and idle-lane thresholds on these
-        sets, keeps the fastest and returns it."""
-        s = self._seeds(seeds)
-        t = Tuning()
-        _check(lib().procell_batch_autotune(self.handle, s.ctypes.data_as(_u64p), int(reps), ctypes.byref(t)),
-               "batch_autotune")
-        return {"segments": t.segments, "idle_lanes": t.idle_lanes, "block_size": t.block_size,
-                "blocks_per_cu": t.blocks_per_cu}
+        """procell_plan_autotune on these sets (one seed each)."""
+        return Plan.autotune(self, seeds, reps)
 
 
 class Relaxed:

@@ -167,11 +167,20 @@ int procell_plan_kernel_times(procell_plan *plan, double *ms, uint32_t max_runs,
 int procell_batch_create(const procell_params *sets, uint32_t n_sets, procell_plan **plan);
 int procell_batch_run(procell_plan *plan, const uint64_t *seeds);              /* seeds[n_sets]; enqueue only */
 int procell_batch_result(procell_plan *plan, uint32_t set, procell_result *r); /* wait, download, merge */
-/* Autotuner: times `reps` runs (seeds[n_sets]) of every candidate claim
- * size (chunk) and idle-lane threshold on this plan's own sets, keeps the
- * fastest for later runs and returns it in *chosen.  Results do not depend
- * on the knobs. */
+/* Autotuner of a batch plan: procell_plan_autotune (below). */
 int procell_batch_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t reps, procell_tuning *chosen);
+/* Autotuner of any plan (one set or a batch): times `reps` runs
+ * (seeds[n_sets], one per set) of every candidate layout -- resident
+ * workgroups per CU, which sets the width of the LDS tally window, times
+ * segments per resident workgroup, i.e. the granularity of the work handed
+ * to the workgroup dispatcher -- and, for each layout, of every idle-lane
+ * threshold (how many lanes of a wavefront may idle before it refills
+ * them), on the plan's own inputs; the plan keeps the fastest for later
+ * runs and *chosen returns it.  One initial cell per lane at a time (the
+ * lane walks its tree depth first).  Results do not depend on the knobs.
+ * No reference counterpart (the reference's one knob, -d, only batched
+ * levels: src/io/cmdargs.cpp:295-351). */
+int procell_plan_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t reps, procell_tuning *chosen);
 
 /* ---- relaxed (differentiable) estimator --------------------------------
  * No reference counterpart (the reference only samples).  For calibrating

@@ -231,3 +231,30 @@ def test_cell_ids_beyond_two_to_the_31(procell_lib):
     assert res.count.tolist() == [9000, big - 9000]
     assert res.type_count.reshape(-1).tolist() == [9000, big - 9000]
     assert res.raw_tally[:, 0, 0].tolist() == [big - 9000, 9000, 0]
+
+
+def test_autotuned_plan_matches_oracle(procell_lib, oracle):
+    """procell_plan_autotune rebuilds the plan's layout (workgroups per CU,
+    segments, idle-lane threshold); runs after it equal the oracle bit for
+    bit, and the timing ring still works."""
+    from procell import workloads
+
+    w = workloads.scaled(workloads.small_histogram(2), 0.3)
+    gp, op = _problem_pair(procell_lib, oracle, w.values, w.counts, w.proportion, w.mean, w.sd, w.t_max, 0.0, 77,
+                           flags=procell_lib.FLAG_TIMING)
+    with procell_lib.Plan(gp) as plan:
+        chosen = plan.autotune([77], reps=1)
+        print("chosen", chosen, flush=True)
+        assert chosen["blocks_per_cu"] in (2, 3) and chosen["segments"] in (1, 2, 3, 4, 8)
+        assert chosen["idle_lanes"] in (8, 12, 16, 24)
+        for seed in (77, 78):
+            plan.run(seed)
+            res = plan.result()
+            tally, _, st = oracle.simulate(oracle.Problem(w.values, w.counts, w.proportion, w.mean, w.sd, w.t_max,
+                                                          0.0, seed))
+            assert np.array_equal(res.raw_tally, tally), seed
+            for k in STATS:
+                assert res.stats[k] == st[k], (seed, k)
+        assert len(plan.kernel_times()) == 2
+        with pytest.raises(ValueError):
+            plan.autotune([1, 2])
