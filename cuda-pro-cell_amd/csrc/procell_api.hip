@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -171,6 +172,7 @@ struct procell_plan {
     LaunchShape shape{};
     RunArgs args{};
     void *d_block = nullptr;  // one allocation holding every table
+    bool own_block = true;    // false: d_block is the static scratch array
     size_t tally_total = 0;
     size_t zero_words = 0;    // counters and tally, cleared per run
     uint64_t *d_seeds = nullptr;       // [n_sets] seeds of the current batch run (in d_block)
@@ -191,6 +193,11 @@ struct procell_plan {
 };
 
 constexpr uint32_t kTimedRuns = 256;
+
+// Device memory of procell_simulate_noalloc, one copy per GPU (see
+// scratch_address below for how host code reaches it).
+constexpr size_t kScratchBytes = PROCELL_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -242,7 +249,7 @@ void destroy_plan(procell_plan *pl)
             (void)hipEventDestroy(e);
     if (pl->h_seeds)
         (void)hipHostFree(pl->h_seeds);
-    if (pl->d_block)
+    if (pl->d_block && pl->own_block)
         (void)hipFree(pl->d_block);
     if (pl->own_stream && pl->stream)
         (void)hipStreamDestroy(pl->stream);
@@ -333,7 +340,12 @@ double set_cost(const Setup &s)
 // A plan over n_sets parameter sets (1: a plain run).  Rows, cells, type
 // entries, tally entries and segments of the sets are concatenated in set
 // order; the device code finds a set's values through its SetInfo.
-int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
+// With `arena` (device memory of `arena_bytes`, already on ps[0].device) the
+// plan's device block is carved from it instead of allocated, no stream or
+// event is created (the plan runs on ps[0].stream, NULL: the default stream)
+// and timing is off.
+int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, void *arena = nullptr,
+                size_t arena_bytes = 0)
 {
     if (!ps || !out || n_sets == 0)
         return PROCELL_E_INVALID;
@@ -376,7 +388,11 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
         }
     }
     pl->device = ps[0].device;
-    pl->timing = (tu.flags & PROCELL_FLAG_TIMING) != 0;
+    pl->timing = !arena && (tu.flags & PROCELL_FLAG_TIMING) != 0;
+    if (arena && n_sets != 1) {
+        delete pl;
+        return PROCELL_E_INVALID;
+    }
     auto fail = [&](int code) {
         destroy_plan(pl);
         return code;
@@ -577,15 +593,23 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out)
     const size_t o_ss = off; off = align_up(off + (size_t)g * 4, 256);
     const size_t o_info = off; off = align_up(off + (size_t)n_sets * sizeof(SetInfo), 256);
     const size_t o_seed = off; off = align_up(off + (size_t)n_sets * 8, 256);
-    if (ps[0].stream) {
+    if (arena) {
+        if (off > arena_bytes)
+            return fail(PROCELL_E_SCRATCH);
         pl->stream = (hipStream_t)ps[0].stream;
+        pl->d_block = arena;
+        pl->own_block = false;
     } else {
-        if (hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) != hipSuccess)
-            return fail(PROCELL_E_HIP);
-        pl->own_stream = true;
+        if (ps[0].stream) {
+            pl->stream = (hipStream_t)ps[0].stream;
+        } else {
+            if (hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) != hipSuccess)
+                return fail(PROCELL_E_HIP);
+            pl->own_stream = true;
+        }
+        if (hipMalloc(&pl->d_block, off) != hipSuccess)
+            return fail(PROCELL_E_OUT_OF_MEMORY);
     }
-    if (hipMalloc(&pl->d_block, off) != hipSuccess)
-        return fail(PROCELL_E_OUT_OF_MEMORY);
     char *dbase = (char *)pl->d_block;
     a.n_sets = n_sets;
     a.n_cells = n_cells;
@@ -839,6 +863,31 @@ int plan_result(procell_plan *pl, uint32_t set, procell_result *r)
     return PROCELL_OK;
 }
 
+// In host code `scratch` names a host-side stand-in, not device memory: the
+// device address of each GPU's copy comes from hipGetSymbolAddress, taken
+// once per device with that device current, and only that address is handed
+// to kernels and copies.
+constexpr int kScratchDevices = 64;
+
+std::mutex g_scratch_mu[kScratchDevices];  // held for a whole call on that device
+std::mutex g_scratch_addr_mu;
+void *g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of `device`'s scratch array; the device must be current.
+int scratch_address(int device, void **ptr)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_addr_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return PROCELL_E_HIP;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return PROCELL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -861,6 +910,7 @@ const char *procell_strerror(int code)
     case PROCELL_E_NO_DEVICE: return "no usable gfx950 device";
     case PROCELL_E_IO: return "cannot write output file";
     case PROCELL_E_OUT_OF_MEMORY: return "out of memory";
+    case PROCELL_E_SCRATCH: return "input needs more device memory than the 16 MiB static scratch of procell_simulate_noalloc";
     default: return "unknown error";
     }
 }
@@ -1087,6 +1137,34 @@ int procell_simulate(const procell_params *p, procell_result *r)
     q.stream = nullptr;
     procell_plan *pl = nullptr;
     int rc = create_plan(&q, 1, &pl);
+    if (rc != PROCELL_OK)
+        return rc;
+    rc = run_plan(pl, nullptr, p->seed);
+    if (rc == PROCELL_OK)
+        rc = plan_result(pl, 0, r);
+    destroy_plan(pl);
+    return rc;
+}
+
+int procell_simulate_noalloc(const procell_params *p, procell_result *r)
+{
+    if (!p || !r)
+        return PROCELL_E_INVALID;
+    int n_cu = 0;
+    int rc = check_device(p->device, &n_cu);  // makes the device current
+    if (rc != PROCELL_OK)
+        return rc;
+    if (p->device >= kScratchDevices)
+        return PROCELL_E_INVALID;
+    std::lock_guard<std::mutex> lk(g_scratch_mu[p->device]);
+    void *ptr = nullptr;
+    rc = scratch_address(p->device, &ptr);
+    if (rc != PROCELL_OK)
+        return rc;
+    procell_params q = *p;
+    q.tuning.flags &= ~PROCELL_FLAG_TIMING;
+    procell_plan *pl = nullptr;
+    rc = create_plan(&q, 1, &pl, ptr, kScratchBytes);
     if (rc != PROCELL_OK)
         return rc;
     rc = run_plan(pl, nullptr, p->seed);

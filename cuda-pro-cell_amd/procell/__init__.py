@@ -32,6 +32,7 @@ E_HIP = -9
 E_NO_DEVICE = -10
 E_IO = -11
 E_OUT_OF_MEMORY = -12
+E_SCRATCH = -13
 
 FLAG_TIMING = 1
 
@@ -95,7 +96,7 @@ class ProcellError(RuntimeError):
 _lib = None
 
 EXPORTS = (
-    "procell_simulate", "procell_output_capacity", "procell_tally_shape", "procell_resolve_phi",
+    "procell_simulate", "procell_simulate_noalloc", "procell_output_capacity", "procell_tally_shape", "procell_resolve_phi",
     "procell_strerror", "procell_abi_version", "procell_plan_create", "procell_plan_run",
     "procell_plan_result", "procell_plan_destroy", "procell_plan_stream", "procell_plan_kernel_times",
     "procell_read_histogram",
@@ -130,6 +131,7 @@ def lib():
     P = ctypes.POINTER(Params)
     R = ctypes.POINTER(ResultStruct)
     L.procell_simulate.argtypes = [P, R]
+    L.procell_simulate_noalloc.argtypes = [P, R]
     L.procell_output_capacity.argtypes = [P, _u64p]
     L.procell_tally_shape.argtypes = [P, _u64p, _u32p]
     L.procell_resolve_phi.argtypes = [P, _dp]
@@ -289,6 +291,14 @@ def simulate(prob: Problem, want_raw: bool = True) -> Result:
     """procell_simulate(): one end-to-end run on the GPU."""
     buf = _ResultBuffers(prob, want_raw)
     _check(lib().procell_simulate(ctypes.byref(prob.params), ctypes.byref(buf.r)), "simulate")
+    return buf.result()
+
+
+def simulate_noalloc(prob: Problem, want_raw: bool = True) -> Result:
+    """procell_simulate_noalloc(): procell_simulate with every device buffer in the
+    library's static per-GPU scratch array (no device allocation)."""
+    buf = _ResultBuffers(prob, want_raw)
+    _check(lib().procell_simulate_noalloc(ctypes.byref(prob.params), ctypes.byref(buf.r)), "simulate_noalloc")
     return buf.result()
 
 

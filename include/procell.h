@@ -45,6 +45,7 @@ extern "C" {
 #define PROCELL_E_NO_DEVICE (-10)     /* no usable gfx950 device */
 #define PROCELL_E_IO (-11)            /* file could not be written */
 #define PROCELL_E_OUT_OF_MEMORY (-12) /* host or device allocation failed */
+#define PROCELL_E_SCRATCH (-13)       /* needs more than PROCELL_NOALLOC_SCRATCH_BYTES (procell_simulate_noalloc) */
 
 /* One histogram row: fluorescence value and number of cells.
  * Replaces struct fluorescence (src/simulation/data_types.h:29-33). */
@@ -122,6 +123,23 @@ typedef struct {
  * reference, results are reproducible for a given seed, counts are 64-bit and
  * errors are returned instead of ignored. */
 int procell_simulate(const procell_params *p, procell_result *r);
+
+/* procell_simulate without device allocation: the same inputs, outputs,
+ * results (bit for bit) and errors, but every device buffer of the run (its
+ * tables, segments, counters and tally) is carved from one static device
+ * array of PROCELL_NOALLOC_SCRATCH_BYTES per GPU, which the library holds
+ * until the process ends.  It calls no hipMalloc/hipFree, creates no stream
+ * or event, and runs on p->stream (NULL: the default stream); host-side
+ * set-up uses ordinary heap memory as procell_simulate does.  An input whose
+ * device buffers exceed the array returns PROCELL_E_SCRATCH before any
+ * kernel runs.  Calls on one GPU are serialised by a lock inside the library
+ * (they would otherwise race on the shared array); the array keeps the last
+ * call's contents and is cleared where a run needs zeros.  PROCELL_FLAG_TIMING
+ * is ignored (no events).
+ * No reference counterpart: this is for callers that cannot allocate device
+ * memory (serving loops and harnesses that forbid hipMalloc). */
+#define PROCELL_NOALLOC_SCRATCH_BYTES (16u * 1024u * 1024u)
+int procell_simulate_noalloc(const procell_params *p, procell_result *r);
 
 /* Number of distinct output values (the output bin set: every v / 2^k >= phi
  * of every nonzero row), an upper bound for r->capacity.

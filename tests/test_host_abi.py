@@ -36,7 +36,7 @@ def _has_gpu():
 
 def test_library_exports_every_declared_function(procell_lib):
     declared = _header_functions(procell_lib.HEADER_PATH)
-    assert len(declared) == 39
+    assert len(declared) == 40
     L = procell_lib.lib()
     for name in declared:
         assert hasattr(L, name), name
@@ -52,13 +52,13 @@ def test_abi_version_and_strerror(procell_lib):
     ver = int(re.search(r"#define PROCELL_ABI_VERSION (\d+)", text).group(1))
     assert procell_lib.lib().procell_abi_version() == ver
     codes = {int(v): k for k, v in re.findall(r"#define (PROCELL_E_[A-Z_]+) \((-\d+)\)", text)}
-    assert sorted(codes) == list(range(-12, 0))
+    assert sorted(codes) == list(range(-13, 0))
     seen = set()
     for c in codes:
         msg = procell_lib.strerror(c)
         assert msg and msg != "unknown error"
         seen.add(msg)
-    assert len(seen) >= 11
+    assert len(seen) >= 12
     assert procell_lib.strerror(0) == "success"
     assert procell_lib.strerror(-99) == "unknown error"
     assert procell_lib.strerror(-2) == "proportion distribution of cell types does not sum to 1"
