@@ -30,7 +30,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE]
 
 HIP_SOURCES = ["procell_kernels.hip", "procell_api.hip", "procell_relaxed.hip"]
-CXX_SOURCES = ["procell_formats.cpp"]
+CXX_SOURCES = ["procell_formats.cpp", "procell_setup.cpp"]
 
 
 def _run(cmd, verbose):

@@ -1,6 +1,6 @@
-// procell_internal.hpp -- host-side problem set-up and the device launch
-// interface shared by the C-ABI (procell_api.hip) and the kernels
-// (procell_kernels.hip).
+// procell_internal.hpp -- the device launch interface shared by the C-ABI
+// (procell_api.hip) and the kernels (procell_kernels.hip); the host-only
+// set-up it builds on is in procell_setup.hpp.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,14 +9,9 @@
 #include <vector>
 
 #include "procell.h"
+#include "procell_setup.hpp"
 
 namespace procell {
-
-// Limits of the packed root entry (cell 32 bits, row 24, type 8).
-constexpr uint64_t kMaxCells = 0xFFFFFFFFull;
-constexpr uint64_t kMaxRows = 1ull << 24;
-constexpr uint32_t kMaxTypes = 256;
-constexpr int kMaxGeneration = 62;  // heap index of a node must fit 64 bits
 
 // Per-parameter-set values of a plan: one set for a plain run, many for a
 // batch.  Rows, initial cells, type-table entries, tally entries and
@@ -40,38 +35,9 @@ struct QueueEntry {
     uint32_t row_type;  // nonzero row | sorted type << 24
 };
 
-// Counters block in device memory (uint64 each).
-enum Counter : int {
-    kPoolCount = 0,   // initial cells that divided at least once
-    kDivisions = 1,
-    kRecorded = 2,    // unused on the device: derived by the host
-    kDropped = 3,     // unused on the device: derived by the host
-    kLost = 4,
-    kRejected = 5,
-    kCells = 6,
-    kNumCounters = 8,
-};
 // Copies of the counters block: workgroup b adds into copy b % kCounterCopies,
 // so the device atomics on any one counter stay few; the host sums the copies.
 constexpr uint32_t kCounterCopies = 64;
-
-// Problem after validation: nonzero rows in file order, types sorted by
-// descending proportion (src/io/parser.cu:184), output bin set.
-struct Setup {
-    std::vector<double> v;          // value per nonzero row
-    std::vector<uint64_t> start;    // first cell id per nonzero row, plus total at the end
-    std::vector<uint32_t> row_info; // generation cap | (value >= phi) << 8
-    uint64_t n_cells = 0;
-    uint32_t n_types = 0;
-    std::vector<uint32_t> type_file;  // sorted position -> file index
-    std::vector<double> type_cum, type_mean, type_sd;  // sorted order
-    double phi = 0.0;
-    double t_max = 0.0;
-    int kmax = 0;
-    std::vector<double> bins;       // output bin set, ascending, distinct
-};
-
-int build_setup(const procell_params *p, Setup &s, bool want_bins);
 
 // Arguments of one run, passed by value to the kernel.
 struct RunArgs {

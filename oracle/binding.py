@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libprocell_oracle.so")
+# PROCELL_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of oracle/build.py).
+LIB_PATH = os.environ.get("PROCELL_ORACLE_LIB") or os.path.join(_HERE, "_build", "libprocell_oracle.so")
 
 STAT_NAMES = ("divisions", "recorded", "dropped", "lost", "rejected", "cells")
 
