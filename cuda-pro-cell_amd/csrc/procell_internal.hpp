@@ -28,12 +28,18 @@ struct SetInfo {
 };
 
 // A dividing initial cell, buffered in LDS between a root step and the
-// hand-out to an idle lane.
+// hand-out to an idle lane, with every field the lane's state needs (three
+// 16-byte LDS reads, no dependent lookups).
 struct QueueEntry {
     double s0;          // completion time of its first division
-    uint32_t cell;      // initial cell id
-    uint32_t row_type;  // nonzero row | sorted type << 24
+    double mean, sd;    // timer parameters of its type
+    uint32_t cell;      // initial cell id, relative to the set
+    uint32_t tb_lds;    // tally window index of (row, generation 0, type), or none
+    uint32_t rowtf;     // row relative to the set's first row | file type << 24
+    uint32_t kcap;      // generation cap of the row
+    uint32_t pad[2];
 };
+static_assert(sizeof(QueueEntry) == 48, "three 16-byte LDS words");
 
 // Copies of the counters block: workgroup b adds into copy b % kCounterCopies,
 // so the device atomics on any one counter stay few; the host sums the copies.

@@ -149,7 +149,7 @@ template <typename N>
 struct Lane {
     uint64_t cell;
     N node, pend;
-    uint64_t tb_glob;   // device tally index of (row, generation 0, type)
+    uint32_t rowtf;     // row relative to the set's first row | file type << 24
     double s, mean, sd;
     double tl, tr;      // daughters' timers of the division in progress
     uint32_t k, kcap;
@@ -277,7 +277,8 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
     load_rows();
 
     Lane<N> L;
-    L.cell = L.tb_glob = 0;
+    L.cell = 0;
+    L.rowtf = 0;
     L.node = L.pend = 0;
     L.s = L.mean = L.sd = L.tl = L.tr = 0.0;
     L.k = L.kcap = 0;
@@ -289,10 +290,12 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
     const uint32_t idle_limit = a.idle_limit;
 
     for (;;) {
-        uint64_t idle = __ballot(!L.active);
-        uint32_t n_idle = uni((uint32_t)__popcll(idle));
-        // ======== root step: lanes are idle and the buffer is empty ========
-        if (n_idle && rpos >= rn && !roots_done) {
+        const uint64_t idle = __ballot(!L.active);
+        const uint32_t n_idle = uni((uint32_t)__popcll(idle));
+        // ======== refill once more than idle_limit lanes are idle ========
+        if (n_idle > idle_limit) {
+        // ---- root step: the buffer is empty ----
+        if (rpos >= rn && !roots_done) {
             // claim the segment's next root step
             uint32_t step = 0;
             if (lane == 0)
@@ -324,8 +327,8 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 // one block per initial cell: type uniform (word 0), phase
                 // (word 1) and the first timer's Box-Muller pair (words 2, 3)
                 const Words w = dev::draw_block(seed, c - S.cell_base, 0, 0);
-                const double ut = dev::uniform_co(w.w0, 0u);
-                const double U = dev::uniform_oc(w.w1, 0u);
+                const double ut = dev::uniform_co_hi(w.w0);
+                const double U = dev::uniform_oc_hi(w.w1);
                 // first type with ut < cumulative proportion (cell.cu:90-101):
                 // scanning from the highest index down, the last hit is the
                 // first; past the last proportion, the last type
@@ -344,7 +347,7 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 // first timer (cell.cu:106-122): two candidates from the root
                 // block, then node 1's redraw stream
                 double z0, z1;
-                dev::normal_pair(Words{w.w2, 0u, w.w3, 0u}, z0, z1, K, tb);
+                dev::normal_pair<true>(Words{w.w2, 0u, w.w3, 0u}, z0, z1, K, tb);
                 const double c0 = fma(z0, sdv, m);
                 const double c1 = fma(z1, sdv, m);
                 const bool live = valid && !quiescent;
@@ -402,40 +405,39 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 if (divides) {
                     QueueEntry &e = ent[lane_rank(dmask)];
                     e.s0 = s0;
+                    e.mean = m;
+                    e.sd = sdv;
                     e.cell = (uint32_t)(c - S.cell_base);
-                    e.row_type = row | (jt << 24);
+                    e.tb_lds = in_win ? woff * kdim * T + tf : kNoWindow;
+                    e.rowtf = (row - S.row_base) | (tf << 24);
+                    e.kcap = kc;
                 }
                 rpos = 0;
                 rn = uni((uint32_t)__popcll(dmask));
                 __builtin_amdgcn_wave_barrier();
             }
         }
-        // ======== hand buffered roots to idle lanes ========
-        if (rpos < rn && n_idle) {
+        // ---- hand buffered roots to idle lanes ----
+        if (rpos < rn) {
             const uint32_t rank = lane_rank(idle);
             const uint32_t avail = rn - rpos;
             if (!L.active && rank < avail) {
-                const QueueEntry e = ent[rpos + rank];
-                const uint32_t row = e.row_type & 0xFFFFFFu;
-                const uint32_t jt = e.row_type >> 24;
-                const uint32_t tf = t_file[jt];
-                const uint32_t off = row - row_lo;
+                const QueueEntry &e = ent[rpos + rank];
                 L.cell = e.cell;
                 L.s = e.s0;
+                L.mean = e.mean;
+                L.sd = e.sd;
                 L.node = (N)1;
                 L.k = 0u;
                 L.pend = (N)0;
                 L.redo = 0u;
-                L.kcap = ri[row] & 0xFFu;
-                const double2 msd = t_msd[jt];
-                L.mean = msd.x;
-                L.sd = msd.y;
-                L.tb_lds = off < a.win_rows ? off * kdim * T + tf : kNoWindow;
-                L.tb_glob = S.tally_base + (uint64_t)(row - S.row_base) * kdim * T + tf;
+                L.kcap = e.kcap;
+                L.tb_lds = e.tb_lds;
+                L.rowtf = e.rowtf;
                 L.active = true;
             }
             rpos = uni(rpos + min(n_idle, avail));
-        } else if (roots_done && n_idle) {
+        } else if (roots_done) {
             // ---- steal pending daughters inside the wavefront ----
             const uint64_t donors = __ballot(L.active && L.pend != 0);
             const uint32_t nd = uni(min((uint32_t)__popcll(donors), n_idle));
@@ -450,7 +452,7 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 const uint64_t d_cell = __shfl(L.cell, (int)src);
                 const uint32_t d_kcap = __shfl(L.kcap, (int)src);
                 const uint32_t d_tbl = __shfl(L.tb_lds, (int)src);
-                const uint64_t d_tbg = __shfl(L.tb_glob, (int)src);
+                const uint32_t d_rtf = __shfl(L.rowtf, (int)src);
                 const double d_mean = __shfl(L.mean, (int)src);
                 const double d_sd = __shfl(L.sd, (int)src);
                 if (L.active && L.pend && lane_rank(donors) < nd)
@@ -462,7 +464,7 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                     L.cell = d_cell;
                     L.kcap = d_kcap;
                     L.tb_lds = d_tbl;
-                    L.tb_glob = d_tbg;
+                    L.rowtf = d_rtf;
                     L.mean = d_mean;
                     L.sd = d_sd;
                     L.pend = 0;
@@ -477,12 +479,13 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 break;
             continue;
         }
+        }
 
-        // ======== division steps until enough lanes are idle ========
+        // ======== division step ========
         // Straight-line code: every outcome is a select, so the step costs a
         // handful of scalar instructions (one scalar unit serves the whole
         // CU).  Idle lanes run it too with their results discarded.
-        do {
+        {
             // One Philox block + Box-Muller pair per lane and step: either the
             // pair block of a new division (timers of both daughters) or one
             // block of a daughter's truncated-normal redraw stream
@@ -495,7 +498,7 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
             // pending level of the step's start; reading it here overlaps
             // the LDS round trip with the division step
             const uint32_t lvl = L.pend ? top_bit(L.pend) : 1u;
-            const double sp = stack[(size_t)(lvl - 1) * bdim + tid];
+            const double sp = stack[__umul24(lvl - 1, bdim) + tid];
             double tl, tr;
             uint32_t redo;
             if (__builtin_expect(__ballot(L.redo != 0) == 0, 1)) {
@@ -554,15 +557,17 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
             const uint32_t nrec = cmp * (rec_l + rec_r);
             const bool in_win = L.tb_lds != kNoWindow;
             // lanes without a completed division add 0 to the spare slot win[win_len]
-            atomicAdd(&win[(cmp && in_win) ? L.tb_lds + k1 * T : win_len], nrec);
+            atomicAdd(&win[(cmp && in_win) ? L.tb_lds + __umul24(k1, T) : win_len], nrec);
             if (__builtin_expect(__ballot(!in_win && nrec != 0) != 0, 0)) {
                 if (!in_win && nrec != 0)
-                    atomicAdd(&a.tally[L.tb_glob + (uint64_t)k1 * T], (unsigned long long)nrec);
+                    atomicAdd(&a.tally[S.tally_base + (uint64_t)(L.rowtf & 0xFFFFFFu) * kdim * T +
+                                       (L.rowtf >> 24) + (uint64_t)k1 * T],
+                              (unsigned long long)nrec);
             }
             // right daughter's completion time; level k1 holds no pending
             // entry of an active lane (idle lanes have none at all), so the
             // store is harmless when it is not pushed
-            stack[(size_t)(act ? k1 - 1 : 0u) * bdim + tid] = sr;
+            stack[__umul24(act ? L.k : 0u, bdim) + tid] = sr;
             const uint32_t go_l = cmp & div_l;
             const uint32_t go_r = cmp & (div_l ^ 1u) & div_r;
             const uint32_t pop = cmp & (div_l ^ 1u) & (div_r ^ 1u);
@@ -584,7 +589,7 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
             L.k = down ? k1 : k_up;
             L.pend = pend2 & ~clear;
             L.active = act && !(pop && !has);
-        } while (uni((uint32_t)__popcll(__ballot(!L.active))) <= idle_limit);
+        }
     }
 
     flush_counts(n, a.counters + (size_t)set * kCounterCopies * kNumCounters, lane, red);

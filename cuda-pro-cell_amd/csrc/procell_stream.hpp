@@ -69,21 +69,24 @@ __device__ __forceinline__ Words draw_block(uint64_t seed, uint64_t cell, uint64
                          (uint32_t)seed, (uint32_t)(seed >> 32));
 }
 
-// 53-bit mantissa hi * 2^21 + (lo >> 11), formed exactly.
-__device__ __forceinline__ double mantissa53(uint32_t hi, uint32_t lo)
-{
-    return fma((double)hi, 0x1p21, (double)(lo >> 11));
-}
-
+// Uniforms from the 53-bit integer m = hi * 2^21 + (lo >> 11):
+//   uniform_co = m 2^-53 in [0, 1),  uniform_oc = (m + 1) 2^-53 in (0, 1].
+// Each is formed with fmas whose exact results are representable (integers
+// below 2^53 scaled by 2^-53), so every step is exact and the value is that
+// of the defining formula; the _hi forms are the same values for lo = 0.
 __device__ __forceinline__ double uniform_co(uint32_t hi, uint32_t lo)  // [0, 1)
 {
-    return mantissa53(hi, lo) * 0x1p-53;
+    return fma((double)hi, 0x1p-32, (double)(lo >> 11) * 0x1p-53);
 }
 
 __device__ __forceinline__ double uniform_oc(uint32_t hi, uint32_t lo)  // (0, 1]
 {
-    return (mantissa53(hi, lo) + 1.0) * 0x1p-53;
+    return fma((double)hi, 0x1p-32, fma((double)(lo >> 11), 0x1p-53, 0x1p-53));
 }
+
+__device__ __forceinline__ double uniform_co_hi(uint32_t hi) { return (double)hi * 0x1p-32; }
+
+__device__ __forceinline__ double uniform_oc_hi(uint32_t hi) { return fma((double)hi, 0x1p-32, 0x1p-53); }
 
 #define PROCELL_TABLE_QUAL __constant__
 #include "procell_math_tables.inc"
@@ -133,13 +136,14 @@ __device__ __forceinline__ uint32_t log_index(double x)
 
 __device__ __forceinline__ double log_with_entry(double x, uint32_t j, double invc, double logc, const Coeffs &K)
 {
+    // x = 2^expo m with m in [1, 2), or in [0.75, 1) when the top fraction
+    // bit is set (j >= 64): then m is halved exactly by its exponent field
     const uint64_t bits = __builtin_bit_cast(uint64_t, x);
-    int expo = (int)(bits >> 52) - 1023;
-    double m = __builtin_bit_cast(double, (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
-    if (j >= 64u) {
-        m = m * 0.5;
-        expo = expo + 1;
-    }
+    const uint32_t hw = (uint32_t)(bits >> 32);
+    const uint32_t big = j >> 6;
+    const int expo = (int)(hw >> 20) - 1023 + (int)big;
+    const double m = __builtin_bit_cast(
+        double, ((uint64_t)((hw & 0x000FFFFFu) | ((0x3FFu - big) << 20)) << 32) | (uint32_t)bits);
     const double r = fma(m, invc, -1.0);
     const double r2 = r * r;
     double p = K.lp[5];
@@ -182,11 +186,15 @@ __device__ __forceinline__ uint32_t sincos_index(uint32_t hi)
     return ((hi >> 23) + 1u) >> 1;  // 0..256
 }
 
+template <bool LoZero = false>
 __device__ __forceinline__ void sincos_with_entry(uint32_t hi, uint32_t lo, uint32_t j, double sa, double ca,
                                                   double &sn_out, double &cs_out, const Coeffs &K)
 {
-    const double u = uniform_co(hi, lo);
-    const double d = u - (double)j * 0x1p-8;       // exact, |d| <= 1/512
+    // d = uniform_co(hi, lo) - j 2^-8, exact (|d| <= 1/512, a multiple of
+    // 2^-53): the inner fma's exact result (lo >> 11) 2^-53 - j 2^-8 and the
+    // outer one's are both representable
+    const double d = LoZero ? fma((double)hi, 0x1p-32, (double)j * -0x1p-8)
+                            : fma((double)hi, 0x1p-32, fma((double)j, -0x1p-8, (double)(lo >> 11) * 0x1p-53));
     const double t = fma(d, K.two_pi_hi, d * K.two_pi_lo);
     const double t2 = t * t;
     double ps = fma(t2, K.s3, K.s2);
@@ -207,16 +215,19 @@ __device__ __forceinline__ void sincos_turns(uint32_t hi, uint32_t lo, double &s
 // Box-Muller (Box & Muller 1958) pair from one block.  Both table entries
 // are read before any arithmetic that needs them, so the two LDS round trips
 // overlap each other and the uniforms' conversions.
+// LoZero: the block's words 1 and 3 are taken as 0 (a root's first timer
+// uses words 2 and 3 as (hi, 0) pairs).
+template <bool LoZero = false>
 __device__ __forceinline__ void normal_pair(const Words &w, double &z0, double &z1, const Coeffs &K,
                                             const Tables &tb)
 {
-    const double u1 = uniform_oc(w.w0, w.w1);
+    const double u1 = LoZero ? uniform_oc_hi(w.w0) : uniform_oc(w.w0, w.w1);
     const uint32_t jl = log_index(u1);
     const uint32_t js = sincos_index(w.w2);
     const double invc = tb.log[jl][0], logc = tb.log[jl][1];
     const double sa = tb.sincos[js & 255u][0], ca = tb.sincos[js & 255u][1];
     double sn, cs;
-    sincos_with_entry(w.w2, w.w3, js, sa, ca, sn, cs, K);
+    sincos_with_entry<LoZero>(w.w2, w.w3, js, sa, ca, sn, cs, K);
     const double rho = sqrt_nonneg(-2.0 * log_with_entry(u1, jl, invc, logc, K));
     z0 = rho * cs;
     z1 = rho * sn;
