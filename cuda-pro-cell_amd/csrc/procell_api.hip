@@ -365,7 +365,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     }
     ls.blocks = (int)g;
     a.win_rows = win_rows;
-    a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 16;
+    a.idle_limit = tu.idle_lanes ? std::min<uint32_t>(tu.idle_lanes, 63) : 12;
 
     // ---- one device allocation: counters | tally | tables ----
     size_t off = 0;
@@ -720,7 +720,7 @@ int procell_plan_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t re
     // threshold, which needs no new layout
     static const uint32_t kBlocks[] = {2, 3};
     static const uint32_t kSegments[] = {1, 2, 3, 4, 8};
-    static const uint32_t kIdle[] = {8, 12, 16, 24};
+    static const uint32_t kIdle[] = {6, 8, 12, 16};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {

@@ -171,9 +171,12 @@ struct Lane {
 //    the lanes hold: type, truncated first timer, phase.  Cells that never
 //    divide are tallied at generation 0 (or lost); dividing ones become
 //    16-byte entries of the wavefront's LDS buffer.
-//  * hand-outs: idle lanes take buffered entries; once the roots are done
-//    and the buffer is empty, idle lanes take the shallowest pending right
-//    daughter of a busy lane of the wavefront.
+//  * hand-outs: lanes that go idle take buffered entries at the end of the
+//    division step itself; once the roots are done and the buffer is empty,
+//    idle lanes take the shallowest pending right daughter of a busy lane of
+//    the wavefront.
+//  * refills (root steps, steals) run as a uniform branch of the one loop,
+//    once more than idle_limit lanes are idle.
 //  * division steps (proliferation.cu:317-381): each busy lane divides its
 //    current cell per step: one Philox block and one Box-Muller pair give
 //    both daughters' timers; daughters that will not divide are tallied or
@@ -289,196 +292,201 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
     Counts n = {0, 0, 0, 0, 0};
     const uint32_t idle_limit = a.idle_limit;
 
+    // idle lanes take buffered dividing roots, in lane order
+    auto hand_out = [&](uint64_t idle, uint32_t n_idle) {
+        const uint32_t rank = lane_rank(idle);
+        const uint32_t avail = rn - rpos;
+        if (!L.active && rank < avail) {
+            const QueueEntry &e = ent[rpos + rank];
+            L.cell = e.cell;
+            L.s = e.s0;
+            L.mean = e.mean;
+            L.sd = e.sd;
+            L.node = (N)1;
+            L.k = 0u;
+            L.pend = (N)0;
+            L.redo = 0u;
+            L.kcap = e.kcap;
+            L.tb_lds = e.tb_lds;
+            L.rowtf = e.rowtf;
+            L.active = true;
+        }
+        rpos = uni(rpos + min(n_idle, avail));
+    };
+
     for (;;) {
         const uint64_t idle = __ballot(!L.active);
         const uint32_t n_idle = uni((uint32_t)__popcll(idle));
         // ======== refill once more than idle_limit lanes are idle ========
         if (n_idle > idle_limit) {
-        // ---- root step: the buffer is empty ----
-        if (rpos >= rn && !roots_done) {
-            // claim the segment's next root step
-            uint32_t step = 0;
-            if (lane == 0)
-                step = atomicAdd(cursor, 1u);
-            const uint64_t c_off = (uint64_t)uni(__shfl(step, 0)) * kWave;
-            if (c_off >= seg_cells)
-                roots_done = 1;
-            if (!roots_done) {
-                const uint64_t c_cur = seg_begin + c_off;
-                const uint32_t n_valid = uni((uint32_t)min<uint64_t>(kWave, seg_cells - c_off));
-                const uint64_t c = c_cur + lane;
-                const bool valid = lane < n_valid;
-                // rows of the step's cells (cell ids < 2^32): move rb to the
-                // row of c_cur, then count the boundaries inside the step
-                const uint32_t c32 = (uint32_t)c_cur;
-                for (uint64_t le = __ballot(bnd <= c32); le; le = __ballot(bnd <= c32)) {
-                    rb = uni(rb + (uint32_t)__popcll(le));
-                    load_rows();
-                }
-                const uint32_t nb = uni((uint32_t)__popcll(__ballot((uint64_t)bnd < c_cur + n_valid)));
-                uint32_t rrel = 0;  // boundaries at or below this lane's cell
-                for (uint32_t j = 0; j < nb; ++j)
-                    rrel += lane >= (uint32_t)__builtin_amdgcn_readlane((int)bnd, (int)j) - c32 ? 1u : 0u;
-                const uint32_t row = rb + rrel;
-                const uint32_t info = nb ? (uint32_t)__shfl(rinfo, (int)rrel)
-                                         : (uint32_t)__builtin_amdgcn_readfirstlane((int)rinfo);
-                const uint32_t kc = info & 0xFFu;
-                const bool ge_phi = (info >> 8) & 1u;
-                // one block per initial cell: type uniform (word 0), phase
-                // (word 1) and the first timer's Box-Muller pair (words 2, 3)
-                const Words w = dev::draw_block(seed, c - S.cell_base, 0, 0);
-                const double ut = dev::uniform_co_hi(w.w0);
-                const double U = dev::uniform_oc_hi(w.w1);
-                // first type with ut < cumulative proportion (cell.cu:90-101):
-                // scanning from the highest index down, the last hit is the
-                // first; past the last proportion, the last type
-                uint32_t jt = T - 1;
-                if (T > 9) {
-                    for (uint32_t k = T - 2; k >= 8; --k)
-                        jt = ut < t_cum[k] ? k : jt;
-                }
-#pragma unroll
-                for (int k = 7; k >= 0; --k)
-                    jt = ut < cum8[k] ? (uint32_t)k : jt;
-                const double2 msd = t_msd[jt];
-                const double m = msd.x;
-                const double sdv = msd.y;
-                const bool quiescent = m < 0.0;
-                // first timer (cell.cu:106-122): two candidates from the root
-                // block, then node 1's redraw stream
-                double z0, z1;
-                dev::normal_pair<true>(Words{w.w2, 0u, w.w3, 0u}, z0, z1, K, tb);
-                const double c0 = fma(z0, sdv, m);
-                const double c1 = fma(z1, sdv, m);
-                const bool live = valid && !quiescent;
-                n.rej += live ? (uint32_t)!(c0 > 0.0) + (uint32_t)(!(c0 > 0.0) && !(c1 > 0.0)) : 0u;
-                double tau0 = c0 > 0.0 ? c0 : c1;
-                if (__builtin_expect(__ballot(live && !(tau0 > 0.0)) != 0, 0)) {
-                    if (live && !(tau0 > 0.0)) {
-                        // rare: node 1's redraw stream from attempt 1
-                        tau0 = m;
-                        for (uint32_t att = 1; att < 65536u; ++att) {
-                            const Words wr = dev::draw_block(seed, c - S.cell_base, 1, att);
-                            double y0, y1;
-                            dev::normal_pair(wr, y0, y1, K, tb);
-                            const double d0 = fma(y0, sdv, m);
-                            if (d0 > 0.0) { tau0 = d0; break; }
-                            ++n.rej;
-                            const double d1 = fma(y1, sdv, m);
-                            if (d1 > 0.0) { tau0 = d1; break; }
-                            ++n.rej;
+            // ---- root step: the buffer is empty ----
+            if (rpos >= rn && !roots_done) {
+                // claim the segment's next root step
+                uint32_t step = 0;
+                if (lane == 0)
+                    step = atomicAdd(cursor, 1u);
+                const uint64_t c_off = (uint64_t)uni(__shfl(step, 0)) * kWave;
+                if (c_off >= seg_cells)
+                    roots_done = 1;
+                if (!roots_done) {
+                    const uint64_t c_cur = seg_begin + c_off;
+                    const uint32_t n_valid = uni((uint32_t)min<uint64_t>(kWave, seg_cells - c_off));
+                    const uint64_t c = c_cur + lane;
+                    const bool valid = lane < n_valid;
+                    // rows of the step's cells (cell ids < 2^32): move rb to the
+                    // row of c_cur, then count the boundaries inside the step
+                    const uint32_t c32 = (uint32_t)c_cur;
+                    for (uint64_t le = __ballot(bnd <= c32); le; le = __ballot(bnd <= c32)) {
+                        rb = uni(rb + (uint32_t)__popcll(le));
+                        load_rows();
+                    }
+                    const uint32_t nb = uni((uint32_t)__popcll(__ballot((uint64_t)bnd < c_cur + n_valid)));
+                    uint32_t rrel = 0;  // boundaries at or below this lane's cell
+                    for (uint32_t j = 0; j < nb; ++j)
+                        rrel += lane >= (uint32_t)__builtin_amdgcn_readlane((int)bnd, (int)j) - c32 ? 1u : 0u;
+                    const uint32_t row = rb + rrel;
+                    const uint32_t info = nb ? (uint32_t)__shfl(rinfo, (int)rrel)
+                                             : (uint32_t)__builtin_amdgcn_readfirstlane((int)rinfo);
+                    const uint32_t kc = info & 0xFFu;
+                    const bool ge_phi = (info >> 8) & 1u;
+                    // one block per initial cell: type uniform (word 0), phase
+                    // (word 1) and the first timer's Box-Muller pair (words 2, 3)
+                    const Words w = dev::draw_block(seed, c - S.cell_base, 0, 0);
+                    const double ut = dev::uniform_co_hi(w.w0);
+                    const double U = dev::uniform_oc_hi(w.w1);
+                    // first type with ut < cumulative proportion (cell.cu:90-101):
+                    // scanning from the highest index down, the last hit is the
+                    // first; past the last proportion, the last type
+                    uint32_t jt = T - 1;
+                    if (T > 9) {
+                        for (uint32_t k = T - 2; k >= 8; --k)
+                            jt = ut < t_cum[k] ? k : jt;
+                    }
+    #pragma unroll
+                    for (int k = 7; k >= 0; --k)
+                        jt = ut < cum8[k] ? (uint32_t)k : jt;
+                    const double2 msd = t_msd[jt];
+                    const double m = msd.x;
+                    const double sdv = msd.y;
+                    const bool quiescent = m < 0.0;
+                    // first timer (cell.cu:106-122): two candidates from the root
+                    // block, then node 1's redraw stream
+                    double z0, z1;
+                    dev::normal_pair<true>(Words{w.w2, 0u, w.w3, 0u}, z0, z1, K, tb);
+                    const double c0 = fma(z0, sdv, m);
+                    const double c1 = fma(z1, sdv, m);
+                    const bool live = valid && !quiescent;
+                    n.rej += live ? (uint32_t)!(c0 > 0.0) + (uint32_t)(!(c0 > 0.0) && !(c1 > 0.0)) : 0u;
+                    double tau0 = c0 > 0.0 ? c0 : c1;
+                    if (__builtin_expect(__ballot(live && !(tau0 > 0.0)) != 0, 0)) {
+                        if (live && !(tau0 > 0.0)) {
+                            // rare: node 1's redraw stream from attempt 1
+                            tau0 = m;
+                            for (uint32_t att = 1; att < 65536u; ++att) {
+                                const Words wr = dev::draw_block(seed, c - S.cell_base, 1, att);
+                                double y0, y1;
+                                dev::normal_pair(wr, y0, y1, K, tb);
+                                const double d0 = fma(y0, sdv, m);
+                                if (d0 > 0.0) { tau0 = d0; break; }
+                                ++n.rej;
+                                const double d1 = fma(y1, sdv, m);
+                                if (d1 > 0.0) { tau0 = d1; break; }
+                                ++n.rej;
+                            }
                         }
                     }
-                }
-                const double t0 = tau0 * U;
-                const double s0 = t0 + tau0;
-                // quiescent or out of time: tallied if its value is in the
-                // output bin set; in time with f/2 <= phi: dropped; else it
-                // divides
-                const bool record = valid && (quiescent || s0 > t_max);
-                const bool rec_ok = record && ge_phi;
-                const bool divides = valid && !record && kc != 0;
-                n.cells += valid ? 1u : 0u;
-                n.lost += record && !ge_phi ? 1u : 0u;
-                n.pool += divides ? 1u : 0u;
-                // generation-0 tallies: the window, else one device atomic
-                // per distinct (row, type) of the step
-                const uint32_t tf = t_file[jt];
-                const uint32_t woff = row - row_lo;
-                const bool in_win = woff < a.win_rows;
-                if (rec_ok && in_win) {
-                    atomicAdd(&win[woff * kdim * T + tf], 1u);
-                }
-                uint64_t pend = __ballot(rec_ok && !in_win);
-                while (pend) {
-                    const uint32_t leader = (uint32_t)__builtin_ctzll(pend);
-                    const uint32_t row_l = uni(__shfl(row, (int)leader));
-                    const uint32_t tf_l = uni(__shfl(tf, (int)leader));
-                    const uint64_t same = __ballot(rec_ok && !in_win && row == row_l && tf == tf_l);
-                    if (lane == leader)
-                        atomicAdd(&a.tally[S.tally_base + (uint64_t)(row_l - S.row_base) * kdim * T + tf_l],
-                                  (unsigned long long)__popcll(same));
-                    pend &= ~same;
-                }
-                // dividing cells: buffered for the hand-outs
-                const uint64_t dmask = __ballot(divides);
-                if (divides) {
-                    QueueEntry &e = ent[lane_rank(dmask)];
-                    e.s0 = s0;
-                    e.mean = m;
-                    e.sd = sdv;
-                    e.cell = (uint32_t)(c - S.cell_base);
-                    e.tb_lds = in_win ? woff * kdim * T + tf : kNoWindow;
-                    e.rowtf = (row - S.row_base) | (tf << 24);
-                    e.kcap = kc;
-                }
-                rpos = 0;
-                rn = uni((uint32_t)__popcll(dmask));
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
-        // ---- hand buffered roots to idle lanes ----
-        if (rpos < rn) {
-            const uint32_t rank = lane_rank(idle);
-            const uint32_t avail = rn - rpos;
-            if (!L.active && rank < avail) {
-                const QueueEntry &e = ent[rpos + rank];
-                L.cell = e.cell;
-                L.s = e.s0;
-                L.mean = e.mean;
-                L.sd = e.sd;
-                L.node = (N)1;
-                L.k = 0u;
-                L.pend = (N)0;
-                L.redo = 0u;
-                L.kcap = e.kcap;
-                L.tb_lds = e.tb_lds;
-                L.rowtf = e.rowtf;
-                L.active = true;
-            }
-            rpos = uni(rpos + min(n_idle, avail));
-        } else if (roots_done) {
-            // ---- steal pending daughters inside the wavefront ----
-            const uint64_t donors = __ballot(L.active && L.pend != 0);
-            const uint32_t nd = uni(min((uint32_t)__popcll(donors), n_idle));
-            if (nd) {
-                const uint32_t my_lvl = L.pend ? low_bit(L.pend) : 0u;
-                const uint32_t r = lane_rank(idle);
-                const bool thief = !L.active && r < nd;
-                const uint32_t src = thief ? nth_set_bit(donors, r) : lane;
-                const uint32_t lvl = __shfl(my_lvl, (int)src);
-                const N d_node = __shfl(L.node, (int)src);
-                const uint32_t d_k = __shfl(L.k, (int)src);
-                const uint64_t d_cell = __shfl(L.cell, (int)src);
-                const uint32_t d_kcap = __shfl(L.kcap, (int)src);
-                const uint32_t d_tbl = __shfl(L.tb_lds, (int)src);
-                const uint32_t d_rtf = __shfl(L.rowtf, (int)src);
-                const double d_mean = __shfl(L.mean, (int)src);
-                const double d_sd = __shfl(L.sd, (int)src);
-                if (L.active && L.pend && lane_rank(donors) < nd)
-                    L.pend &= ~((N)1 << my_lvl);
-                if (thief) {
-                    L.s = stack[(size_t)(lvl - 1) * bdim + wave * kWave + src];
-                    L.node = (d_node >> (d_k - lvl)) | (N)1;
-                    L.k = lvl;
-                    L.cell = d_cell;
-                    L.kcap = d_kcap;
-                    L.tb_lds = d_tbl;
-                    L.rowtf = d_rtf;
-                    L.mean = d_mean;
-                    L.sd = d_sd;
-                    L.pend = 0;
-                    L.redo = 0;
-                    L.active = true;
+                    const double t0 = tau0 * U;
+                    const double s0 = t0 + tau0;
+                    // quiescent or out of time: tallied if its value is in the
+                    // output bin set; in time with f/2 <= phi: dropped; else it
+                    // divides
+                    const bool record = valid && (quiescent || s0 > t_max);
+                    const bool rec_ok = record && ge_phi;
+                    const bool divides = valid && !record && kc != 0;
+                    n.cells += valid ? 1u : 0u;
+                    n.lost += record && !ge_phi ? 1u : 0u;
+                    n.pool += divides ? 1u : 0u;
+                    // generation-0 tallies: the window, else one device atomic
+                    // per distinct (row, type) of the step
+                    const uint32_t tf = t_file[jt];
+                    const uint32_t woff = row - row_lo;
+                    const bool in_win = woff < a.win_rows;
+                    if (rec_ok && in_win) {
+                        atomicAdd(&win[woff * kdim * T + tf], 1u);
+                    }
+                    uint64_t pend = __ballot(rec_ok && !in_win);
+                    while (pend) {
+                        const uint32_t leader = (uint32_t)__builtin_ctzll(pend);
+                        const uint32_t row_l = uni(__shfl(row, (int)leader));
+                        const uint32_t tf_l = uni(__shfl(tf, (int)leader));
+                        const uint64_t same = __ballot(rec_ok && !in_win && row == row_l && tf == tf_l);
+                        if (lane == leader)
+                            atomicAdd(&a.tally[S.tally_base + (uint64_t)(row_l - S.row_base) * kdim * T + tf_l],
+                                      (unsigned long long)__popcll(same));
+                        pend &= ~same;
+                    }
+                    // dividing cells: buffered for the hand-outs
+                    const uint64_t dmask = __ballot(divides);
+                    if (divides) {
+                        QueueEntry &e = ent[lane_rank(dmask)];
+                        e.s0 = s0;
+                        e.mean = m;
+                        e.sd = sdv;
+                        e.cell = (uint32_t)(c - S.cell_base);
+                        e.tb_lds = in_win ? woff * kdim * T + tf : kNoWindow;
+                        e.rowtf = (row - S.row_base) | (tf << 24);
+                        e.kcap = kc;
+                    }
+                    rpos = 0;
+                    rn = uni((uint32_t)__popcll(dmask));
+                    __builtin_amdgcn_wave_barrier();
                 }
             }
-        }
+            // ---- hand buffered roots to idle lanes ----
+            if (rpos < rn) {
+                hand_out(idle, n_idle);
+            } else if (roots_done) {
+                // ---- steal pending daughters inside the wavefront ----
+                const uint64_t donors = __ballot(L.active && L.pend != 0);
+                const uint32_t nd = uni(min((uint32_t)__popcll(donors), n_idle));
+                if (nd) {
+                    const uint32_t my_lvl = L.pend ? low_bit(L.pend) : 0u;
+                    const uint32_t r = lane_rank(idle);
+                    const bool thief = !L.active && r < nd;
+                    const uint32_t src = thief ? nth_set_bit(donors, r) : lane;
+                    const uint32_t lvl = __shfl(my_lvl, (int)src);
+                    const N d_node = __shfl(L.node, (int)src);
+                    const uint32_t d_k = __shfl(L.k, (int)src);
+                    const uint64_t d_cell = __shfl(L.cell, (int)src);
+                    const uint32_t d_kcap = __shfl(L.kcap, (int)src);
+                    const uint32_t d_tbl = __shfl(L.tb_lds, (int)src);
+                    const uint32_t d_rtf = __shfl(L.rowtf, (int)src);
+                    const double d_mean = __shfl(L.mean, (int)src);
+                    const double d_sd = __shfl(L.sd, (int)src);
+                    if (L.active && L.pend && lane_rank(donors) < nd)
+                        L.pend &= ~((N)1 << my_lvl);
+                    if (thief) {
+                        L.s = stack[(size_t)(lvl - 1) * bdim + wave * kWave + src];
+                        L.node = (d_node >> (d_k - lvl)) | (N)1;
+                        L.k = lvl;
+                        L.cell = d_cell;
+                        L.kcap = d_kcap;
+                        L.tb_lds = d_tbl;
+                        L.rowtf = d_rtf;
+                        L.mean = d_mean;
+                        L.sd = d_sd;
+                        L.pend = 0;
+                        L.redo = 0;
+                        L.active = true;
+                    }
+                }
+            }
 
-        if (__ballot(L.active) == 0) {
-            if (roots_done && rpos >= rn)
-                break;
-            continue;
-        }
+            if (__ballot(L.active) == 0) {
+                if (roots_done && rpos >= rn)
+                    break;
+                continue;
+            }
         }
 
         // ======== division step ========
@@ -589,6 +597,11 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
             L.k = down ? k1 : k_up;
             L.pend = pend2 & ~clear;
             L.active = act && !(pop && !has);
+            // lanes that went idle take buffered roots at once
+            if (rpos < rn) {
+                const uint64_t idle2 = __ballot(!L.active);
+                hand_out(idle2, uni((uint32_t)__popcll(idle2)));
+            }
         }
     }
 

@@ -72,7 +72,17 @@ constexpr double kLogMaxSdRatio = -0.22314355131420976;
 // A stacked node: s, W, d (both passes), b, dlogW/dmean, dlogW/dsd (adjoint),
 // padded to one 64-byte line so that a push or a pop of a lane moves one line.
 constexpr uint32_t kFrameDoubles = 8;
+// The adjoint pass (without a cell stride) keeps its two most recently pushed
+// frames (the six tangent fields) in LDS, field-major so that a wavefront's
+// accesses are bank-conflict free; only older frames reach the device stack.
+// Its cotangent window is smaller, so that window and cache fit the LDS of
+// four wavefronts per SIMD.  256 sets x 2e5 cells: adjoint 13.4 -> 12.6 ms,
+// calibration step 23.3 -> 22.5 ms (one slot, or the cache in the Jacobian
+// pass as well, measured slower: they cost a wavefront per SIMD).
+constexpr uint32_t kTanFields = 6;
+constexpr uint32_t kTanSlots = 2;
 constexpr uint32_t kWindowDoubles = 1024;  // LDS window of a wavefront (8 KB)
+constexpr uint32_t kWindowAdj = 384;       // window of the adjoint pass beside its frame cache (3 KB)
 constexpr uint32_t kRowCap = 64;           // rows of a wavefront's range staged in LDS
 // Wavefronts of a plan over several sets, in resident wavefronts: short
 // ranges cover few rows, so the tallies stay in the LDS window (the
@@ -184,9 +194,11 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     const uint32_t win_n = (r_hi - r_lo + 1) * KT;
     // window: h (forward), g (adjoint) or h | dh/dmean | dh/dsd (Jacobian)
     const uint32_t win_t = Jac ? 3 * win_n : win_n;
-    const bool in_lds = win_t <= kWindowDoubles;
+    constexpr bool kLdsCache = Grad && !Strided;  // frames cached in LDS
+    constexpr uint32_t kWin = kLdsCache ? kWindowAdj : kWindowDoubles;
+    const bool in_lds = win_t <= kWin;
     const uint64_t hw0 = S.h_base + (uint64_t)(r_lo - S.row_base) * KT;  // tally index of the window
-    double *gsum = lds + kWindowDoubles;
+    double *gsum = lds + kWin;
     double *theta = gsum + 3 * T;
     uint32_t *rs = (uint32_t *)(theta + 3 * T);
     uint32_t *ri = rs + kRowCap + 1;
@@ -220,6 +232,9 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     const double t_max = S.t_max, ib = 1.0 / a.beta;
     const uint32_t n_prolif = S.n_prolif;
     double *stk = a.stack + ((uint64_t)w * 64 + lane) * a.kdim_max * kFrameDoubles;
+    // LDS frame cache of the adjoint pass: field f of slot q of this lane
+    double *fc = (double *)(((uintptr_t)(prolif + T) + 7) & ~(uintptr_t)7) + lane;
+    auto fcp = [&](uint32_t q, uint32_t f) -> double & { return fc[(q * kTanFields + f) * 64]; };
 
     bool active = false;
     uint64_t cell = 0, node = 0, pending = 0;
@@ -240,6 +255,10 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     bool c_ok = false;
     uint32_t c_lvl = 0;
     double c_s = 0.0, c_W = 0.0, c_d = 0.0;
+    // adjoint pass: frames cached in LDS slots (c_n of them, c_top the
+    // deepest); the levels of slots 0 and 1 in t_lvl0, t_lvl1
+    static_assert(kTanSlots == 1 || kTanSlots == 2, "one or two cached frames");
+    uint32_t c_n = 0, c_top = 0, t_lvl0 = 0, t_lvl1 = 0;
     const uint32_t smask = Strided ? a.stride_mask : 0u;
     const uint32_t sphase = Strided ? ((a.step ? (uint32_t)a.step[0] : 0u) + a.phase) & smask : 0u;
 
@@ -270,6 +289,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
                     node = 0;  // the next step draws the root from block (cell, 0, 0)
                     pending = 0;
                     c_ok = false;
+                    c_n = 0;
                     walk_exp = 0;
                     active = true;
                 }
@@ -407,7 +427,27 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
             c_W = WA;
             c_d = dB;
             pending |= 1ull << kA;
-        } else if (!kCache && expA && expB) {
+        } else if (kLdsCache && expA && expB) {
+            if (c_n == kTanSlots) {  // spill the oldest cached frame to its level
+                const uint32_t bot = kTanSlots == 1 ? 0u : c_top ^ 1u;
+                double2 *f = (double2 *)(stk + (uint64_t)(bot ? t_lvl1 : t_lvl0) * kFrameDoubles);
+                f[0] = make_double2(fcp(bot, 0), fcp(bot, 1));
+                f[1] = make_double2(fcp(bot, 2), fcp(bot, 3));
+                f[2] = make_double2(fcp(bot, 4), fcp(bot, 5));
+                --c_n;
+            }
+            c_top = kTanSlots == 1 ? 0u : c_top ^ 1u;
+            fcp(c_top, 0) = sB;
+            fcp(c_top, 1) = WA;
+            fcp(c_top, 2) = dB;
+            fcp(c_top, 3) = bB;
+            fcp(c_top, 4) = gmA;
+            fcp(c_top, 5) = gsA;
+            t_lvl0 = c_top ? t_lvl0 : kA;
+            t_lvl1 = c_top ? kA : t_lvl1;
+            ++c_n;
+            pending |= 1ull << kA;
+        } else if (expA && expB) {
             double2 *f = (double2 *)(stk + (uint64_t)kA * kFrameDoubles);
             f[0] = make_double2(sB, WA);
             f[1] = make_double2(dB, bB);
@@ -434,6 +474,15 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
                 W = c_W;
                 d = c_d;
                 c_ok = false;
+            } else if (kLdsCache && c_n) {  // the deepest cached frame is level L
+                s = fcp(c_top, 0);
+                W = fcp(c_top, 1);
+                d = fcp(c_top, 2);
+                b = fcp(c_top, 3);
+                gm = fcp(c_top, 4);
+                gs = fcp(c_top, 5);
+                c_top = kTanSlots == 1 ? 0u : c_top ^ 1u;
+                --c_n;
             } else {
                 const double2 *f = (const double2 *)(stk + (uint64_t)L * kFrameDoubles);
                 const double2 f0 = f[0];
@@ -557,9 +606,12 @@ thread_local hipError_t g_relaxed_hip = hipSuccess;
         }                                    \
     } while (0)
 
-size_t relaxed_lds_bytes(uint32_t n_types)
+// adj: the adjoint kernel without a cell stride (smaller window, frame cache)
+size_t relaxed_lds_bytes(uint32_t n_types, bool adj)
 {
-    return (kWindowDoubles + 6ull * n_types) * sizeof(double) + (2ull * kRowCap + 1 + n_types) * sizeof(uint32_t);
+    const size_t base = ((adj ? kWindowAdj : kWindowDoubles) + 6ull * n_types) * sizeof(double) +
+                        (2ull * kRowCap + 1 + n_types) * sizeof(uint32_t);
+    return adj ? (base + 7) / 8 * 8 + (size_t)kTanSlots * kTanFields * 64 * sizeof(double) : base;
 }
 
 // Expected expansions of one walk of a type with this mean in a row with
@@ -597,7 +649,8 @@ struct RelaxCore {
     std::vector<RelaxSet> sets;
     std::vector<uint32_t> quiet_flag;  // per type of every set: 1 = quiescent
     uint32_t n_waves = 0;
-    size_t lds_bytes = 0;
+    size_t lds_bytes = 0;  // every walk kernel but the adjoint without a stride
+    size_t lds_adj = 0;    // the adjoint without a stride
     bool any_prolif = false, any_quiet = false;
     uint64_t h_total = 0;      // tally entries of all sets
     uint32_t types_total = 0;  // types of all sets
@@ -739,10 +792,11 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     row_start.push_back((uint32_t)cells);
 
     // ---- wavefronts ----
-    c.lds_bytes = relaxed_lds_bytes(t_max_types);
+    c.lds_bytes = relaxed_lds_bytes(t_max_types, false);
+    c.lds_adj = relaxed_lds_bytes(t_max_types, true);
     int occ = 0, occ_f = 0;
     int occ_s = 0, occ_sf = 0;
-    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, relaxed_kernel<true>, 64, c.lds_bytes));
+    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, relaxed_kernel<true>, 64, c.lds_adj));
     RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, relaxed_kernel<false>, 64, c.lds_bytes));
     RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, relaxed_strided_kernel<true>, 64, c.lds_bytes));
     RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_sf, relaxed_strided_kernel<false>, 64, c.lds_bytes));
@@ -871,7 +925,7 @@ hipError_t core_launch(const RelaxCore &c, const RelaxArgs &a, int mode, hipStre
             else
                 hipLaunchKernelGGL(relaxed_strided_kernel<false>, g, b, c.lds_bytes, st, a);
         } else if (grad) {
-            hipLaunchKernelGGL(relaxed_kernel<true>, g, b, c.lds_bytes, st, a);
+            hipLaunchKernelGGL(relaxed_kernel<true>, g, b, c.lds_adj, st, a);
         } else {
             hipLaunchKernelGGL(relaxed_kernel<false>, g, b, c.lds_bytes, st, a);
         }
