@@ -315,7 +315,7 @@ def recovery(np, gen, thetas):
 
 
 def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_rounds=300, with_nm=True,
-                            with_cpu=True, lm_iters=15):
+                            with_cpu=True, lm_iters=15, rich_sets=64):
     """Batched calibration: 256 candidate parameter sets x 2e5 cells, 2-4
     types, t_max U(96,240), targets from the plain method at the generating
     parameters; 200 steps of relaxed forward + loss + adjoint + on-device
@@ -385,33 +385,55 @@ def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_roun
                       "note": "forward pass on the cells of phase step mod 4, adjoint on phase step + 1 mod 4 "
                               "(double sampling: an unbiased gradient of the squared loss)"}
     if lm_iters:
-        out["levenberg_marquardt"] = lm_measurement(procell, calibrate, np, gen, probs, targets, lm_iters, rec0,
+        out["levenberg_marquardt"] = lm_measurement(procell, calibrate, np, gen, start, targets, lm_iters, rec0,
                                                     traj)
+        out["levenberg_marquardt_strided"] = lm_measurement(procell, calibrate, np, gen, start, targets, lm_iters,
+                                                            rec0, traj, stride=4)
     if with_nm:
-        out["derivative_free"] = nelder_mead_baseline(procell, calibrate, np, gen, start, targets, nm_rounds, traj,
-                                                      straj, out.get("levenberg_marquardt", {}).get("trajectory"))
+        out["derivative_free"] = nelder_mead_baseline(
+            procell, calibrate, np, gen, start, targets, nm_rounds, traj, straj,
+            out.get("levenberg_marquardt", {}).get("trajectory"),
+            out.get("levenberg_marquardt_strided", {}).get("trajectory"))
+    if with_nm and lm_iters and rich_sets:
+        out["rich_target"] = rich_target_measurement(procell, calibrate, np, workloads, gen, start, targets,
+                                                     rich_sets, lm_iters, nm_rounds)
     if with_cpu:
         out["cpu_baseline"] = calibration_cpu_baseline(calibrate, np, start, targets)
     return out
 
 
-def lm_measurement(procell, calibrate, np, gen, probs, targets, iters, rec0, adam_traj):
-    """Levenberg-Marquardt per set over one batch relaxed plan of all sets:
-    each iteration = one Jacobian pass (h, dh/dmean, dh/dsd) of every set's
-    trial point + batched torch algebra on the device; the start point's
-    pass is counted in the first iteration.  gpu_s = wall time around the
-    iterations with a device synchronisation after each, after one untimed
-    iteration of a throw-away instance (torch's first-call set-up of its
-    BLAS and index kernels)."""
+def lm_run(procell, calibrate, np, gen, start, targets, iters, rec0, stride=1, replicas=1, seed_base=50000,
+           beta=0.5, thetas=None, gpu0=0.0):
+    """Levenberg-Marquardt per set over one batch relaxed plan: each
+    iteration = one Jacobian pass (h, dh/dmean, dh/dsd) of every set's trial
+    point + batched torch algebra on the device; the start point's pass is
+    counted in the first iteration.  stride > 1: every pass walks the
+    initial cells of one fixed residue class (the normalised loss does not
+    see the scale).  replicas > 1: the plan holds that many copies of each
+    set with other relaxed-stream seeds and the estimator is their average.
+    thetas: per set [3, n_types] (mean, sd, proportion) to start from instead
+    of the start workloads' parameters.  gpu_s = wall time around the
+    iterations with a device synchronisation after each (from gpu0), after
+    one untimed iteration of a throw-away instance (torch's first-call set-up
+    of its BLAS and index kernels).  Returns the trajectory, the GPU seconds,
+    whether a walk was cut and the final parameters."""
     import torch
 
-    plan = procell.RelaxedBatch(probs, 0.5)
-    warm = calibrate.BatchLM(plan, probs, targets)
+    n = len(start)
+    probs = []
+    for r in range(replicas):
+        for i, w in enumerate(start):
+            m, sd, p = (w.mean, w.sd, w.proportion) if thetas is None else thetas[i]
+            probs.append(procell.Problem(w.values, w.counts, p, m, sd, w.t_max, w.phi, seed_base + 1000 * r + i))
+    plan = procell.RelaxedBatch(probs, beta)
+    if stride > 1:
+        plan.set_stride(stride, 0)
+    warm = calibrate.BatchLM(plan, probs[:n], targets, replicas=replicas)
     warm.step(1)
     del warm
-    lm = calibrate.BatchLM(plan, probs, targets)
+    lm = calibrate.BatchLM(plan, probs[:n], targets, replicas=replicas)
     torch.cuda.synchronize()
-    traj, gpu = [{"steps": 0, "gpu_s": 0.0, **rec0}], 0.0
+    traj, gpu = [{"steps": 0, "gpu_s": gpu0, **rec0}], gpu0
     for it in range(iters):
         t0 = time.perf_counter()
         lm.step(1)
@@ -419,25 +441,34 @@ def lm_measurement(procell, calibrate, np, gen, probs, targets, iters, rec0, ada
         gpu += time.perf_counter() - t0
         traj.append({"steps": it + 1, "gpu_s": gpu, "loss_median": float(lm.loss.median()),
                      **recovery(np, gen, lm.thetas())})
+    final = [np.array(t) for t in lm.thetas()]
     try:
         plan.expansions()
         cut = False
     except procell.ProcellError:
         cut = True
     plan.close()
+    return traj, gpu, cut, final
+
+
+def lm_measurement(procell, calibrate, np, gen, start, targets, iters, rec0, adam_traj, stride=1):
+    """Levenberg-Marquardt over all sets of the calibration workload (lm_run),
+    and the GPU time it needs to reach the loss the Adam run ends at."""
+    traj, gpu, cut, _ = lm_run(procell, calibrate, np, gen, start, targets, iters, rec0, stride=stride)
     # GPU time to the median loss Adam ends at (same loss, same relaxed stream seeds)
     goal = adam_traj[-1]["loss_median"]
     t_lm = next((p["gpu_s"] for p in traj[1:] if p["loss_median"] <= goal), None)
     t_adam = next((p["gpu_s"] for p in adam_traj[1:] if p["loss_median"] <= goal), None)
-    log(f"levenberg-marquardt: {iters} iterations, {gpu:.2f} s ({gpu / iters * 1e3:.2f} ms/iteration); "
-        f"mean rel err median {traj[-1]['mean_rel_median']:.4f}; Adam's final median loss {goal:.3e} "
-        f"after {t_lm} s (Adam {t_adam} s)")
-    return {"iterations": iters, "ms_per_iteration": gpu / iters * 1e3, "walks_cut": cut,
+    log(f"levenberg-marquardt (cell stride {stride}): {iters} iterations, {gpu:.2f} s ({gpu / iters * 1e3:.2f} "
+        f"ms/iteration); mean rel err median {traj[-1]['mean_rel_median']:.4f}; Adam's final median loss "
+        f"{goal:.3e} after {t_lm} s (Adam {t_adam} s)")
+    return {"iterations": iters, "cell_stride": stride, "ms_per_iteration": gpu / iters * 1e3, "walks_cut": cut,
             "adam_final_loss_median": goal, "gpu_s_to_adam_final_loss": t_lm, "gpu_s_adam_to_it": t_adam,
             "gpu_time_ratio_adam_over_lm_at_loss": (t_adam / t_lm) if (t_lm and t_adam) else None,
             "method": "Levenberg-Marquardt per set on (log mean, log sd, proportion logits), damping x 0.2 on an "
                       "accepted trial and x 4 on a rejected one, steps limited to 0.5 per parameter, "
-                      "sd <= 0.8 mean; Gauss-Newton system from one relaxed Jacobian pass per iteration",
+                      "sd <= 0.8 mean; Gauss-Newton system from one relaxed Jacobian pass per iteration"
+                      + (f" over the initial cells of one residue class mod {stride}" if stride > 1 else ""),
             "trajectory": traj}
 
 
@@ -460,11 +491,11 @@ def calibration_cpu_baseline(calibrate, np, start, targets, n_sample=4):
                                       f"({dt:.2f} s wall), scaled by {len(start) / n_sample:g}", "cpu": cpu_model()}
 
 
-def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, grad_traj, strided_traj=None,
-                         lm_traj=None):
+def nelder_mead_run(procell, calibrate, np, gen, start, targets, rounds, model_factor=1):
     """Nelder-Mead per set over the plain method: one batched plain run of
-    all sets per round (each set's next simplex point, its own fixed seed),
-    GPU time = the batch runs' kernel times."""
+    all sets per round (each set's next simplex point, its own fixed seed,
+    counts x model_factor), GPU time = the batch runs' kernel times.
+    Returns the trajectory (every 25 rounds), GPU seconds and wall seconds."""
     n = len(gen)
     prolif = [w.mean >= 0 for w in start]
     qs, binsets = [], []
@@ -489,7 +520,7 @@ def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, gr
             xe[:k] = np.clip(xe[:k], lo[0], hi[0])
             xe[k:2 * k] = np.clip(xe[k:2 * k], lo[1], hi[1])
             m, s, p = calibrate.unpack(xe, prolif[i])
-            probs.append(procell.Problem(w.values, w.counts, p, m, s, w.t_max, w.phi, 0,
+            probs.append(procell.Problem(w.values, w.counts * np.uint64(model_factor), p, m, s, w.t_max, w.phi, 0,
                                          flags=procell.FLAG_TIMING if i == 0 else 0))
         with procell.Batch(probs) as b:
             b.run(seeds)
@@ -505,16 +536,26 @@ def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, gr
             traj.append({"rounds": r + 1, "gpu_s": gpu_ms * 1e-3, "wall_s": time.perf_counter() - wall0,
                          **recovery(np, gen, th)})
     wall = time.perf_counter() - wall0
-    # GPU time each method needs to reach (a) the gradient method's final median mean error and
-    # (b) the accuracy both reach (the worse of the two final medians)
-    def first(tr, goal):
-        return next((p["gpu_s"] for p in tr if p["mean_rel_median"] <= goal), None)
+    return traj, gpu_ms * 1e-3, wall
+
+
+def first_at(tr, goal):
+    """GPU seconds at which a trajectory first reaches a median mean error <= goal."""
+    return next((p["gpu_s"] for p in tr if p["mean_rel_median"] <= goal), None)
+
+
+def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, grad_traj, strided_traj=None,
+                         lm_traj=None, lm_strided_traj=None):
+    """Nelder-Mead per set over the plain method (nelder_mead_run) against the
+    gradient trajectories: GPU time of each to the accuracy both reach."""
+    traj, gpu_s, wall = nelder_mead_run(procell, calibrate, np, gen, start, targets, rounds)
+    first = first_at
 
     goal = grad_traj[-1]["mean_rel_median"]
     t_grad, t_nm = first(grad_traj, goal), first(traj, goal)
     common = max(goal, traj[-1]["mean_rel_median"])
     c_grad, c_nm = first(grad_traj, common), first(traj, common)
-    log(f"nelder-mead: {rounds} rounds, {gpu_ms * 1e-3:.2f} s GPU ({wall:.1f} s wall); mean rel err median "
+    log(f"nelder-mead: {rounds} rounds, {gpu_s:.2f} s GPU ({wall:.1f} s wall); mean rel err median "
         f"{traj[-1]['mean_rel_median']:.4f}; goal {goal:.4f}: gradient {t_grad} s, nelder-mead {t_nm} s; "
         f"common {common:.4f}: gradient {c_grad} s, nelder-mead {c_nm} s")
     return {
@@ -522,7 +563,7 @@ def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, gr
                   "on log mean, log sd, log proportions; objective = normalised-L2 of the plain histogram "
                   "(fixed seed per set) to the target",
         "rounds": rounds,
-        "gpu_s": gpu_ms * 1e-3,
+        "gpu_s": gpu_s,
         "wall_s": wall,
         "trajectory": traj,
         "goal_mean_rel_median": goal,
@@ -535,6 +576,7 @@ def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, gr
         "gpu_time_ratio_at_common": (c_nm / c_grad) if (c_nm and c_grad) else None,
         **strided_ratio(first, strided_traj, traj),
         **strided_ratio(first, lm_traj, traj, "levenberg_marquardt"),
+        **strided_ratio(first, lm_strided_traj, traj, "levenberg_marquardt_strided"),
         "wall_s_to_common_nelder_mead": next((p["wall_s"] for p in traj if p["mean_rel_median"] <= common), None),
         "wall_note": "the calibration steps run back to back on the device (their gpu_s is wall time around "
                      "the replays); each Nelder-Mead round re-creates a batch plan (host set-up and uploads), "
@@ -542,6 +584,77 @@ def nelder_mead_baseline(procell, calibrate, np, gen, start, targets, rounds, gr
         "note": "gpu_time_ratio = Nelder-Mead GPU time / gradient GPU time to the same median mean error; "
                 "> 1 favours the gradient method",
     }
+
+
+def rich_target_measurement(procell, calibrate, np, workloads, gen, start, targets, n_sets, iters, nm_rounds,
+                            target_factor=64, replicas=8):
+    """Where the 2e5-cell calibration's accuracy floor comes from, on the
+    first n_sets sets: Levenberg-Marquardt against (a) the 2e5-cell target
+    with one relaxed seed, (b) the same target with the estimator averaged
+    over `replicas` seeds, (c) a target with target_factor x the cells
+    (the plain histogram of 1.28e7 cells per set, as from many seeds) with
+    the averaged estimator; and Nelder-Mead on (c) with the plain method at
+    replicas x the cells.  Reports whether 2 % median recovery of the means
+    is reached and the GPU-time ratio at the accuracy both methods reach."""
+    gen, start, targets = gen[:n_sets], start[:n_sets], targets[:n_sets]
+    rich = [workloads.scaled(w, target_factor) for w in gen]
+    with procell.Batch([procell.Problem.from_workload(w, 0) for w in rich]) as b:
+        b.run(np.arange(n_sets, dtype=np.uint64) + 1000)
+        rich_targets = [b.result(i, want_raw=False) for i in range(n_sets)]
+    rec0 = recovery(np, gen, [np.stack([w.mean, w.sd, w.proportion]) for w in start])
+    runs = {}
+    for name, tg, rep_n in (("target_2e5_seeds_1", targets, 1), (f"target_2e5_seeds_{replicas}", targets, replicas),
+                            (f"target_rich_seeds_{replicas}", rich_targets, replicas)):
+        traj, gpu, cut, final = lm_run(procell, calibrate, np, gen, start, tg, iters, rec0, replicas=rep_n)
+        runs[name] = {"ms_per_iteration": gpu / iters * 1e3, "walks_cut": cut, "trajectory": traj}
+        log(f"rich-target leg, LM {name}: mean rel err median {traj[-1]['mean_rel_median']:.4f} "
+            f"({gpu / iters * 1e3:.1f} ms/iteration)")
+    # the last run continued at a 8x lower temperature: whether the relaxation's
+    # smoothing, rather than its other approximations, sets the plateau
+    traj_c, _, cut_c, _ = lm_run(procell, calibrate, np, gen, start, rich_targets, iters, traj[-1], replicas=replicas,
+                                 beta=0.0625, thetas=final, gpu0=gpu)
+    runs["target_rich_beta_0.0625_continued"] = {"walks_cut": cut_c, "trajectory": traj_c}
+    log(f"rich-target leg, LM continued at beta 0.0625: mean rel err median {traj_c[-1]['mean_rel_median']:.4f}")
+    # the relaxed loss at the generating parameters, against the loss where LM ends
+    gen_probs = [procell.Problem(w.values, w.counts, g.proportion, g.mean, g.sd, w.t_max, w.phi, 50000 + 1000 * r + i)
+                 for r in range(replicas) for i, (w, g) in enumerate(zip(start, gen))]
+    plan = procell.RelaxedBatch(gen_probs, 0.5)
+    at_gen = calibrate.BatchLM(plan, gen_probs[:n_sets], rich_targets, replicas=replicas)
+    at_gen.start()
+    loss_at_gen = float(at_gen.loss.median())
+    del at_gen
+    plan.close()
+    nm_traj, nm_gpu, nm_wall = nelder_mead_run(procell, calibrate, np, gen, start, rich_targets, nm_rounds,
+                                               model_factor=replicas)
+    lm_traj = runs[f"target_rich_seeds_{replicas}"]["trajectory"]
+    common = max(lm_traj[-1]["mean_rel_median"], nm_traj[-1]["mean_rel_median"])
+    t_lm, t_nm = first_at(lm_traj, common), first_at(nm_traj, common)
+    log(f"rich-target leg, Nelder-Mead: mean rel err median {nm_traj[-1]['mean_rel_median']:.4f} "
+        f"({nm_gpu:.2f} s GPU); common {common:.4f}: LM {t_lm} s, NM {t_nm} s")
+    summary = {
+        "sets": n_sets, "target_cells_per_set": int(rich[0].n_cells), "estimator_seeds": replicas,
+        "lm_mean_rel_2e5_target_1_seed": runs["target_2e5_seeds_1"]["trajectory"][-1]["mean_rel_median"],
+        f"lm_mean_rel_2e5_target_{replicas}_seeds": runs[f"target_2e5_seeds_{replicas}"]["trajectory"][-1][
+            "mean_rel_median"],
+        "lm_mean_rel_rich_target": lm_traj[-1]["mean_rel_median"],
+        "lm_mean_rel_rich_target_beta_0.0625": runs["target_rich_beta_0.0625_continued"]["trajectory"][-1][
+            "mean_rel_median"],
+        "lm_rich_within_2pct": lm_traj[-1]["mean_within_2pct"],
+        "relaxed_loss_median_lm_end": lm_traj[-1]["loss_median"],
+        "relaxed_loss_median_at_generating": loss_at_gen,
+        "lm_rich_gpu_s_to_2pct": first_at(lm_traj, 0.02),
+        "nm_mean_rel_rich_target": nm_traj[-1]["mean_rel_median"],
+        "nm_rich_gpu_s_to_2pct": first_at(nm_traj, 0.02),
+        "common_mean_rel_median": common,
+        "gpu_time_ratio_nm_over_lm_at_common": (t_nm / t_lm) if (t_lm and t_nm) else None,
+    }
+    return {"workload": f"the first {n_sets} sets of the calibration workload; rich target = plain method at the "
+                        f"generating parameters with {target_factor}x the cells (seeds 1000+i); estimator = relaxed "
+                        f"plan with {replicas} copies of each set (seeds 50000 + 1000 r + i); Nelder-Mead over the "
+                        f"plain method with {replicas}x the cells",
+            "levenberg_marquardt": runs,
+            "nelder_mead": {"gpu_s": nm_gpu, "wall_s": nm_wall, "rounds": nm_rounds, "trajectory": nm_traj},
+            "summary": summary}
 
 
 def strided_ratio(first, straj, nm_traj, name="strided"):
@@ -610,15 +723,18 @@ def compact_line(full: dict) -> dict:
         if c.get("strided"):
             cs["strided"] = {"ms_per_step": c["strided"]["ms_per_step"],
                              **_pick(c["strided"]["trajectory"][-1], ("mean_rel_median", "gpu_s"))}
-        lm = c.get("levenberg_marquardt")
-        if lm:
-            cs["lm"] = {**_pick(lm, ("ms_per_iteration", "walks_cut", "gpu_time_ratio_adam_over_lm_at_loss")),
-                        **_pick(lm["trajectory"][-1], ("mean_rel_median", "sd_rel_median", "gpu_s"))}
+        for key, short in (("levenberg_marquardt", "lm"), ("levenberg_marquardt_strided", "lm_strided")):
+            lm = c.get(key)
+            if lm:
+                cs[short] = {**_pick(lm, ("cell_stride", "ms_per_iteration", "walks_cut",
+                                          "gpu_time_ratio_adam_over_lm_at_loss")),
+                             **_pick(lm["trajectory"][-1], ("mean_rel_median", "sd_rel_median", "gpu_s"))}
         nm = c.get("derivative_free")
         if nm:
             cs["nelder_mead"] = {**_pick(nm, ("gpu_s", "gpu_time_ratio_at_common",
                                               "gpu_time_ratio_strided_at_common",
-                                              "gpu_time_ratio_levenberg_marquardt_at_common")),
+                                              "gpu_time_ratio_levenberg_marquardt_at_common",
+                                              "gpu_time_ratio_levenberg_marquardt_strided_at_common")),
                                  **_pick(nm["trajectory"][-1], ("mean_rel_median",))}
         rich = c.get("rich_target")
         if rich:
@@ -649,6 +765,8 @@ def main():
     ap.add_argument("--calib-steps", type=int, default=200, help="steps of the batched calibration")
     ap.add_argument("--nm-rounds", type=int, default=300, help="rounds of the Nelder-Mead baseline (0: skip)")
     ap.add_argument("--lm-iters", type=int, default=15, help="Levenberg-Marquardt iterations (0: skip)")
+    ap.add_argument("--rich-sets", type=int, default=64,
+                    help="sets of the rich-target calibration leg (0: skip; needs --lm-iters and --nm-rounds)")
     ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
                     help="file for the full record (every trajectory); '' to skip")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -787,7 +905,8 @@ def main():
     if not args.no_calib:
         out["calibration"] = calibration_measurement(procell, workloads, np, steps=args.calib_steps,
                                                      nm_rounds=args.nm_rounds, with_nm=args.nm_rounds > 0,
-                                                     with_cpu=not args.no_cpu, lm_iters=args.lm_iters)
+                                                     with_cpu=not args.no_cpu, lm_iters=args.lm_iters,
+                                                     rich_sets=args.rich_sets)
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_budget)
         out["cpu_baseline"] = cb
