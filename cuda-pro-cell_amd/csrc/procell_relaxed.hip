@@ -5,9 +5,12 @@
 //
 // Relaxation of the plain method (proliferation.cu:26-215, cell.cu:80-125):
 //   * every initial cell takes every type j with weight p_j (no type draw);
-//   * timers are mean_j + sd_j z, untruncated, with z the first candidates of
-//     the plain stream's blocks: the root's pair from block (cell, 0, 0)
-//     words {2, -, 3, -}, a node's daughters' pair from block (cell, node, 0);
+//   * timers are the plain method's own, mean_j + sd_j z with z the first
+//     candidate > 0 of the plain stream: the root's pair from block
+//     (cell, 0, 0) words {2, -, 3, -}, then node 1's redraw stream; a node's
+//     daughters' pair from block (cell, node, 0), a daughter whose candidate
+//     is <= 0 continuing in its own node's redraw stream (with beta -> 0 and
+//     one type the estimator is the plain tally of the same seed);
 //     the root's phase U comes from word 1 of block (cell, 0, 0);
 //   * the hard test "completes by t_max" becomes d = sigmoid((t_max - s)/beta)
 //     for a node whose division would complete at s;
@@ -62,12 +65,13 @@ namespace {
 constexpr double kRelaxEps = 1e-15;        // smallest W d that is still expanded
 // Expansions a single walk may take: a walk that needs more is cut short and
 // counted, and the call reports PROCELL_E_DEPTH (parameters far outside the
-// estimator's domain, e.g. sd >= 0.85 mean with deep rows, would otherwise
-// run for practically ever).
+// estimator's domain, e.g. sd a few times the mean with deep rows, where most
+// timers lie near 0, would otherwise run for practically ever).
 constexpr uint32_t kWalkBudget = 1u << 22;
 // Calibration keeps sd <= 0.8 mean (log 0.8): the expected number of
-// generation-k nodes, 2^k P(s_k <= t), stays bounded only while
-// mean^2 / (2 sd^2) > ln 2, i.e. sd / mean < 0.85.
+// generation-k nodes, 2^k P(s_k <= t), grows with the timers' density near 0
+// (for untruncated normal timers it would diverge beyond sd / mean = 0.85,
+// where mean^2 / (2 sd^2) = ln 2), so the bound keeps the walks cheap.
 constexpr double kLogMaxSdRatio = -0.22314355131420976;
 // A stacked node: s, W, d (both passes), b, dlogW/dmean, dlogW/dsd (adjoint),
 // padded to one 64-byte line so that a push or a pop of a lane moves one line.
@@ -324,6 +328,36 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
         }
         double z0, z1;
         dev::normal_pair(wd, z0, z1, K, tb);
+        // the plain method's timers (cell.cu:106-122): a candidate <= 0 gives
+        // way to the first candidate > 0 of a redraw stream -- the root's to
+        // its second candidate, then node 1's stream; a daughter's to its own
+        // node's stream; after 65535 blocks the timer is the mean (z = 0)
+        {
+            const bool redoA = !(fma(z0, sd, mean) > 0.0);
+            const bool redoB = !pre && !(fma(z1, sd, mean) > 0.0);
+            if (__builtin_expect(__ballot(redoA || redoB) != 0, 0)) {
+                auto redraw = [&](uint64_t rnode) -> double {
+                    for (uint32_t att = 1; att < 65536u; ++att) {
+                        const dev::Words wr = dev::draw_block(S.seed, cell, rnode, att);
+                        double y0, y1;
+                        dev::normal_pair(wr, y0, y1, K, tb);
+                        if (fma(y0, sd, mean) > 0.0)
+                            return y0;
+                        if (fma(y1, sd, mean) > 0.0)
+                            return y1;
+                    }
+                    return 0.0;
+                };
+                if (redoA) {
+                    if (pre)
+                        z0 = fma(z1, sd, mean) > 0.0 ? z1 : redraw(1);
+                    else
+                        z0 = redraw(2 * node);
+                }
+                if (redoB)
+                    z1 = redraw(2 * node + 1);
+            }
+        }
 
         double sA, bA, WA, gmA, gsA, sB = 0.0, bB = 0.0;
         uint32_t kA;
@@ -548,7 +582,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
 }
 
 template <bool Grad>
-__global__ __launch_bounds__(64) void relaxed_kernel(RelaxArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void relaxed_kernel(RelaxArgs a)
 {
     relaxed_walks<Grad ? kAdjoint : kForward, false>(a);
 }
@@ -1094,8 +1128,8 @@ __global__ __launch_bounds__(256) void calib_adam_kernel(CalibArgs c)
                 adam_update(c.x[tb + j], c.m1[tb + j], c.m2[tb + j], gr[j] * th[j], c, bc1, bc2);
                 adam_update(c.x[tb + T + j], c.m1[tb + T + j], c.m2[tb + T + j], gr[T + j] * th[T + j], c, bc1,
                             bc2);
-                // keep sd <= kMaxSdRatio mean: with untruncated timers the
-                // expected tree stays finite only while sd / mean < 0.85
+                // keep sd <= 0.8 mean (kLogMaxSdRatio): the expected tree
+                // grows quickly beyond it
                 c.x[tb + T + j] = fmin(c.x[tb + T + j], c.x[tb + j] + kLogMaxSdRatio);
             }
             const uint32_t z = tb + 2 * T + j;

@@ -226,7 +226,7 @@ def plain_loss(bins, value, count, target_q):
 # ---------------------------------------------------------------------------
 # Second order: Levenberg-Marquardt per set over a batched relaxed plan.
 # ---------------------------------------------------------------------------
-LOG_MAX_SD_RATIO = float(np.log(0.8))  # sd <= 0.8 mean keeps the relaxed tree finite
+LOG_MAX_SD_RATIO = float(np.log(0.8))  # sd <= 0.8 mean keeps the relaxed trees small
 MAX_LOG_STEP = 0.5                     # largest change of one parameter per iteration
 
 

@@ -615,9 +615,10 @@ int oracle_simulate(const oracle_problem *p, int64_t *tally, int64_t *drops,
 /*
  * This project's own relaxation of the plain method (no reference
  * counterpart): every type j of an initial cell is followed with weight
- * p_j (exact ratio weighting instead of a type draw); timers are
- * mean_j + sd_j z with z the first candidates of the plain stream's blocks
- * (no truncation, no redraws); a node whose division would complete at s
+ * p_j (exact ratio weighting instead of a type draw); timers are the plain
+ * method's own, mean_j + sd_j z with z the first candidate > 0 of the plain
+ * stream (the node's pair block, then the daughter's redraw stream, as in
+ * divide() and root() above); a node whose division would complete at s
  * divides with weight d = sigmoid((t_max - s) / beta) and is recorded with
  * weight 1 - d, so a node's weight W is the product of its ancestors' d.
  * Fluorescence stays exact: a node at its row's generation cap records
@@ -625,10 +626,15 @@ int oracle_simulate(const oracle_problem *p, int64_t *tally, int64_t *drops,
  * nothing (proliferation.cu:317-381 with the hard test replaced).
  * Subtrees whose weight W d falls below 1e-15 are not followed.
  *
+ * With beta -> 0 and one type the estimator is the plain tally of the
+ * same seed.
+ *
  * The gradient of L with respect to (mean_j, sd_j, p_j), given
  * g = dL/dh, is accumulated along the same traversal with forward
  * tangents: ds/dmean = (1 + U) + k, ds/dsd = z_0 (1 + U) + z_1 + ... + z_k
- * and dlog W/dtheta = -sum over ancestors of (1 - d) / beta ds/dtheta.
+ * (the accepted candidates' z; which candidate is accepted is held fixed,
+ * the pathwise derivative) and dlog W/dtheta = -sum over ancestors of
+ * (1 - d) / beta ds/dtheta.
  */
 #define RELAXED_EPS 1e-15
 
@@ -647,6 +653,35 @@ typedef struct {
 } relaxed_walk;
 
 static double sigmoid(double x) { return 1.0 / (1.0 + exp(-x)); }
+
+/* The timer of a relaxed walk from candidate z: kept when > 0, else the
+ * first candidate > 0 of the redraw stream of `node` (attempts 1, 2, ...,
+ * two candidates each, as truncated_timer()); *z becomes the accepted
+ * candidate (0 after 65535 attempts, where the timer is the mean). */
+static double relaxed_timer(const relaxed_walk *w, uint64_t node, double *z)
+{
+    double t = fma(*z, w->sd, w->mean);
+    if (t > 0.0)
+        return t;
+    uint32_t blk[4];
+    double y0, y1;
+    for (uint32_t a = 1; a < 65536u; ++a) {
+        oracle_block(w->s->seed, w->cell, node, a, blk);
+        oracle_normal_pair(blk, &y0, &y1);
+        t = fma(y0, w->sd, w->mean);
+        if (t > 0.0) {
+            *z = y0;
+            return t;
+        }
+        t = fma(y1, w->sd, w->mean);
+        if (t > 0.0) {
+            *z = y1;
+            return t;
+        }
+    }
+    *z = 0.0;
+    return w->mean;
+}
 
 static void relaxed_node(relaxed_walk *w, uint64_t node, int k, double s, double a, double b,
                          double W, double gm, double gs)
@@ -674,8 +709,10 @@ static void relaxed_node(relaxed_walk *w, uint64_t node, int k, double s, double
     oracle_normal_pair(blk, &z0, &z1);
     const double f = (1.0 - d) / w->beta;
     const double Wc = W * d, gmc = gm - f * a, gsc = gs - f * b;
-    relaxed_node(w, 2 * node, k + 1, s + fma(z0, w->sd, w->mean), a + 1.0, b + z0, Wc, gmc, gsc);
-    relaxed_node(w, 2 * node + 1, k + 1, s + fma(z1, w->sd, w->mean), a + 1.0, b + z1, Wc, gmc, gsc);
+    const double tl = relaxed_timer(w, 2 * node, &z0);
+    const double tr = relaxed_timer(w, 2 * node + 1, &z1);
+    relaxed_node(w, 2 * node, k + 1, s + tl, a + 1.0, b + z0, Wc, gmc, gsc);
+    relaxed_node(w, 2 * node + 1, k + 1, s + tr, a + 1.0, b + z1, Wc, gmc, gsc);
 }
 
 static void relaxed_cell(relaxed_walk *w)
@@ -703,8 +740,15 @@ static void relaxed_cell(relaxed_walk *w)
             }
             continue;
         }
-        const double tau0 = fma(z0, w->sd, w->mean);
-        relaxed_node(w, 1, 0, tau0 * (1.0 + U), 1.0 + U, z0 * (1.0 + U), 1.0, 0.0, 0.0);
+        /* the root's candidates z0 then z1 (cell.cu:106-122), then node 1's
+         * redraw stream, as in root() */
+        double z = z0;
+        double tau0 = fma(z, w->sd, w->mean);
+        if (!(tau0 > 0.0)) {
+            z = z1;
+            tau0 = relaxed_timer(w, 1, &z);
+        }
+        relaxed_node(w, 1, 0, tau0 * (1.0 + U), 1.0 + U, z * (1.0 + U), 1.0, 0.0, 0.0);
     }
 }
 

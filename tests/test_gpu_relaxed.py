@@ -365,3 +365,25 @@ def test_runaway_walk_is_cut_and_reported(procell_lib):
         th2 = torch.tensor([[100.0], [10.0], [1.0]], dtype=torch.float64, device="cuda")
         rp.forward(th2, h)
         assert rp.expansions() > 0
+
+
+def test_tiny_beta_one_type_is_the_plain_tally(procell_lib):
+    """beta -> 0, one type: the relaxed walks take the plain method's timers
+    (redraw streams included) and its hard test, so the device's relaxed
+    tally is the device's plain tally of the same seed (integers summed in
+    FP64, exact in any order)."""
+    import torch
+
+    values = [2.0**12, 3000.0, 700.0, 40.0, 9.0]
+    counts = [3000, 2000, 5000, 1000, 500]
+    for mean, sd, seed in ((20.0, 10.0, 3), (30.0, 15.0, 4)):
+        print("tiny beta", mean, sd, seed, flush=True)
+        prob = procell_lib.Problem(values, counts, [1.0], [mean], [sd], 150.0, 10.0, seed)
+        plain = procell_lib.simulate(prob)
+        assert plain.stats["rejected"] > 0  # redraw streams exercised
+        th = torch.tensor([[mean], [sd], [1.0]], dtype=torch.float64, device="cuda")
+        with procell_lib.Relaxed(prob, 1e-9) as rp:
+            h = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+            rp.forward(th, h)
+            torch.cuda.synchronize()
+        assert np.array_equal(h.cpu().numpy(), plain.raw_tally.astype(np.float64))

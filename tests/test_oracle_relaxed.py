@@ -1,9 +1,13 @@
 """CPU checks that pin the oracle's relaxed estimator and its gradient.
 
 The relaxed estimator (oracle_relaxed) weights every type by its
-proportion, uses untruncated timers mean + sd z with z from the plain
-stream's blocks, and replaces the division test by sigmoid((t_max - s)/beta).
+proportion, uses the plain method's timers mean + sd z (z the first
+candidate > 0 of the plain stream, redraw streams included), and replaces
+the division test by sigmoid((t_max - s)/beta).
 Pinned here by what does not depend on its code:
+  * beta -> 0 with one type: the relaxed tally is the plain oracle's tally
+    of the same seed, entry for entry, with sd = mean / 2 so that the redraw
+    streams are exercised in most trees;
   * central finite differences of the forward with common z (the same
     seed) against the adjoint gradient, for every mean, sd and (along
     directions that keep the sum 1) proportion;
@@ -11,8 +15,9 @@ Pinned here by what does not depend on its code:
     closed form per initial cell (2^k nodes at generation k, each with
     the product of its ancestors' sigmoids), computed here with numpy
     from the cells' phases, and its derivative by finite differences;
-  * small beta, where the estimator is an unbiased Monte Carlo estimate of
-    the untruncated Gaussian-sum expected histogram
+  * small beta with sd << mean (no candidate is ever <= 0), where the
+    estimator is an unbiased Monte Carlo estimate of the Gaussian-sum
+    expected histogram
     2^k (P(s_{k-1} <= t_max) - P(s_k <= t_max)), s_k | U ~ N(m (1+U+k),
     sd^2 ((1+U)^2 + k)): histogram and gradient agree within their
     sampling error.
@@ -27,6 +32,21 @@ from scipy import stats as sst
 
 def _prob(oracle, values, counts, prop, mean, sd, t_max, phi, seed):
     return oracle.Problem(values, counts, prop, mean, sd, t_max, phi, seed)
+
+
+def test_tiny_beta_one_type_is_the_plain_tally(oracle):
+    """With beta -> 0 the sigmoid is the plain method's hard test and the
+    timers are the plain method's own, so for one type the relaxed tally is
+    the plain tally of the same seed."""
+    values, counts = [2.0**12, 3000.0, 700.0, 40.0, 9.0], [300, 200, 500, 100, 50]
+    for mean, sd, seed in ((20.0, 10.0, 3), (30.0, 15.0, 4), (25.0, 5.0, 5)):
+        p = _prob(oracle, values, counts, [1.0], [mean], [sd], 150.0, 10.0, seed)
+        tally, _, st = oracle.simulate(p)
+        if sd / mean >= 0.5:
+            assert st["rejected"] > 0  # redraw streams exercised
+        h, _ = oracle.relaxed(p, 1e-9)
+        assert h.shape == tally.shape
+        assert np.array_equal(h, tally.astype(np.float64)), (mean, sd, np.abs(h - tally).max())
 
 
 def _L(oracle, args, g, beta):
