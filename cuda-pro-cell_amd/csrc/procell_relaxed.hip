@@ -81,12 +81,17 @@ constexpr uint32_t kFrameDoubles = 8;
 // accesses are bank-conflict free; only older frames reach the device stack.
 // Its cotangent window is smaller, so that window and cache fit the LDS of
 // four wavefronts per SIMD.  256 sets x 2e5 cells: adjoint 13.4 -> 12.6 ms,
-// calibration step 23.3 -> 22.5 ms (one slot, or the cache in the Jacobian
-// pass as well, measured slower: they cost a wavefront per SIMD).
+// calibration step 23.3 -> 22.5 ms (one slot measured slower).  The Jacobian
+// pass without a stride keeps one frame beside a 768-double window: HBM
+// written per expansion 26.5 -> 10.6 B, read 25.7 -> 8.7 B, for 1% more time
+// (14.09 -> 14.24 ms; two slots beside 384 doubles measured 2% slower, the
+// cache in the stride-4 Jacobian pass 7% slower).
 constexpr uint32_t kTanFields = 6;
-constexpr uint32_t kTanSlots = 2;
+constexpr uint32_t kTanSlots = 2;  // adjoint
+constexpr uint32_t kJacSlots = 1;  // Jacobian pass
 constexpr uint32_t kWindowDoubles = 1024;  // LDS window of a wavefront (8 KB)
 constexpr uint32_t kWindowAdj = 384;       // window of the adjoint pass beside its frame cache (3 KB)
+constexpr uint32_t kWindowJac = 768;       // window of the Jacobian pass beside its frame cache (6 KB)
 constexpr uint32_t kRowCap = 64;           // rows of a wavefront's range staged in LDS
 // Wavefronts of a plan over several sets, in resident wavefronts: short
 // ranges cover few rows, so the tallies stay in the LDS window (the
@@ -198,8 +203,9 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     const uint32_t win_n = (r_hi - r_lo + 1) * KT;
     // window: h (forward), g (adjoint) or h | dh/dmean | dh/dsd (Jacobian)
     const uint32_t win_t = Jac ? 3 * win_n : win_n;
-    constexpr bool kLdsCache = Grad && !Strided;  // frames cached in LDS
-    constexpr uint32_t kWin = kLdsCache ? kWindowAdj : kWindowDoubles;
+    constexpr bool kLdsCache = (Grad || Jac) && !Strided;  // frames cached in LDS
+    constexpr uint32_t kWin = kLdsCache ? (Jac ? kWindowJac : kWindowAdj) : kWindowDoubles;
+    constexpr uint32_t kSlots = Jac ? kJacSlots : kTanSlots;
     const bool in_lds = win_t <= kWin;
     const uint64_t hw0 = S.h_base + (uint64_t)(r_lo - S.row_base) * KT;  // tally index of the window
     double *gsum = lds + kWin;
@@ -259,9 +265,9 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     bool c_ok = false;
     uint32_t c_lvl = 0;
     double c_s = 0.0, c_W = 0.0, c_d = 0.0;
-    // adjoint pass: frames cached in LDS slots (c_n of them, c_top the
+    // adjoint and Jacobian passes: frames cached in LDS slots (c_n of them, c_top the
     // deepest); the levels of slots 0 and 1 in t_lvl0, t_lvl1
-    static_assert(kTanSlots == 1 || kTanSlots == 2, "one or two cached frames");
+    static_assert(kSlots == 1 || kSlots == 2, "one or two cached frames");
     uint32_t c_n = 0, c_top = 0, t_lvl0 = 0, t_lvl1 = 0;
     const uint32_t smask = Strided ? a.stride_mask : 0u;
     const uint32_t sphase = Strided ? ((a.step ? (uint32_t)a.step[0] : 0u) + a.phase) & smask : 0u;
@@ -462,15 +468,15 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
             c_d = dB;
             pending |= 1ull << kA;
         } else if (kLdsCache && expA && expB) {
-            if (c_n == kTanSlots) {  // spill the oldest cached frame to its level
-                const uint32_t bot = kTanSlots == 1 ? 0u : c_top ^ 1u;
+            if (c_n == kSlots) {  // spill the oldest cached frame to its level
+                const uint32_t bot = kSlots == 1 ? 0u : c_top ^ 1u;
                 double2 *f = (double2 *)(stk + (uint64_t)(bot ? t_lvl1 : t_lvl0) * kFrameDoubles);
                 f[0] = make_double2(fcp(bot, 0), fcp(bot, 1));
                 f[1] = make_double2(fcp(bot, 2), fcp(bot, 3));
                 f[2] = make_double2(fcp(bot, 4), fcp(bot, 5));
                 --c_n;
             }
-            c_top = kTanSlots == 1 ? 0u : c_top ^ 1u;
+            c_top = kSlots == 1 ? 0u : c_top ^ 1u;
             fcp(c_top, 0) = sB;
             fcp(c_top, 1) = WA;
             fcp(c_top, 2) = dB;
@@ -515,7 +521,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
                 b = fcp(c_top, 3);
                 gm = fcp(c_top, 4);
                 gs = fcp(c_top, 5);
-                c_top = kTanSlots == 1 ? 0u : c_top ^ 1u;
+                c_top = kSlots == 1 ? 0u : c_top ^ 1u;
                 --c_n;
             } else {
                 const double2 *f = (const double2 *)(stk + (uint64_t)L * kFrameDoubles);
@@ -640,12 +646,15 @@ thread_local hipError_t g_relaxed_hip = hipSuccess;
         }                                    \
     } while (0)
 
-// adj: the adjoint kernel without a cell stride (smaller window, frame cache)
-size_t relaxed_lds_bytes(uint32_t n_types, bool adj)
+// mode 0: the forward kernels and the strided passes; 1: the adjoint without
+// a stride; 2: the Jacobian pass without a stride (the last two: smaller
+// window, frame cache)
+size_t relaxed_lds_bytes(uint32_t n_types, int mode)
 {
-    const size_t base = ((adj ? kWindowAdj : kWindowDoubles) + 6ull * n_types) * sizeof(double) +
-                        (2ull * kRowCap + 1 + n_types) * sizeof(uint32_t);
-    return adj ? (base + 7) / 8 * 8 + (size_t)kTanSlots * kTanFields * 64 * sizeof(double) : base;
+    const uint32_t win = mode == 1 ? kWindowAdj : mode == 2 ? kWindowJac : kWindowDoubles;
+    const size_t base = (win + 6ull * n_types) * sizeof(double) + (2ull * kRowCap + 1 + n_types) * sizeof(uint32_t);
+    const uint32_t slots = mode == 1 ? kTanSlots : mode == 2 ? kJacSlots : 0u;
+    return slots ? (base + 7) / 8 * 8 + (size_t)slots * kTanFields * 64 * sizeof(double) : base;
 }
 
 // Expected expansions of one walk of a type with this mean in a row with
@@ -683,8 +692,9 @@ struct RelaxCore {
     std::vector<RelaxSet> sets;
     std::vector<uint32_t> quiet_flag;  // per type of every set: 1 = quiescent
     uint32_t n_waves = 0;
-    size_t lds_bytes = 0;  // every walk kernel but the adjoint without a stride
+    size_t lds_bytes = 0;  // every walk kernel but the two below
     size_t lds_adj = 0;    // the adjoint without a stride
+    size_t lds_jac = 0;    // the Jacobian pass without a stride
     bool any_prolif = false, any_quiet = false;
     uint64_t h_total = 0;      // tally entries of all sets
     uint32_t types_total = 0;  // types of all sets
@@ -826,15 +836,18 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     row_start.push_back((uint32_t)cells);
 
     // ---- wavefronts ----
-    c.lds_bytes = relaxed_lds_bytes(t_max_types, false);
-    c.lds_adj = relaxed_lds_bytes(t_max_types, true);
+    c.lds_bytes = relaxed_lds_bytes(t_max_types, 0);
+    c.lds_adj = relaxed_lds_bytes(t_max_types, 1);
+    c.lds_jac = relaxed_lds_bytes(t_max_types, 2);
     int occ = 0, occ_f = 0;
-    int occ_s = 0, occ_sf = 0;
+    int occ_s = 0, occ_sf = 0, occ_j = 0, occ_js = 0;
     RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, relaxed_kernel<true>, 64, c.lds_adj));
     RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, relaxed_kernel<false>, 64, c.lds_bytes));
     RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, relaxed_strided_kernel<true>, 64, c.lds_bytes));
     RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_sf, relaxed_strided_kernel<false>, 64, c.lds_bytes));
-    occ = std::min(std::min(occ, occ_f), std::min(occ_s, occ_sf));
+    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_j, relaxed_jacobian_kernel<false>, 64, c.lds_jac));
+    RELAX_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_js, relaxed_jacobian_kernel<true>, 64, c.lds_bytes));
+    occ = std::min(std::min(std::min(occ, occ_f), std::min(occ_s, occ_sf)), std::min(occ_j, occ_js));
     if (occ < 1)
         return PROCELL_E_HIP;
     if (ps[0].tuning.blocks_per_cu)
@@ -952,7 +965,7 @@ hipError_t core_launch(const RelaxCore &c, const RelaxArgs &a, int mode, hipStre
             if (a.stride_mask)
                 hipLaunchKernelGGL(relaxed_jacobian_kernel<true>, g, b, c.lds_bytes, st, a);
             else
-                hipLaunchKernelGGL(relaxed_jacobian_kernel<false>, g, b, c.lds_bytes, st, a);
+                hipLaunchKernelGGL(relaxed_jacobian_kernel<false>, g, b, c.lds_jac, st, a);
         } else if (a.stride_mask) {
             if (grad)
                 hipLaunchKernelGGL(relaxed_strided_kernel<true>, g, b, c.lds_bytes, st, a);
