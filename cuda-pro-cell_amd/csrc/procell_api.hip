@@ -225,7 +225,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     // ---- launch shape: block size, resident workgroups, LDS split ----
     LaunchShape &ls = pl->shape;
     RunArgs &a = pl->args;
-    a.stack_levels = kdim > 1 ? kdim - 1 : 1;  // levels 1 .. kmax
+    a.stack_levels = kdim > 3 ? kdim - 1 : 2;  // levels 1 .. kmax, at least 2 (idle lanes store at level 2)
     // row tables (8 bytes per row) go to LDS when they are small
     const uint32_t lds_rows = (size_t)(2 * n_rows + 1) * 4 <= 16384 ? n_rows : 0u;
     a.rows_in_lds = lds_rows ? 1u : 0u;

@@ -170,11 +170,14 @@ struct Lane {
 //    64 lanes draw the next 64 initial cells of the segment, one each, whatever
 //    the lanes hold: type, truncated first timer, phase.  Cells that never
 //    divide are tallied at generation 0 (or lost); dividing ones become
-//    16-byte entries of the wavefront's LDS buffer.
-//  * hand-outs: lanes that go idle take buffered entries at the end of the
-//    division step itself; once the roots are done and the buffer is empty,
-//    idle lanes take the shallowest pending right daughter of a busy lane of
-//    the wavefront.
+//    48-byte entries of the wavefront's LDS buffer.
+//  * hand-outs: idle lanes take buffered entries right before each division
+//    step (the one place in the loop where lane state is handed over, so
+//    that the compiler keeps the lane state in the same registers on every
+//    path: with a second hand-out site in the refill branch the loop carried
+//    about 40 register copies per refill, 3.5 % of the run); once the roots
+//    are done and the buffer is empty, idle lanes take the shallowest pending
+//    right daughter of a busy lane of the wavefront.
 //  * refills (root steps, steals) run as a uniform branch of the one loop,
 //    once more than idle_limit lanes are idle.
 //  * division steps (proliferation.cu:317-381): each busy lane divides its
@@ -229,6 +232,11 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
     QueueEntry *ent = (QueueEntry *)((unsigned char *)red + kSimSharedBytes + (size_t)wave * sim_wave_lds_bytes());
     unsigned int *win = (unsigned int *)((unsigned char *)red + kSimSharedBytes + (size_t)n_waves * sim_wave_lds_bytes());
     double *stack = (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
+    // the lane's stack column: level l (1-based) at stk_pop + l * stride,
+    // i.e. at stk_push + (l - 1) * stride (bytes)
+    const uint32_t stride = bdim * (uint32_t)sizeof(double);
+    unsigned char *const stk_push = (unsigned char *)(stack + tid);
+    unsigned char *const stk_pop = stk_push - stride;
 
     const uint64_t seg_begin = uni64(a.seg_begin[seg0]);
     const uint64_t seg_cells = uni64(a.seg_end[seg0]) - seg_begin;
@@ -290,6 +298,7 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
     L.tb_lds = kNoWindow;
     L.active = false;
     Counts n = {0, 0, 0, 0, 0};
+    uint32_t w_div = 0;  // divisions of the wavefront's common-case steps (scalar)
     const uint32_t idle_limit = a.idle_limit;
 
     // idle lanes take buffered dividing roots, in lane order
@@ -442,11 +451,8 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                     __builtin_amdgcn_wave_barrier();
                 }
             }
-            // ---- hand buffered roots to idle lanes ----
-            if (rpos < rn) {
-                hand_out(idle, n_idle);
-            } else if (roots_done) {
-                // ---- steal pending daughters inside the wavefront ----
+            // ---- steal pending daughters inside the wavefront ----
+            if (rpos >= rn && roots_done) {
                 const uint64_t donors = __ballot(L.active && L.pend != 0);
                 const uint32_t nd = uni(min((uint32_t)__popcll(donors), n_idle));
                 if (nd) {
@@ -482,47 +488,59 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 }
             }
 
-            if (__ballot(L.active) == 0) {
-                if (roots_done && rpos >= rn)
+            if (rpos >= rn && __ballot(L.active) == 0) {
+                if (roots_done)
                     break;
                 continue;
             }
         }
+        // ---- idle lanes take buffered roots ----
+        if (rpos < rn) {
+            const uint64_t idle2 = __ballot(!L.active);
+            hand_out(idle2, uni((uint32_t)__popcll(idle2)));
+        }
 
         // ======== division step ========
-        // Straight-line code: every outcome is a select, so the step costs a
-        // handful of scalar instructions (one scalar unit serves the whole
-        // CU).  Idle lanes run it too with their results discarded.
+        // Straight-line code: every outcome is a select.  The step's flags
+        // are lane masks, so their logic runs on the scalar unit beside the
+        // vector work (the vector unit is the bound: 4.9 VALU instructions
+        // per division against 1.1 scalar).  Idle lanes run the step too
+        // with their results discarded.
         {
             // One Philox block + Box-Muller pair per lane and step: either the
             // pair block of a new division (timers of both daughters) or one
             // block of a daughter's truncated-normal redraw stream
             // (cell.cu:106-122); a lane with a redraw pending does not advance
-            // its tree in that step.  Flags are 0/1 integers so the step's
-            // logic stays in vector instructions.
+            // its tree in that step.
             const uint32_t act = L.active ? 1u : 0u;
             // the level a pop at the end of this step reads: a lane pops
             // only when the step pushes nothing, so it is the deepest
             // pending level of the step's start; reading it here overlaps
             // the LDS round trip with the division step
-            const uint32_t lvl = L.pend ? top_bit(L.pend) : 1u;
-            const double sp = stack[__umul24(lvl - 1, bdim) + tid];
+            const uint32_t lvl = top_bit(L.pend | (N)2);  // 1 when nothing is pending (level bits start at 1)
+            const double sp = *(const double *)(stk_pop + __umul24(lvl, stride));
             double tl, tr;
-            uint32_t redo;
+            bool cmp;  // division complete this step
             if (__builtin_expect(__ballot(L.redo != 0) == 0, 1)) {
                 // no redraw pending in the wavefront (the common case): every
-                // busy lane starts a new division with its pair block
+                // busy lane starts a new division with its pair block; every
+                // lane's redo is 0 and its attempt number 1 (attempts restart
+                // at 1 whenever a redraw ends)
                 const Words w = dev::draw_block(seed, L.cell, L.node, 0u);
                 double z0, z1;
                 dev::normal_pair(w, z0, z1, K, tb);
                 tl = fma(z0, L.sd, L.mean);
                 tr = fma(z1, L.sd, L.mean);
-                const uint32_t r0 = tl > 0.0 ? 0u : 1u;
-                const uint32_t r1 = tr > 0.0 ? 0u : 1u;
-                redo = r0 | (r1 << 1);
-                n.div += act;
-                n.rej += act * (r0 + r1);
-                L.att = 1u;
+                const bool ok = tl > 0.0 && tr > 0.0;
+                cmp = L.active && ok;
+                w_div += (uint32_t)__popcll(__ballot(L.active));
+                if (__builtin_expect(__ballot(L.active && !ok) != 0, 0)) {
+                    // a timer <= 0: that daughter continues in its redraw stream
+                    const uint32_t r0 = tl > 0.0 ? 0u : 1u;
+                    const uint32_t r1 = tr > 0.0 ? 0u : 1u;
+                    L.redo = act ? r0 | (r1 << 1) : 0u;
+                    n.rej += act * (r0 + r1);
+                }
             } else {
                 const uint32_t fresh = L.redo == 0 ? 1u : 0u;
                 const uint32_t rright = (L.redo & 1u) ? 0u : 1u;   // redraw the right daughter?
@@ -542,27 +560,29 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                 const uint32_t rbit = rright ? 2u : 1u;
                 tl = fresh ? c0 : ((done & (rright ^ 1u)) ? v : L.tl);
                 tr = fresh ? c1 : ((done & rright) ? v : L.tr);
-                redo = fresh ? (r0 | (r1 << 1)) : (done ? L.redo & ~rbit : L.redo);
+                const uint32_t redo = fresh ? (r0 | (r1 << 1)) : (done ? L.redo & ~rbit : L.redo);
                 n.div += act & fresh;
                 n.rej += act * (fresh ? r0 + r1 : r0 + (r0 & r1));
                 L.att = fresh | done ? 1u : L.att + 1u;
+                L.redo = act ? redo : 0u;
+                cmp = L.active && redo == 0;
             }
             L.tl = tl;
             L.tr = tr;
-            L.redo = act ? redo : 0u;
-            const uint32_t cmp = act & (redo == 0 ? 1u : 0u);  // division complete this step
 
             // daughters (proliferation.cu:317-381): out of time -> tallied,
-            // f/2 <= phi -> dropped, else divides later
+            // f/2 <= phi -> dropped, else divides later.  The flags below are
+            // lane masks (scalar registers), so their logic runs on the
+            // scalar unit.
             const double sl = L.s + tl;
             const double sr = L.s + tr;
             const uint32_t k1 = L.k + 1;
-            const uint32_t rec_l = sl > t_max ? 1u : 0u;
-            const uint32_t rec_r = sr > t_max ? 1u : 0u;
-            const uint32_t can = k1 < L.kcap ? 1u : 0u;
-            const uint32_t div_l = (rec_l ^ 1u) & can;
-            const uint32_t div_r = (rec_r ^ 1u) & can;
-            const uint32_t nrec = cmp * (rec_l + rec_r);
+            const bool rec_l = sl > t_max;
+            const bool rec_r = sr > t_max;
+            const bool can = k1 < L.kcap;
+            const bool div_l = !rec_l && can;
+            const bool div_r = !rec_r && can;
+            const uint32_t nrec = cmp ? (uint32_t)rec_l + (uint32_t)rec_r : 0u;
             const bool in_win = L.tb_lds != kNoWindow;
             // lanes without a completed division add 0 to the spare slot win[win_len]
             atomicAdd(&win[(cmp && in_win) ? L.tb_lds + __umul24(k1, T) : win_len], nrec);
@@ -572,19 +592,20 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
                                        (L.rowtf >> 24) + (uint64_t)k1 * T],
                               (unsigned long long)nrec);
             }
-            // right daughter's completion time; level k1 holds no pending
-            // entry of an active lane (idle lanes have none at all), so the
-            // store is harmless when it is not pushed
-            stack[__umul24(act ? L.k : 0u, bdim) + tid] = sr;
-            const uint32_t go_l = cmp & div_l;
-            const uint32_t go_r = cmp & (div_l ^ 1u) & div_r;
-            const uint32_t pop = cmp & (div_l ^ 1u) & (div_r ^ 1u);
+            // right daughter's completion time, at level k1; level k1 holds
+            // no pending entry of an active lane, and idle lanes have none at
+            // all (their L.k is at most 1 or a level they popped, below the
+            // stack's depth), so the store is harmless when it is not pushed
+            *(double *)(stk_push + __umul24(L.k, stride)) = sr;
+            const bool go_l = cmp && div_l;
+            const bool go_r = cmp && !div_l && div_r;
+            const bool pop = cmp && !div_l && !div_r;
             // (k1 stays below the width of N for a lane that pushes; the
             // mask keeps the shift defined for the others)
-            const N pend2 = L.pend | ((N)(go_l & div_r) << (k1 & (sizeof(N) * 8 - 1)));
-            const uint32_t has = pend2 != 0 ? 1u : 0u;
+            const N pend2 = L.pend | ((N)(go_l && div_r) << (k1 & (sizeof(N) * 8 - 1)));
+            const bool has = pend2 != 0;
             // single-level selects only (nested ones turn into branches)
-            const uint32_t down = go_l | go_r;
+            const bool down = go_l || go_r;
             const N child = (L.node << 1) | (N)go_r;
             const uint32_t sh = pop ? ((L.k - lvl) & (sizeof(N) * 8 - 1)) : 0u;
             const N upn = (L.node >> sh) | (N)pop;  // pending right daughter
@@ -596,15 +617,11 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
             L.s = down ? s_child : s_up;
             L.k = down ? k1 : k_up;
             L.pend = pend2 & ~clear;
-            L.active = act && !(pop && !has);
-            // lanes that went idle take buffered roots at once
-            if (rpos < rn) {
-                const uint64_t idle2 = __ballot(!L.active);
-                hand_out(idle2, uni((uint32_t)__popcll(idle2)));
-            }
+            L.active = L.active && !(pop && !has);
         }
     }
 
+    n.div += lane == 0 ? w_div : 0u;
     flush_counts(n, a.counters + (size_t)set * kCounterCopies * kNumCounters, lane, red);
     const uint64_t gbase = S.tally_base + (uint64_t)(row_lo - S.row_base) * kdim * T;
     for (uint32_t i = tid; i < win_len; i += bdim) {
