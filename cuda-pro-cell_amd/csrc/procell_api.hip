@@ -226,9 +226,10 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     LaunchShape &ls = pl->shape;
     RunArgs &a = pl->args;
     a.stack_levels = kdim > 3 ? kdim - 1 : 2;  // levels 1 .. kmax, at least 2 (idle lanes store at level 2)
-    // row tables (8 bytes per row) go to LDS when they are small
-    const uint32_t lds_rows = (size_t)(2 * n_rows + 1) * 4 <= 16384 ? n_rows : 0u;
-    a.rows_in_lds = lds_rows ? 1u : 0u;
+    // row tables (8 bytes per row) go to LDS when they are small and do not
+    // cost a resident workgroup (the root steps read them when a wavefront's
+    // rows advance, so from device memory they cost little)
+    uint32_t lds_rows = (size_t)(2 * n_rows + 1) * 4 <= 16384 ? n_rows : 0u;
     int bs = tu.block_size ? (int)tu.block_size : 256;
     int occ = 0;
     for (;;) {
@@ -244,10 +245,24 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
             return fail(PROCELL_E_DEPTH);
         bs -= 64;  // deep trees: a smaller workgroup keeps the stack within LDS
     }
-    // by default at most 12 resident wavefronts per CU (3 per SIMD): more
-    // shrink every workgroup's tally window without adding issue rate
-    // (measured on the 1e7-cell histogram workload)
-    occ = std::min<int>(occ, tu.blocks_per_cu ? (int)tu.blocks_per_cu : std::max(1, 12 * 64 / bs));
+    // by default at most 16 resident wavefronts per CU (4 per SIMD, the
+    // kernel's register budget): 4 workgroups of 256 per CU against 3 measured
+    // 0.2592 -> 0.2470 ms on the 1e7-cell histogram workload (more waves hide
+    // the division step's dependency waits; the smaller tally windows cost
+    // less than that)
+    const int want = tu.blocks_per_cu ? (int)tu.blocks_per_cu : std::max(1, 16 * 64 / bs);
+    if (lds_rows && occ < want) {
+        int occ_g = 0;
+        if (sim_occupancy(bs, sim_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, 0), false, kdim, &occ_g) !=
+            hipSuccess)
+            return fail(PROCELL_E_HIP);
+        if (occ_g > occ) {
+            lds_rows = 0;
+            occ = occ_g;
+        }
+    }
+    a.rows_in_lds = lds_rows ? 1u : 0u;
+    occ = std::min<int>(occ, want);
     ls.block_size = bs;
     const size_t base = sim_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
     const size_t budget = std::min(lds_per_cu / (size_t)occ, lds_per_block);
@@ -727,7 +742,7 @@ int procell_plan_autotune(procell_plan *plan, const uint64_t *seeds, uint32_t re
     // layouts: resident workgroups per CU (sets the LDS tally window's
     // width) x segments per resident workgroup; for each, the idle-lane
     // threshold, which needs no new layout
-    static const uint32_t kBlocks[] = {2, 3};
+    static const uint32_t kBlocks[] = {2, 3, 4};
     static const uint32_t kSegments[] = {1, 2, 3, 4, 8};
     static const uint32_t kIdle[] = {6, 8, 12, 16};
     hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -201,7 +201,7 @@ __host__ __device__ constexpr size_t sim_wave_lds_bytes()
 constexpr size_t kSimSharedBytes = 48;  // counter sums u32 [8], root step cursor u32 [4]
 
 template <bool RowsLds, typename N>
-__global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
+__global__ void __launch_bounds__(256, 4) sim_kernel(RunArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t tid = threadIdx.x;
@@ -269,12 +269,12 @@ __global__ void __launch_bounds__(256, 3) sim_kernel(RunArgs a)
 
     uint32_t rpos = 0, rn = 0;      // buffered dividing roots: entries [rpos, rn) of ent
     uint32_t roots_done = 0;
-    // the first eight cumulative proportions of the type search, in
+    // the first eight cumulative proportions of the type search, in scalar
     // registers (-inf for types not searched: the last one and those beyond T)
     double cum8[8];
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k)
-        cum8[k] = k + 1 < T ? t_cum[k] : -__builtin_inf();
+        cum8[k] = __builtin_bit_cast(double, uni64(__builtin_bit_cast(uint64_t, k + 1 < T ? t_cum[k] : -__builtin_inf())));
     // Row boundaries of the wavefront's root steps, in registers: lane j holds
     // the first cell of row rb + 1 + j (~0 past the last row) and the info of
     // row rb + j.  A wavefront's root steps only move forward in its segment,

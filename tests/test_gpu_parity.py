@@ -245,8 +245,8 @@ def test_autotuned_plan_matches_oracle(procell_lib, oracle):
     with procell_lib.Plan(gp) as plan:
         chosen = plan.autotune([77], reps=1)
         print("chosen", chosen, flush=True)
-        assert chosen["blocks_per_cu"] in (2, 3) and chosen["segments"] in (1, 2, 3, 4, 8)
-        assert chosen["idle_lanes"] in (8, 12, 16, 24)
+        assert chosen["blocks_per_cu"] in (2, 3, 4) and chosen["segments"] in (1, 2, 3, 4, 8)
+        assert chosen["idle_lanes"] in (6, 8, 12, 16)
         for seed in (77, 78):
             plan.run(seed)
             res = plan.result()
