@@ -86,6 +86,12 @@ constexpr uint32_t kFrameDoubles = 8;
 // written per expansion 26.5 -> 10.6 B, read 25.7 -> 8.7 B, for 1% more time
 // (14.09 -> 14.24 ms; two slots beside 384 doubles measured 2% slower, the
 // cache in the stride-4 Jacobian pass 7% slower).
+// Idle lanes a wavefront tolerates before it claims walks for them: a claim
+// costs the wavefront about 50 instructions, so claiming for every idle lane
+// at every step costs more than the idle lanes.  256 sets x 2e5 cells
+// (with the merged redraw loop): calibration step 23.96 ms (0), 23.38 (4),
+// 23.33 (8), 24.06 (16).
+constexpr uint32_t kRefillIdle = 8;
 constexpr uint32_t kTanFields = 6;
 constexpr uint32_t kTanSlots = 2;  // adjoint
 constexpr uint32_t kJacSlots = 1;  // Jacobian pass
@@ -165,6 +171,41 @@ enum WalkMode : int {
 // 1 / x for x >= 1: the hardware reciprocal estimate refined by two Newton
 // steps (each squares the relative error), within an ulp or two of the
 // quotient.
+// exp(x) for -40 <= x <= 700 (below -40, 1 + exp(x) rounds to 1): x = k ln 2
+// + r with |r| <= ln 2 / 2 (Cody-Waite reduction, ln 2 in two parts), exp(r)
+// by its Taylor polynomial of degree 12 (truncation below 2e-16 relative),
+// times 2^k.  Horner steps p = fma(p, r, c) keep each coefficient in its
+// register (no copy before a two-address fma).
+// p r + c as one three-address fma (the compiler otherwise ties the result
+// to a copy of the loop-invariant coefficient)
+__device__ __forceinline__ double horner(double p, double r, double c)
+{
+    double out;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(out) : "v"(p), "v"(r), "v"(c));
+    return out;
+}
+
+__device__ __forceinline__ double exp_sig(double x)
+{
+    const double k = __builtin_rint(x * 1.4426950408889634);
+    double r = fma(k, -6.93147180369123816490e-01, x);
+    r = fma(k, -1.90821492927058770002e-10, r);
+    double p = 2.08767569878681e-09;
+    p = horner(p, r, 2.505210838544172e-08);
+    p = horner(p, r, 2.755731922398589e-07);
+    p = horner(p, r, 2.7557319223985893e-06);
+    p = horner(p, r, 2.48015873015873e-05);
+    p = horner(p, r, 0.0001984126984126984);
+    p = horner(p, r, 0.001388888888888889);
+    p = horner(p, r, 0.008333333333333333);
+    p = horner(p, r, 0.041666666666666664);
+    p = horner(p, r, 0.16666666666666666);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)k);
+}
+
 __device__ __forceinline__ double recip_ge1(double x)
 {
     double r = __builtin_amdgcn_rcp(x);
@@ -177,7 +218,7 @@ __device__ __forceinline__ double recip_ge1(double x)
 // inf x 0); there the sigmoid is below 1e-304 either way.
 __device__ __forceinline__ double sigmoid_scaled(double y, double ib)
 {
-    return recip_ge1(1.0 + exp(fmin(-(y * ib), 700.0)));
+    return recip_ge1(1.0 + exp_sig(fmin(fmax(-(y * ib), -40.0), 700.0)));
 }
 
 template <int Mode, bool Strided>
@@ -273,12 +314,13 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     const uint32_t sphase = Strided ? ((a.step ? (uint32_t)a.step[0] : 0u) + a.phase) & smask : 0u;
 
     for (;;) {
-        // ---- refill idle lanes from the wavefront's range (with a cell
-        //      stride, claimed walks of skipped cells leave their lane idle
-        //      and it claims again, at most stride rounds per step) ----
+        // ---- refill idle lanes from the wavefront's range once more than
+        //      kRefillIdle are idle (or all are); with a cell stride, claimed
+        //      walks of skipped cells leave their lane idle and it claims
+        //      again, at most stride rounds per step ----
         for (uint32_t round = 0; round <= smask; ++round) {
             const uint64_t idle = __ballot(!active);
-            if (!idle || next >= u_end)
+            if (!idle || next >= u_end || (__popcll(idle) <= kRefillIdle && ~idle != 0))
                 break;
             if (!active) {
                 const uint64_t u = next + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
@@ -339,29 +381,43 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
         // its second candidate, then node 1's stream; a daughter's to its own
         // node's stream; after 65535 blocks the timer is the mean (z = 0)
         {
-            const bool redoA = !(fma(z0, sd, mean) > 0.0);
+            bool redoA = !(fma(z0, sd, mean) > 0.0);
             const bool redoB = !pre && !(fma(z1, sd, mean) > 0.0);
             if (__builtin_expect(__ballot(redoA || redoB) != 0, 0)) {
-                auto redraw = [&](uint64_t rnode) -> double {
-                    for (uint32_t att = 1; att < 65536u; ++att) {
+                // the root's second candidate first
+                if (pre && redoA && fma(z1, sd, mean) > 0.0) {
+                    z0 = z1;
+                    redoA = false;
+                }
+                // one loop for both daughters: each round draws one block per
+                // lane, of A's stream while A is pending, then of B's (a lane
+                // needing both takes them in turn; the values do not depend on
+                // the order); a stream gives up after 65535 blocks (z = 0)
+                uint32_t need = (redoA ? 1u : 0u) | (redoB ? 2u : 0u);
+                uint32_t att = 1;
+                while (__ballot(need != 0) != 0) {
+                    if (need != 0) {
+                        const bool onA = (need & 1u) != 0;
+                        const uint64_t rnode = onA ? (pre ? 1ull : 2 * node) : 2 * node + 1;
                         const dev::Words wr = dev::draw_block(S.seed, cell, rnode, att);
                         double y0, y1;
                         dev::normal_pair(wr, y0, y1, K, tb);
-                        if (fma(y0, sd, mean) > 0.0)
-                            return y0;
-                        if (fma(y1, sd, mean) > 0.0)
-                            return y1;
+                        const bool ok0 = fma(y0, sd, mean) > 0.0;
+                        const bool ok1 = fma(y1, sd, mean) > 0.0;
+                        const bool done = ok0 || ok1 || att == 65535u;
+                        const double v = ok0 ? y0 : (ok1 ? y1 : 0.0);
+                        if (done) {
+                            if (onA)
+                                z0 = v;
+                            else
+                                z1 = v;
+                            need &= onA ? ~1u : ~2u;
+                            att = 1;
+                        } else {
+                            ++att;
+                        }
                     }
-                    return 0.0;
-                };
-                if (redoA) {
-                    if (pre)
-                        z0 = fma(z1, sd, mean) > 0.0 ? z1 : redraw(1);
-                    else
-                        z0 = redraw(2 * node);
                 }
-                if (redoB)
-                    z1 = redraw(2 * node + 1);
             }
         }
 
@@ -695,6 +751,10 @@ struct RelaxCore {
     size_t lds_bytes = 0;  // every walk kernel but the two below
     size_t lds_adj = 0;    // the adjoint without a stride
     size_t lds_jac = 0;    // the Jacobian pass without a stride
+    // the wavefront layout of the strided and Jacobian passes (equal-cost ranges)
+    const uint32_t *eq_set = nullptr;
+    const uint64_t *eq_unit = nullptr;
+    const uint32_t *eq_row = nullptr;
     bool any_prolif = false, any_quiet = false;
     uint64_t h_total = 0;      // tally entries of all sets
     uint32_t types_total = 0;  // types of all sets
@@ -717,7 +777,7 @@ void core_free(RelaxCore &c)
 // Splits walks [0, n_units) of one set into n_w ranges of equal estimated
 // cost (unit u = cell u / n_prolif, type prolif[u % n_prolif]).
 void split_set(const Setup &s, const std::vector<double> &unit_cost, uint32_t n_prolif, uint32_t n_w,
-               uint32_t row_base, uint64_t *unit_out, uint32_t *row_out)
+               uint32_t row_base, uint64_t *unit_out, uint32_t *row_out, const double *cumw = nullptr)
 {
     const uint32_t n_rows = (uint32_t)s.v.size();
     const uint64_t n_units = s.n_cells * (uint64_t)n_prolif;
@@ -732,7 +792,7 @@ void split_set(const Setup &s, const std::vector<double> &unit_cost, uint32_t n_
         const uint64_t u0 = s.start[i] * n_prolif, u1 = s.start[i + 1] * n_prolif;
         const double per = unit_cost[i];
         while (wv < n_w) {
-            const double need = (total * (double)wv / (double)n_w - acc) / per;
+            const double need = (total * (cumw ? cumw[wv - 1] : (double)wv / (double)n_w) - acc) / per;
             if (need >= (double)(u1 - u0))
                 break;
             uint64_t c = u0 + (uint64_t)std::max(0.0, ceil(need));
@@ -888,15 +948,67 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     const uint32_t NW = std::max(1u, c.n_waves);
     std::vector<uint32_t> wave_set(NW, 0), wave_row(2 * NW, 0);
     std::vector<uint64_t> wave_unit(2 * NW, 0);
-    uint32_t wpos = 0;
+    // Two layouts of the same number of wavefronts.  Passes with a cell
+    // stride and the Jacobian pass take ranges of equal cost, set by set;
+    // the full forward and adjoint passes take ranges that shrink
+    // geometrically over rounds (the last 1/8 of the first), launched round
+    // by round over all sets, so that the wavefronts dispatched last are
+    // short.  256 sets x 2e5 cells: calibration step 23.26 -> 22.48 ms,
+    // forward 10.50 -> 10.28, adjoint 13.17 -> 13.03 (the strided passes
+    // measured 12 % slower and the Jacobian pass 3 % slower with it: their
+    // per-wavefront set-up weighs more in short ranges).
+    std::vector<uint32_t> eq_set(NW, 0), eq_row(2 * NW, 0);
+    std::vector<uint64_t> eq_unit(2 * NW, 0);
+    {
+        uint32_t wpos = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!waves_of[i])
+                continue;
+            split_set(c.setups[i], ucost[i], c.sets[i].n_prolif, waves_of[i], c.sets[i].row_base,
+                      &eq_unit[2 * wpos], &eq_row[2 * wpos]);
+            for (uint32_t v = 0; v < waves_of[i]; ++v)
+                eq_set[wpos + v] = i;
+            wpos += waves_of[i];
+        }
+    }
+    struct Piece {
+        uint32_t round, set;
+        uint64_t u0, u1;
+        uint32_t r0, r1;
+    };
+    std::vector<Piece> pieces;
+    pieces.reserve(NW);
+    const uint32_t R = std::max(1u, wave_factor);
     for (uint32_t i = 0; i < n; ++i) {
-        if (!waves_of[i])
+        const uint32_t nw = waves_of[i];
+        if (!nw)
             continue;
-        split_set(c.setups[i], ucost[i], c.sets[i].n_prolif, waves_of[i], c.sets[i].row_base,
-                  &wave_unit[2 * wpos], &wave_row[2 * wpos]);
-        for (uint32_t v = 0; v < waves_of[i]; ++v)
-            wave_set[wpos + v] = i;
-        wpos += waves_of[i];
+        const uint32_t rounds = std::min(nw, R);
+        const double q = rounds > 1 ? pow(0.125, 1.0 / (double)(rounds - 1)) : 1.0;
+        std::vector<double> wk(nw), cumw(nw);
+        double tot = 0.0;
+        for (uint32_t k = 0; k < nw; ++k) {
+            wk[k] = pow(q, (double)((uint64_t)k * rounds / nw));
+            tot += wk[k];
+        }
+        double acc = 0.0;
+        for (uint32_t k = 0; k < nw; ++k) {
+            acc += wk[k];
+            cumw[k] = k + 1 == nw ? 1.0 : acc / tot;
+        }
+        std::vector<uint64_t> wu(2 * nw);
+        std::vector<uint32_t> wr(2 * nw);
+        split_set(c.setups[i], ucost[i], c.sets[i].n_prolif, nw, c.sets[i].row_base, wu.data(), wr.data(), cumw.data());
+        for (uint32_t k = 0; k < nw; ++k)
+            pieces.push_back(Piece{(uint32_t)((uint64_t)k * rounds / nw), i, wu[2 * k], wu[2 * k + 1], wr[2 * k], wr[2 * k + 1]});
+    }
+    std::stable_sort(pieces.begin(), pieces.end(), [](const Piece &x, const Piece &y) { return x.round < y.round; });
+    for (uint32_t v = 0; v < (uint32_t)pieces.size(); ++v) {
+        wave_set[v] = pieces[v].set;
+        wave_unit[2 * v] = pieces[v].u0;
+        wave_unit[2 * v + 1] = pieces[v].u1;
+        wave_row[2 * v] = pieces[v].r0;
+        wave_row[2 * v + 1] = pieces[v].r1;
     }
 
     // ---- upload ----
@@ -909,6 +1021,7 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     const size_t o_sets = take(c.sets.size() * sizeof(RelaxSet));
     const size_t o_ws = take(wave_set.size() * 4), o_wu = take(wave_unit.size() * 8);
     const size_t o_wr = take(wave_row.size() * 4), o_rs = take(row_start.size() * 4);
+    const size_t o_es = take(eq_set.size() * 4), o_eu = take(eq_unit.size() * 8), o_er = take(eq_row.size() * 4);
     const size_t o_ri = take(row_info.size() * 4 + 4), o_rt = take(row_set.size() * 4 + 4);
     const size_t o_pr = take(prolif.size() * 4 + 4), o_qu = take(quiet.size() * 4 + 4);
     const size_t o_ex = take(16 + (size_t)n * 8);  // expansions, walks cut, walks cut per set
@@ -922,6 +1035,9 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     RELAX_TRY(up(o_ws, wave_set.data(), wave_set.size() * 4));
     RELAX_TRY(up(o_wu, wave_unit.data(), wave_unit.size() * 8));
     RELAX_TRY(up(o_wr, wave_row.data(), wave_row.size() * 4));
+    RELAX_TRY(up(o_es, eq_set.data(), eq_set.size() * 4));
+    RELAX_TRY(up(o_eu, eq_unit.data(), eq_unit.size() * 8));
+    RELAX_TRY(up(o_er, eq_row.data(), eq_row.size() * 4));
     RELAX_TRY(up(o_rs, row_start.data(), row_start.size() * 4));
     RELAX_TRY(up(o_ri, row_info.data(), row_info.size() * 4));
     RELAX_TRY(up(o_rt, row_set.data(), row_set.size() * 4));
@@ -936,6 +1052,9 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
     a.wave_set = (const uint32_t *)(base + o_ws);
     a.wave_unit = (const uint64_t *)(base + o_wu);
     a.wave_row = (const uint32_t *)(base + o_wr);
+    c.eq_set = (const uint32_t *)(base + o_es);
+    c.eq_unit = (const uint64_t *)(base + o_eu);
+    c.eq_row = (const uint32_t *)(base + o_er);
     a.row_start = (const uint32_t *)(base + o_rs);
     a.row_info = (const uint32_t *)(base + o_ri);
     a.row_set = (const uint32_t *)(base + o_rt);
@@ -956,9 +1075,15 @@ int core_build(RelaxCore &c, const procell_params *ps, uint32_t n, double beta, 
 // Enqueues one pass of a WalkMode: the walk kernel and the quiescent kernel
 // (the caller has cleared h, the Jacobian or the gradient, and the expansion
 // counter).
-hipError_t core_launch(const RelaxCore &c, const RelaxArgs &a, int mode, hipStream_t st)
+hipError_t core_launch(const RelaxCore &c, const RelaxArgs &a_in, int mode, hipStream_t st)
 {
     const bool grad = mode == kAdjoint;
+    RelaxArgs a = a_in;
+    if (a.stride_mask || mode == kJacobian) {
+        a.wave_set = c.eq_set;
+        a.wave_unit = c.eq_unit;
+        a.wave_row = c.eq_row;
+    }
     if (c.any_prolif && c.n_waves) {
         const dim3 g(c.n_waves), b(64);
         if (mode == kJacobian) {
