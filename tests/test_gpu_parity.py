@@ -93,7 +93,9 @@ def test_large_histogram_bitexact(procell_lib, oracle):
 
 @pytest.mark.parametrize("knobs", [dict(block_size=64, segments=1), dict(block_size=128, segments=7),
                                    dict(block_size=256, segments=64, blocks_per_cu=1),
-                                   dict(block_size=192, segments=33, blocks_per_cu=2, idle_lanes=1)])
+                                   dict(block_size=192, segments=33, blocks_per_cu=2, idle_lanes=1),
+                                   dict(block_size=256, segments=3, blocks_per_cu=3, idle_lanes=20),
+                                   dict(block_size=256, segments=2, blocks_per_cu=4, idle_lanes=12)])
 def test_launch_knobs_do_not_change_results(procell_lib, oracle, knobs):
     from procell import workloads
 
