@@ -147,7 +147,7 @@ __device__ __forceinline__ uint32_t low_bit(uint64_t m) { return (uint32_t)__bui
 // 31 generations, else 64.
 template <typename N>
 struct Lane {
-    uint64_t cell;
+    uint32_t cell;      // initial cell, relative to its set (cell ids < 2^32)
     N node, pend;
     uint32_t rowtf;     // row relative to the set's first row | file type << 24
     double s, mean, sd;
@@ -380,16 +380,22 @@ __global__ void __launch_bounds__(256, 4) sim_kernel(RunArgs a)
                     const bool quiescent = m < 0.0;
                     // first timer (cell.cu:106-122): two candidates from the root
                     // block, then node 1's redraw stream
+                    // (the second candidate only where the first is <= 0: its
+                    // sine is computed there alone)
                     double z0, z1;
                     dev::normal_pair<true>(Words{w.w2, 0u, w.w3, 0u}, z0, z1, K, tb);
                     const double c0 = fma(z0, sdv, m);
-                    const double c1 = fma(z1, sdv, m);
                     const bool live = valid && !quiescent;
-                    n.rej += live ? (uint32_t)!(c0 > 0.0) + (uint32_t)(!(c0 > 0.0) && !(c1 > 0.0)) : 0u;
-                    double tau0 = c0 > 0.0 ? c0 : c1;
-                    if (__builtin_expect(__ballot(live && !(tau0 > 0.0)) != 0, 0)) {
+                    double tau0 = c0;
+                    if (__builtin_expect(__ballot(live && !(c0 > 0.0)) != 0, 0)) {
+                        if (live && !(c0 > 0.0)) {
+                            ++n.rej;
+                            const double c1 = fma(z1, sdv, m);
+                            tau0 = c1;
+                        }
                         if (live && !(tau0 > 0.0)) {
                             // rare: node 1's redraw stream from attempt 1
+                            ++n.rej;
                             tau0 = m;
                             for (uint32_t att = 1; att < 65536u; ++att) {
                                 const Words wr = dev::draw_block(seed, c - S.cell_base, 1, att);
@@ -463,7 +469,7 @@ __global__ void __launch_bounds__(256, 4) sim_kernel(RunArgs a)
                     const uint32_t lvl = __shfl(my_lvl, (int)src);
                     const N d_node = __shfl(L.node, (int)src);
                     const uint32_t d_k = __shfl(L.k, (int)src);
-                    const uint64_t d_cell = __shfl(L.cell, (int)src);
+                    const uint32_t d_cell = __shfl(L.cell, (int)src);
                     const uint32_t d_kcap = __shfl(L.kcap, (int)src);
                     const uint32_t d_tbl = __shfl(L.tb_lds, (int)src);
                     const uint32_t d_rtf = __shfl(L.rowtf, (int)src);
@@ -526,7 +532,7 @@ __global__ void __launch_bounds__(256, 4) sim_kernel(RunArgs a)
                 // busy lane starts a new division with its pair block; every
                 // lane's redo is 0 and its attempt number 1 (attempts restart
                 // at 1 whenever a redraw ends)
-                const Words w = dev::draw_block(seed, L.cell, L.node, 0u);
+                const Words w = dev::draw_block(seed, (uint64_t)L.cell, L.node, 0u);
                 double z0, z1;
                 dev::normal_pair(w, z0, z1, K, tb);
                 tl = fma(z0, L.sd, L.mean);
@@ -545,7 +551,7 @@ __global__ void __launch_bounds__(256, 4) sim_kernel(RunArgs a)
                 const uint32_t fresh = L.redo == 0 ? 1u : 0u;
                 const uint32_t rright = (L.redo & 1u) ? 0u : 1u;   // redraw the right daughter?
                 const N bnode = fresh ? L.node : (L.node << 1) | (N)rright;
-                const Words w = dev::draw_block(seed, L.cell, bnode, fresh ? 0u : L.att);
+                const Words w = dev::draw_block(seed, (uint64_t)L.cell, bnode, fresh ? 0u : L.att);
                 double z0, z1;
                 dev::normal_pair(w, z0, z1, K, tb);
                 const double c0 = fma(z0, L.sd, L.mean);
