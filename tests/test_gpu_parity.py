@@ -159,6 +159,18 @@ def test_heavy_truncation(procell_lib, oracle):
     assert st["rejected"] > 1000
 
 
+@pytest.mark.parametrize("top", [1.0, 3.0, 7.0, 15.0])
+def test_shallow_generation_caps(procell_lib, oracle, top):
+    """Values a few halvings above phi = 1 (generation caps 0..3, so the
+    per-lane stack is one to three levels deep and idle lanes park on its
+    last level); t_max 9 against mean timers of 4 and 6, so cells end both
+    out of time (recorded) and at the cap (dropped)."""
+    vals = np.array([1.0, 2.0, 3.0, 5.0, 8.0, 15.0]) * (top / 15.0) + 1.0
+    gp, op = _problem_pair(procell_lib, oracle, vals, [3000, 2000, 1500, 1000, 800, 500],
+                           [0.6, 0.4], [4.0, 6.0], [1.5, 2.0], 9.0, 1.0, 17)
+    assert_same(procell_lib, oracle, gp, op)
+
+
 def test_rows_below_phi_and_duplicate_values(procell_lib, oracle):
     # 40 and 20 halve onto shared values; 3 is below phi; unsorted file order
     gp, op = _problem_pair(procell_lib, oracle, [40.0, 3.0, 20.0, 160.0, 20.0], [300, 200, 100, 50, 7],
