@@ -270,7 +270,9 @@ __global__ void __launch_bounds__(256, 4) sim_kernel(RunArgs a)
     uint32_t rpos = 0, rn = 0;      // buffered dividing roots: entries [rpos, rn) of ent
     uint32_t roots_done = 0;
     // the first eight cumulative proportions of the type search, in scalar
-    // registers (-inf for types not searched: the last one and those beyond T)
+    // registers (-inf for types not searched: the last one and those beyond T;
+    // in vector registers they would take the 16 that keep the kernel within
+    // the 128 of four wavefronts per SIMD)
     double cum8[8];
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k)
