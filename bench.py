@@ -38,9 +38,11 @@ LANE_OPS_PER_DIVISION = 180
 # the Philox block, two uniforms and the Box-Muller pair as for a division
 # (98 + 8 + 60), both daughters' times (4), two sigmoids (2 x (scale, exp
 # ~22, add, reciprocal + 2 Newton steps ~5) ~ 56) and the two records (8);
-# the adjoint adds the tangents and gradient sums of both daughters (34).
+# the adjoint adds the tangents and gradient sums of both daughters (34); the
+# Jacobian pass adds the tangents and two more records of both daughters (42).
 LANE_OPS_PER_EXPANSION_FWD = 234
 LANE_OPS_PER_EXPANSION_ADJ = 268
+LANE_OPS_PER_EXPANSION_JAC = 276
 # MI355X VALU issue peak: 256 CUs x 64 lanes/clk x 2.4 GHz (lane-ops/s).
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "sim_kernel_hist1024_1e7_hbm_traffic.txt")
@@ -318,8 +320,8 @@ def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_roun
                             with_cpu=True, lm_iters=15, rich_sets=64):
     """Batched calibration: 256 candidate parameter sets x 2e5 cells, 2-4
     types, t_max U(96,240), targets from the plain method at the generating
-    parameters; 200 steps of relaxed forward + loss + adjoint + on-device
-    Adam replayed from one hipGraph per step.  Beside it, the derivative-free
+    parameters; 200 steps of the relaxed Jacobian pass + loss and gradient
+    (J^T dL/dh) + on-device Adam replayed from one hipGraph per step.  Beside it, the derivative-free
     baseline: Nelder-Mead per set over the plain method (common random
     numbers), all sets evaluated by one batched plain run per round."""
     from procell import calibrate
@@ -360,8 +362,7 @@ def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_roun
         "set_steps_per_s": n * steps / gpu_s,
         "expansions_per_step": exp_step,
         "expansions_per_s": exp_step * steps / gpu_s,
-        "roofline_frac_nominal": exp_step / 2 * (LANE_OPS_PER_EXPANSION_FWD + LANE_OPS_PER_EXPANSION_ADJ)
-                                 / (gpu_s / steps) / VALU_PEAK_LANE_OPS,
+        "roofline_frac_nominal": exp_step * LANE_OPS_PER_EXPANSION_JAC / (gpu_s / steps) / VALU_PEAK_LANE_OPS,
         "optimizer": "Adam lr 0.02 (log mean, log sd, proportion logits), on the device",
         "targets": "plain method at the generating parameters, seeds 1000+i; relaxed seeds 50000+i",
         "trajectory": traj,

@@ -1146,8 +1146,10 @@ struct CalibArgs {
     double *hb;                // per-bin scratch: relaxed histogram, then dL/dhb
     double *loss;              // [n_sets]
     const double *h;
-    double *g;
-    const double *grad;        // dL/dtheta per set [3][T]
+    double *g;                 // dL/dh, written for the adjoint pass (null with jac)
+    double *grad;              // dL/dtheta per set [3][T]
+    const double *jac;         // dh/dmean | dh/dsd, each laid out like h; null: adjoint route
+    uint64_t h_total;
     double *x;                 // per set [3][T]: log mean, log sd, logits
     double *m1, *m2;           // Adam moments, layout of x
     double *theta;             // per set [3][T]
@@ -1207,15 +1209,45 @@ __global__ __launch_bounds__(256) void calib_loss_kernel(CalibArgs c)
     for (uint32_t b = threadIdx.x; b < C.n_bins; b += blockDim.x)
         hb[b] = tot > 0.0 ? 2.0 * inv * (hb[b] - Cc) : 0.0;
     __syncthreads();
-    double *g = c.g + S.h_base;
-    for (uint32_t e = threadIdx.x; e < C.n_ent; e += blockDim.x) {
-        const int32_t b = c.ent_bin[C.ent_base + e];
-        const double gv = b >= 0 ? hb[b] : 0.0;
-        for (uint32_t j = 0; j < T; ++j)
-            g[(uint64_t)e * T + j] = gv;
-    }
     if (threadIdx.x == 0)
         c.loss[set] = L;
+    if (!c.jac) {
+        double *g = c.g + S.h_base;
+        for (uint32_t e = threadIdx.x; e < C.n_ent; e += blockDim.x) {
+            const int32_t b = c.ent_bin[C.ent_base + e];
+            const double gv = b >= 0 ? hb[b] : 0.0;
+            for (uint32_t j = 0; j < T; ++j)
+                g[(uint64_t)e * T + j] = gv;
+        }
+        return;
+    }
+    // Jacobian route: dL/dtheta = J^T dL/dh.  An entry depends on its own
+    // type's parameters only, and h is linear in p_j (dh/dp_j = h / p_j).
+    const double *jm = c.jac + S.h_base, *js = c.jac + c.h_total + S.h_base;
+    const double *th = c.theta + S.theta_base;
+    double *gr = c.grad + S.theta_base;
+    for (uint32_t j = 0; j < T; ++j) {
+        double sm = 0.0, ss = 0.0, sp = 0.0;
+        for (uint32_t e = threadIdx.x; e < C.n_ent; e += blockDim.x) {
+            const int32_t b = c.ent_bin[C.ent_base + e];
+            if (b < 0)
+                continue;
+            const double gv = hb[b];
+            const uint64_t at = (uint64_t)e * T + j;
+            sm += gv * jm[at];
+            ss += gv * js[at];
+            sp += gv * h[at];
+        }
+        sm = block_sum_256(sm, red);
+        ss = block_sum_256(ss, red);
+        sp = block_sum_256(sp, red);
+        if (threadIdx.x == 0) {
+            const double pj = th[2 * T + j];
+            gr[j] = sm;
+            gr[T + j] = ss;
+            gr[2 * T + j] = pj > 0.0 ? sp / pj : 0.0;
+        }
+    }
 }
 
 // theta of one set from its unconstrained parameters.
@@ -1300,7 +1332,8 @@ struct procell_calib {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     CalibArgs cargs{};
-    RelaxArgs fwd{}, bwd{};
+    RelaxArgs fwd{}, bwd{};  // fwd is the Jacobian pass without a cell stride
+    bool jac_route = false;
     void *d_calib = nullptr;        // tables of the loss kernel
     void *d_state = nullptr;        // x, m1, m2, theta, grad, h, g, per-bin scratch, loss, step
     unsigned long long *d_clear = nullptr;  // h | grad | expansion counter: cleared every step
@@ -1361,18 +1394,21 @@ int relaxed_call(procell_relaxed *r, int mode, const double *theta, double *h, c
     return PROCELL_OK;
 }
 
-// One calibration step: clear, forward, loss and cotangent, adjoint, Adam.
+// One calibration step.  Without a cell stride: clear, the Jacobian pass (h
+// and dh/dtheta in one walk), loss and gradient J^T dL/dh, Adam.  With one:
+// clear, forward, loss and cotangent, adjoint on the next class of cells,
+// Adam.
 hipError_t calib_enqueue(procell_calib *c, hipStream_t st)
 {
     hipError_t e = launch_zero(c->d_clear, c->clear_words, st);
     if (e != hipSuccess)
         return e;
-    if ((e = core_launch(c->core, c->fwd, kForward, st)) != hipSuccess)
+    if ((e = core_launch(c->core, c->fwd, c->jac_route ? kJacobian : kForward, st)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(calib_loss_kernel, dim3(c->core.n_sets), dim3(256), 0, st, c->cargs);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    if ((e = core_launch(c->core, c->bwd, kAdjoint, st)) != hipSuccess)
+    if (!c->jac_route && (e = core_launch(c->core, c->bwd, kAdjoint, st)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(calib_adam_kernel, dim3(1), dim3(256), 0, st, c->cargs);
     return hipGetLastError();
@@ -1633,12 +1669,15 @@ int procell_calib_create(const procell_params *sets, uint32_t n_sets, double bet
         so = (so + bytes + 255) / 256 * 256;
         return o;
     };
-    // cleared every step: h | grad | expansions (one contiguous block)
+    // cleared every step: h | Jacobian (without a stride) | grad | expansions
+    // (one contiguous block)
+    c->jac_route = cell_stride == 1;
+    const size_t jac_words = c->jac_route ? 2ull * core.h_total : 0;
     const size_t s_h = stake(0);
-    const size_t clear_bytes = core.h_total * 8 + P * 8 + 16;
+    const size_t clear_bytes = (core.h_total + jac_words) * 8 + P * 8 + 16;
     so = (clear_bytes + 255) / 256 * 256;
     const size_t s_x = stake(P * 8), s_m1 = stake(P * 8), s_m2 = stake(P * 8), s_th = stake(P * 8);
-    const size_t s_g = stake(core.h_total * 8 + 8), s_hb = stake(n_bins_total * 8 + 8);
+    const size_t s_g = stake(c->jac_route ? 8 : core.h_total * 8 + 8), s_hb = stake(n_bins_total * 8 + 8);
     const size_t s_loss = stake(n_sets * 8), s_step = stake(8);
     if (hipMalloc(&c->d_state, so) != hipSuccess)
         return fail(PROCELL_E_OUT_OF_MEMORY);
@@ -1666,7 +1705,8 @@ int procell_calib_create(const procell_params *sets, uint32_t n_sets, double bet
     c->d_clear = (unsigned long long *)(sb + s_h);
     c->clear_words = clear_bytes / 8;
     double *d_h = (double *)(sb + s_h);
-    double *d_grad = d_h + core.h_total;
+    double *d_jac = c->jac_route ? d_h + core.h_total : nullptr;
+    double *d_grad = d_h + core.h_total + jac_words;
     unsigned long long *d_exp = (unsigned long long *)(d_grad + P);
 
     CalibArgs &ca = c->cargs;
@@ -1687,6 +1727,8 @@ int procell_calib_create(const procell_params *sets, uint32_t n_sets, double bet
     ca.h = d_h;
     ca.g = (double *)(sb + s_g);
     ca.grad = d_grad;
+    ca.jac = d_jac;
+    ca.h_total = core.h_total;
     ca.x = (double *)(sb + s_x);
     ca.m1 = (double *)(sb + s_m1);
     ca.m2 = (double *)(sb + s_m2);
@@ -1699,6 +1741,7 @@ int procell_calib_create(const procell_params *sets, uint32_t n_sets, double bet
     c->fwd.h = d_h;
     c->fwd.expansions = d_exp;
     c->fwd.set_cuts = nullptr;  // a calibration reports the step's total only
+    c->fwd.jac = d_jac;
     c->bwd = c->fwd;
     // with a stride, the adjoint walks the next class of cells: the residual
     // (from the forward pass) and the Jacobian then come from independent
