@@ -273,7 +273,9 @@ int procell_relaxed_destroy(procell_relaxed *plan);
  * squared L2 distance between the normalised relaxed histogram (h summed
  * over types, each (row, generation) entry merged into its output value
  * v / 2^k) and the normalised target; one step runs, for all sets at once,
- * the relaxed forward pass, the loss and its cotangent, the adjoint pass and
+ * the relaxed Jacobian pass (h and dh/dtheta in one walk), the loss and its
+ * gradient (the Jacobian contracted with dL/dh; with a cell stride k > 1:
+ * the forward pass, the loss and its cotangent, the adjoint pass) and
  * an Adam update of (log mean, log sd) of every proliferating type and the
  * proportion logits of every type (proportions = softmax of the logits), all
  * on the device, replayed from one hipGraph captured at the first step.
@@ -306,8 +308,9 @@ int procell_calib_step(procell_calib *calib, uint32_t n_steps);
 /* Waits; current parameters: per set [3][n_types] (mean, sd, proportion;
  * -1 for a quiescent type's mean and sd), sets concatenated. */
 int procell_calib_theta(procell_calib *calib, double *theta, uint64_t capacity);
-/* Waits; loss of each set at the last step's forward pass, the steps taken
- * and the expansions of the last step (forward and adjoint pass); steps and
+/* Waits; loss of each set at the last step's parameters, the steps taken
+ * and the expansions of the last step (its walks: the Jacobian pass, or
+ * with a cell stride the forward and adjoint passes); steps and
  * expansions may be NULL.  Returns PROCELL_E_DEPTH when a walk of the last
  * step exceeded 2^22 expansions and was cut short. */
 int procell_calib_loss(procell_calib *calib, double *loss, uint32_t capacity, uint64_t *steps,
