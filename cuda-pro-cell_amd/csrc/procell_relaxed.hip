@@ -1222,7 +1222,8 @@ __global__ __launch_bounds__(256) void calib_loss_kernel(CalibArgs c)
         return;
     }
     // Jacobian route: dL/dtheta = J^T dL/dh.  An entry depends on its own
-    // type's parameters only, and h is linear in p_j (dh/dp_j = h / p_j).
+    // type's parameters only, and h is linear in p_j (dh/dp_j = h / p_j; at
+    // p_j = 0 the entry is left 0: the Adam step multiplies it by p_j).
     const double *jm = c.jac + S.h_base, *js = c.jac + c.h_total + S.h_base;
     const double *th = c.theta + S.theta_base;
     double *gr = c.grad + S.theta_base;

@@ -423,6 +423,7 @@ class Relaxed:
         _check(lib().procell_relaxed_shape(self.handle, ctypes.byref(n), ctypes.byref(k), ctypes.byref(t)),
                "relaxed_shape")
         self.shape = (int(n.value), int(k.value), int(t.value))
+        self.stride = (1, 0)
 
     @staticmethod
     def _ptr(x, shape, what):
@@ -469,6 +470,7 @@ class Relaxed:
         """Later passes walk only the initial cells c with (c + phase) a
         multiple of `stride` (a power of two); 1 walks every cell."""
         _check(lib().procell_relaxed_set_stride(self.handle, int(stride), int(phase)), "relaxed_set_stride")
+        self.stride = (int(stride), int(phase) % int(stride))
 
     def set_cuts(self, cuts, stream=None):
         """Enqueues a copy of the last call's walks cut short, per set, into
@@ -533,6 +535,7 @@ class RelaxedBatch(Relaxed):
         last = self.shapes[-1]
         self.h_total = self.h_base[-1] + last[0] * last[1] * last[2]
         self.shape = (self.h_total,)
+        self.stride = (1, 0)
 
     def forward(self, theta, h, stream=None):
         _check(lib().procell_relaxed_forward(self.handle, self._ptr(theta, (3 * self.types_total,), "theta"),

@@ -1,11 +1,16 @@
 """Calibration binding: the relaxed estimator under torch.autograd.
 
-RelaxedTally is a torch.autograd.Function over the C ABI
-(procell_relaxed_forward / procell_relaxed_backward): its forward returns the
-relaxed expected tally h[row, generation, type] of the plan's histogram for
-theta = [mean; sd; proportion] (3 x n_types, float64, on the GPU), and its
-backward hands the incoming cotangent dL/dh to the library's adjoint pass,
-which recomputes every subtree from the random stream.  The helpers below
+RelaxedTally is a torch.autograd.Function over the C ABI: its forward returns
+the relaxed expected tally h[row, generation, type] of the plan's histogram
+for theta = [mean; sd; proportion] (3 x n_types, float64, on the GPU).  When
+theta needs a gradient and the plan walks every cell, the forward runs the
+Jacobian pass (procell_relaxed_jacobian: h, dh/dmean and dh/dsd in one walk)
+and the backward contracts it with the incoming cotangent dL/dh; otherwise
+the forward runs procell_relaxed_forward and the backward hands dL/dh to the
+library's adjoint pass (procell_relaxed_backward), which recomputes every
+subtree from the random stream -- with a cell stride, on the cells the stride
+selects when the backward runs.  Both give the same vector-Jacobian product
+of the cells walked.  The helpers below
 merge h into the plain method's output bins (value / 2^k) and form the
 squared L2 distance between normalised histograms.
 """
@@ -21,15 +26,32 @@ class RelaxedTally(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta, plan):
         theta = theta.contiguous()
+        ctx.plan = plan
+        # one walk for h and its Jacobian (single-set plans without a stride):
+        # every entry depends on its own type's parameters only
+        ctx.jac = ctx.needs_input_grad[0] and len(plan.shape) == 3 and plan.stride == (1, 0)
+        if ctx.jac:
+            # the pass at unit proportions: h is linear in p_j and the pass
+            # scales its records by p_j only, so it gives
+            # [3, ...] = dh/dmean / p | dh/dsd / p | dh/dp exactly (also at p_j = 0)
+            unit = torch.cat([theta[:2], torch.ones_like(theta[2:])])
+            buf = torch.empty((3,) + tuple(plan.shape), dtype=torch.float64, device=theta.device)
+            plan.jacobian(unit, buf[2], buf[:2])
+            ctx.save_for_backward(theta, buf)
+            return buf[2] * theta[2]
         h = torch.empty(plan.shape, dtype=torch.float64, device=theta.device)
         plan.forward(theta, h)
-        ctx.plan = plan
         ctx.save_for_backward(theta)
         return h
 
     @staticmethod
     def backward(ctx, g):
-        (theta,) = ctx.saved_tensors
+        if ctx.jac and ctx.plan.stride == (1, 0):
+            theta, buf = ctx.saved_tensors
+            grad = (buf * g).sum((1, 2))
+            grad[:2] *= theta[2]
+            return grad, None
+        theta = ctx.saved_tensors[0]
         grad = torch.empty_like(theta)
         ctx.plan.backward(theta, g.contiguous(), grad)
         return grad, None

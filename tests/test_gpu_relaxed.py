@@ -140,6 +140,57 @@ def test_autograd_function_and_finite_differences(procell_lib):
         assert abs(fd - ad) <= 1e-5 * max(abs(fd), 1e-12) + 1e-12, (fd, ad)
 
 
+def test_autograd_routes(procell_lib):
+    """RelaxedTally walks once (Jacobian pass, contracted in backward) when
+    theta needs a gradient and the plan has no stride; without a gradient it
+    runs the forward pass only, and a stride set between forward and backward
+    sends the backward to the adjoint pass on the cells that stride selects.
+    Each route equals the library call it stands for."""
+    import torch
+    from procell import calibrate
+
+    d, beta = _problems()[1]
+    prob = procell_lib.Problem(*_args(d))
+    print("autograd routes, beta", beta, flush=True)
+    with procell_lib.Relaxed(prob, beta) as rp:
+        th0 = torch.tensor(np.stack([d["mean"], d["sd"], d["proportion"]]), dtype=torch.float64, device="cuda")
+        w = torch.randn(rp.shape, dtype=torch.float64, device="cuda",
+                        generator=torch.Generator(device="cuda").manual_seed(3))
+        h_ref = torch.empty(rp.shape, dtype=torch.float64, device="cuda")
+        rp.forward(th0, h_ref)
+        g_ref = torch.empty_like(th0)
+        rp.backward(th0, w, g_ref)
+        with torch.no_grad():
+            h_ng = calibrate.relaxed_tally(rp, th0)
+        th = th0.clone().requires_grad_(True)
+        h_jac = calibrate.relaxed_tally(rp, th)
+        (h_jac * w).sum().backward()
+        tol = dict(rtol=1e-12, atol=1e-12 * float(h_ref.abs().max()))
+        assert torch.allclose(h_ng, h_ref, **tol)
+        assert torch.allclose(h_jac.detach(), h_ref, **tol)
+        assert torch.allclose(th.grad, g_ref, rtol=1e-12, atol=1e-12 * float(g_ref.abs().max()))
+        # a proportion of 0: dL/dp of that type is still the adjoint's
+        thz = th0.clone()
+        thz[2, 0] = 0.0
+        g_z = torch.empty_like(th0)
+        rp.backward(thz, w, g_z)
+        xz = thz.clone().requires_grad_(True)
+        (calibrate.relaxed_tally(rp, xz) * w).sum().backward()
+        assert float(g_z[2, 0]) != 0.0
+        assert torch.allclose(xz.grad, g_z, rtol=1e-12, atol=1e-12 * float(g_z.abs().max()))
+        # a stride set after the forward pass: the adjoint on phase 1 of 2
+        th2 = th0.clone().requires_grad_(True)
+        h2 = calibrate.relaxed_tally(rp, th2)
+        rp.set_stride(2, 1)
+        (h2 * w).sum().backward()
+        g_half = torch.empty_like(th0)
+        rp.backward(th0, w, g_half)
+        rp.set_stride(1)
+        torch.cuda.synchronize()
+        assert torch.allclose(th2.grad, g_half, rtol=1e-12, atol=1e-12 * float(g_half.abs().max()))
+        assert not torch.allclose(g_half, g_ref, rtol=1e-3)
+
+
 def test_plain_method_unchanged(procell_lib, oracle):
     import torch
     from procell import workloads
