@@ -321,9 +321,10 @@ def calibration_measurement(procell, workloads, np, steps=200, chunk=25, nm_roun
     """Batched calibration: 256 candidate parameter sets x 2e5 cells, 2-4
     types, t_max U(96,240), targets from the plain method at the generating
     parameters; 200 steps of the relaxed Jacobian pass + loss and gradient
-    (J^T dL/dh) + on-device Adam replayed from one hipGraph per step.  Beside it, the derivative-free
-    baseline: Nelder-Mead per set over the plain method (common random
-    numbers), all sets evaluated by one batched plain run per round."""
+    (J^T dL/dh) + on-device Adam replayed from one hipGraph per step.
+    Beside it, the derivative-free baseline: Nelder-Mead per set over the
+    plain method (common random numbers), all sets evaluated by one batched
+    plain run per round."""
     from procell import calibrate
 
     gen, start = workloads.calibration_batch(0, 256, 2e5)
