@@ -237,7 +237,7 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     const uint32_t T = S.n_types;
     const uint32_t KT = S.kdim * T;
     uint64_t next = a.wave_unit[2 * w];
-    const uint64_t u_end = a.wave_unit[2 * w + 1];
+    uint64_t u_end = a.wave_unit[2 * w + 1];
     if (next >= u_end)
         return;
     const uint32_t r_lo = a.wave_row[2 * w], r_hi = a.wave_row[2 * w + 1];
@@ -312,22 +312,34 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
     uint32_t c_n = 0, c_top = 0, t_lvl0 = 0, t_lvl1 = 0;
     const uint32_t smask = Strided ? a.stride_mask : 0u;
     const uint32_t sphase = Strided ? ((a.step ? (uint32_t)a.step[0] : 0u) + a.phase) & smask : 0u;
+    // With a cell stride the range is enumerated over the selected cells only:
+    // walk index v stands for local cell soff + stride * (v / n_prolif), type
+    // v mod n_prolif (soff: the first cell c with (c + phase) a multiple of
+    // the stride), so every claim takes a walk.
+    const uint32_t sshift = Strided ? (uint32_t)__popc(smask) : 0u;
+    const uint64_t soff = Strided ? (uint64_t)((0u - sphase) & smask) : 0ull;
+    if (Strided) {
+        auto compress = [&](uint64_t u) -> uint64_t {
+            const uint64_t c = u / n_prolif, t = u - c * n_prolif;
+            const uint64_t below = c > soff ? (c - soff + smask) >> sshift : 0ull;  // selected cells < c
+            return below * n_prolif + ((((uint32_t)c + sphase) & smask) == 0u ? t : 0ull);
+        };
+        next = compress(next);
+        u_end = compress(u_end);
+    }
 
     for (;;) {
         // ---- refill idle lanes from the wavefront's range once more than
-        //      kRefillIdle are idle (or all are); with a cell stride, claimed
-        //      walks of skipped cells leave their lane idle and it claims
-        //      again, at most stride rounds per step ----
-        for (uint32_t round = 0; round <= smask; ++round) {
+        //      kRefillIdle are idle (or all are) ----
+        {
             const uint64_t idle = __ballot(!active);
-            if (!idle || next >= u_end || (__popcll(idle) <= kRefillIdle && ~idle != 0))
-                break;
-            if (!active) {
+            const bool refill = idle && next < u_end && (__popcll(idle) > kRefillIdle || ~idle == 0);
+            if (refill && !active) {
                 const uint64_t u = next + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
-                const uint64_t cu = small_units ? (uint64_t)((uint32_t)u / n_prolif) : u / n_prolif;
-                if (u < u_end && (!Strided || (((uint32_t)cu + sphase) & smask) == 0u)) {
-                    cell = cu;
-                    j = prolif[(uint32_t)(u - cell * n_prolif)];
+                const uint64_t wi = small_units ? (uint64_t)((uint32_t)u / n_prolif) : u / n_prolif;
+                if (u < u_end) {
+                    cell = Strided ? soff + (wi << sshift) : wi;
+                    j = prolif[(uint32_t)(u - wi * n_prolif)];
                     const uint64_t gcell = S.cell_base + cell;
                     row = row_cursor;
                     while (row_start[row + 1 - roff] <= gcell)
@@ -346,21 +358,20 @@ __device__ __forceinline__ void relaxed_walks(const RelaxArgs &a)
                     active = true;
                 }
             }
-            next = next + (uint64_t)__popcll(idle);
-            if (next > u_end)
-                next = u_end;
-            if (next < u_end) {
-                const uint64_t c = S.cell_base + (small_units ? (uint64_t)((uint32_t)next / n_prolif)
-                                                              : next / n_prolif);
-                while (row_start[row_cursor + 1 - roff] <= c)
-                    ++row_cursor;
+            if (refill) {
+                next = next + (uint64_t)__popcll(idle);
+                if (next > u_end)
+                    next = u_end;
+                if (next < u_end) {
+                    const uint64_t wi = small_units ? (uint64_t)((uint32_t)next / n_prolif) : next / n_prolif;
+                    const uint64_t c = S.cell_base + (Strided ? soff + (wi << sshift) : wi);
+                    while (row_start[row_cursor + 1 - roff] <= c)
+                        ++row_cursor;
+                }
             }
         }
-        if (__ballot(active) == 0) {
-            if (!Strided || next >= u_end)
-                break;
-            continue;  // every claimed cell was skipped by the stride: claim again
-        }
+        if (__ballot(active) == 0)
+            break;
         if (!active)
             continue;
 
