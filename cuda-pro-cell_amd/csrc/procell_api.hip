@@ -33,6 +33,8 @@ constexpr uint32_t kSeedSlots = 4;  // pinned host staging slots for batch seeds
 // balances better with finer segments.
 constexpr uint32_t kSegmentsPerWorkgroup = 2;
 constexpr uint32_t kBatchSegmentsPerWorkgroup = 8;
+// most lane-stack levels a batch keeps in LDS (a 256-lane workgroup: 2 KB per level)
+constexpr uint32_t kLdsStackLevels = 11;
 
 struct procell_plan {
     int device = 0;
@@ -226,6 +228,13 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     LaunchShape &ls = pl->shape;
     RunArgs &a = pl->args;
     a.stack_levels = kdim > 3 ? kdim - 1 : 2;  // levels 1 .. kmax, at least 2 (idle lanes store at level 2)
+    // A batch whose deepest set needs more than kLdsStackLevels stack levels
+    // keeps the lanes' stacks in device memory: in LDS they would be sized for
+    // that set and leave no room for the tally window at more than two
+    // workgroups per CU (512-set batch of 128-1024 bins, up to 17 generations:
+    // 2.65 ms per run at 2 workgroups per CU, 5.9 ms at 3 or 4 without a window).
+    const bool gstack = n_sets > 1 && a.stack_levels > kLdsStackLevels;
+    const uint32_t lds_levels = gstack ? 0u : a.stack_levels;
     // row tables (8 bytes per row) go to LDS when they are small and do not
     // cost a resident workgroup (the root steps read them when a wavefront's
     // rows advance, so from device memory they cost little)
@@ -233,9 +242,9 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     int bs = tu.block_size ? (int)tu.block_size : 256;
     int occ = 0;
     for (;;) {
-        const size_t base = sim_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
+        const size_t base = sim_lds_bytes(0, kdim, T, lds_levels, (uint32_t)bs, lds_rows);
         int occ_v = 0;
-        if (sim_occupancy(bs, base, lds_rows != 0, kdim, &occ_v) != hipSuccess)
+        if (sim_occupancy(bs, base, lds_rows != 0, kdim, gstack, &occ_v) != hipSuccess)
             return fail(PROCELL_E_HIP);
         if (occ_v >= 1) {
             occ = occ_v;
@@ -253,7 +262,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     const int want = tu.blocks_per_cu ? (int)tu.blocks_per_cu : std::max(1, 16 * 64 / bs);
     if (lds_rows && occ < want) {
         int occ_g = 0;
-        if (sim_occupancy(bs, sim_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, 0), false, kdim, &occ_g) !=
+        if (sim_occupancy(bs, sim_lds_bytes(0, kdim, T, lds_levels, (uint32_t)bs, 0), false, kdim, gstack, &occ_g) !=
             hipSuccess)
             return fail(PROCELL_E_HIP);
         if (occ_g > occ) {
@@ -264,7 +273,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     a.rows_in_lds = lds_rows ? 1u : 0u;
     occ = std::min<int>(occ, want);
     ls.block_size = bs;
-    const size_t base = sim_lds_bytes(0, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
+    const size_t base = sim_lds_bytes(0, kdim, T, lds_levels, (uint32_t)bs, lds_rows);
     const size_t budget = std::min(lds_per_cu / (size_t)occ, lds_per_block);
     const size_t row_bytes = (size_t)kdim * T * 4;
     uint32_t win_rows = budget > base + 16 ? (uint32_t)((budget - base - 16) / row_bytes) : 0;
@@ -377,9 +386,9 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     }
     // the window must not cost a resident workgroup (LDS allocation granularity)
     for (;;) {
-        ls.lds_bytes = sim_lds_bytes(win_rows, kdim, T, a.stack_levels, (uint32_t)bs, lds_rows);
+        ls.lds_bytes = sim_lds_bytes(win_rows, kdim, T, lds_levels, (uint32_t)bs, lds_rows);
         int occ_w = 0;
-        if (sim_occupancy(bs, ls.lds_bytes, lds_rows != 0, kdim, &occ_w) != hipSuccess)
+        if (sim_occupancy(bs, ls.lds_bytes, lds_rows != 0, kdim, gstack, &occ_w) != hipSuccess)
             return fail(PROCELL_E_HIP);
         // keep a margin for the per-workgroup LDS allocation granularity
         const bool fits = (size_t)occ * align_up(ls.lds_bytes, 2048) <= lds_per_cu;
@@ -409,6 +418,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     const size_t o_ss = off; off = align_up(off + (size_t)g * 4, 256);
     const size_t o_info = off; off = align_up(off + (size_t)n_sets * sizeof(SetInfo), 256);
     const size_t o_seed = off; off = align_up(off + (size_t)n_sets * 8, 256);
+    const size_t o_gs = off; off = align_up(off + (gstack ? (size_t)g * a.stack_levels * bs * sizeof(double) : 0), 256);
     if (arena) {
         if (off > arena_bytes)
             return fail(PROCELL_E_SCRATCH);
@@ -434,6 +444,7 @@ int create_plan(const procell_params *ps, uint32_t n_sets, procell_plan **out, v
     a.kdim = kdim;
     a.counters = (unsigned long long *)(dbase + o_cnt);
     a.tally = (unsigned long long *)(dbase + o_tally);
+    a.gstack = gstack ? (double *)(dbase + o_gs) : nullptr;
 
     a.row_start = (const uint32_t *)(dbase + o_rs);
     a.row_info = (const uint32_t *)(dbase + o_ri);

@@ -53,7 +53,7 @@ struct RunArgs {
     uint32_t n_rows;
     uint32_t n_types;       // most types of any set
     uint32_t kdim;          // most generations + 1 of any set
-    uint32_t stack_levels;  // per-lane LDS stack depth (levels 1 .. kdim-1)
+    uint32_t stack_levels;  // per-lane stack depth (levels 1 .. kdim-1)
     uint32_t win_rows;      // rows of the LDS tally window (0: none)
     uint32_t idle_limit;    // idle lanes tolerated before a wavefront refills
     uint32_t rows_in_lds;   // nonzero: the row tables are copied to LDS
@@ -71,6 +71,7 @@ struct RunArgs {
     const uint32_t *seg_row;     // [segments] row of each segment's first cell (window base)
     unsigned long long *tally;   // per set [rows][kdim][n_types] of the set, file type order
     unsigned long long *counters;  // [n_sets][kCounterCopies][kNumCounters]
+    double *gstack;              // [segments][stack_levels][block] lane stacks in device memory, or null: in LDS
 };
 
 struct LaunchShape {
@@ -86,6 +87,7 @@ hipError_t launch_sim(const RunArgs &a, const LaunchShape &ls, hipStream_t st);
 hipError_t launch_debug_draws(uint64_t seed, const uint64_t *cell, const uint64_t *node,
                               const uint32_t *attempt, uint64_t n, uint32_t *words, double *z,
                               hipStream_t st);
-hipError_t sim_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, uint32_t kdim, int *blocks_per_cu);
+hipError_t sim_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, uint32_t kdim, bool gstack,
+                         int *blocks_per_cu);
 
 }  // namespace procell

@@ -200,7 +200,7 @@ __host__ __device__ constexpr size_t sim_wave_lds_bytes()
 
 constexpr size_t kSimSharedBytes = 48;  // counter sums u32 [8], root step cursor u32 [4]
 
-template <bool RowsLds, typename N>
+template <bool RowsLds, typename N, bool GStack>
 __global__ void __launch_bounds__(256, 4) sim_kernel(RunArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -231,7 +231,11 @@ __global__ void __launch_bounds__(256, 4) sim_kernel(RunArgs a)
     uint32_t *cursor = red + kNumCounters;  // root steps of the segment claimed so far
     QueueEntry *ent = (QueueEntry *)((unsigned char *)red + kSimSharedBytes + (size_t)wave * sim_wave_lds_bytes());
     unsigned int *win = (unsigned int *)((unsigned char *)red + kSimSharedBytes + (size_t)n_waves * sim_wave_lds_bytes());
-    double *stack = (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
+    // the lanes' stacks: in LDS after the window, or (GStack: plans whose
+    // deep trees would leave no LDS for the window) the workgroup's block of
+    // device memory, same layout
+    double *stack = GStack ? a.gstack + (size_t)seg0 * a.stack_levels * bdim
+                           : (double *)((unsigned char *)win + ((size_t)win_len * 4 + 4 + 15) / 16 * 16);
     // the lane's stack column: level l (1-based) at stk_pop + l * stride,
     // i.e. at stk_push + (l - 1) * stride (bytes)
     const uint32_t stride = bdim * (uint32_t)sizeof(double);
@@ -680,20 +684,29 @@ hipError_t launch_zero(unsigned long long *p, uint64_t n, hipStream_t st)
     return hipGetLastError();
 }
 
+template <bool GStack>
+void launch_sim_as(const RunArgs &a, const LaunchShape &ls, hipStream_t st)
+{
+    const dim3 g(ls.blocks), b(ls.block_size);
+    const bool narrow = a.kdim <= 32;  // heap indices and level masks fit 32 bits
+    if (a.rows_in_lds && narrow)
+        hipLaunchKernelGGL((sim_kernel<true, uint32_t, GStack>), g, b, ls.lds_bytes, st, a);
+    else if (a.rows_in_lds)
+        hipLaunchKernelGGL((sim_kernel<true, uint64_t, GStack>), g, b, ls.lds_bytes, st, a);
+    else if (narrow)
+        hipLaunchKernelGGL((sim_kernel<false, uint32_t, GStack>), g, b, ls.lds_bytes, st, a);
+    else
+        hipLaunchKernelGGL((sim_kernel<false, uint64_t, GStack>), g, b, ls.lds_bytes, st, a);
+}
+
 hipError_t launch_sim(const RunArgs &a, const LaunchShape &ls, hipStream_t st)
 {
     if (a.n_cells == 0)
         return hipSuccess;
-    const dim3 g(ls.blocks), b(ls.block_size);
-    const bool narrow = a.kdim <= 32;  // heap indices and level masks fit 32 bits
-    if (a.rows_in_lds && narrow)
-        hipLaunchKernelGGL((sim_kernel<true, uint32_t>), g, b, ls.lds_bytes, st, a);
-    else if (a.rows_in_lds)
-        hipLaunchKernelGGL((sim_kernel<true, uint64_t>), g, b, ls.lds_bytes, st, a);
-    else if (narrow)
-        hipLaunchKernelGGL((sim_kernel<false, uint32_t>), g, b, ls.lds_bytes, st, a);
+    if (a.gstack)
+        launch_sim_as<true>(a, ls, st);
     else
-        hipLaunchKernelGGL((sim_kernel<false, uint64_t>), g, b, ls.lds_bytes, st, a);
+        launch_sim_as<false>(a, ls, st);
     return hipGetLastError();
 }
 
@@ -708,20 +721,28 @@ hipError_t launch_debug_draws(uint64_t seed, const uint64_t *cell, const uint64_
     return hipGetLastError();
 }
 
-hipError_t sim_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, uint32_t kdim, int *blocks_per_cu)
+template <bool GStack>
+hipError_t sim_occupancy_as(int block_size, size_t lds_bytes, bool rows_in_lds, uint32_t kdim, int *blocks_per_cu)
 {
     const bool narrow = kdim <= 32;
     if (rows_in_lds && narrow)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sim_kernel<true, uint32_t>, block_size,
-                                                            lds_bytes);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sim_kernel<true, uint32_t, GStack>,
+                                                            block_size, lds_bytes);
     if (rows_in_lds)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sim_kernel<true, uint64_t>, block_size,
-                                                            lds_bytes);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sim_kernel<true, uint64_t, GStack>,
+                                                            block_size, lds_bytes);
     if (narrow)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sim_kernel<false, uint32_t>, block_size,
-                                                            lds_bytes);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sim_kernel<false, uint64_t>, block_size,
-                                                        lds_bytes);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sim_kernel<false, uint32_t, GStack>,
+                                                            block_size, lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sim_kernel<false, uint64_t, GStack>,
+                                                        block_size, lds_bytes);
+}
+
+hipError_t sim_occupancy(int block_size, size_t lds_bytes, bool rows_in_lds, uint32_t kdim, bool gstack,
+                         int *blocks_per_cu)
+{
+    return gstack ? sim_occupancy_as<true>(block_size, lds_bytes, rows_in_lds, kdim, blocks_per_cu)
+                  : sim_occupancy_as<false>(block_size, lds_bytes, rows_in_lds, kdim, blocks_per_cu);
 }
 
 }  // namespace procell

@@ -94,6 +94,45 @@ def test_batch_matches_oracle(procell_lib, oracle):
                 assert res.stats[k] == st[k], (i, k)
 
 
+@pytest.mark.parametrize("knobs", [(2, 8, 16), (3, 8, 12), (4, 2, 6)])
+def test_deep_batch_stacks_in_device_memory(procell_lib, oracle, knobs):
+    """A batch whose deepest set needs more lane-stack levels than a batch
+    keeps in LDS (here 14 and 15 levels) runs with the lanes' stacks in
+    device memory; every set, shallow or deep, still equals the oracle and
+    its own single run (which keeps its stack in LDS), under several launch
+    knobs (windows, occupancy, segment counts and hand-out thresholds)."""
+    from procell import workloads
+
+    bpc, seg, idle = knobs
+    kw = dict(blocks_per_cu=bpc, segments=seg, idle_lanes=idle)
+    deep = [
+        (["16384", "8192", "5000", "900"], [300, 200, 400, 500], [0.6, 0.4], [6.0, 9.0], [2.0, 3.0], 80.0, 1.0),
+        (["65536"], [50], [1.0], [4.0], [1.0], 60.0, 1.0),
+    ]
+    ws = workloads.parameter_sets(7, 4, 0.02)
+    seeds = [31 + i for i in range(len(deep) + len(ws))]
+    print("deep batch, knobs", knobs, flush=True)
+    probs, ops = [], []
+    for i, (v, c, pr, m, sd, tm, phi) in enumerate(deep):
+        v = [float(x) for x in v]
+        probs.append(procell_lib.Problem(v, c, pr, m, sd, tm, phi, 0, **kw))
+        ops.append(oracle.Problem(v, c, pr, m, sd, tm, phi, seeds[i]))
+    for i, w in enumerate(ws):
+        probs.append(procell_lib.Problem.from_workload(w, 0, **kw))
+        ops.append(oracle.Problem.from_workload(w, seeds[len(deep) + i]))
+    with procell_lib.Batch(probs) as b:
+        b.run(seeds)
+        for i, op in enumerate(ops):
+            res = b.result(i)
+            tally, _, st = oracle.simulate(op)
+            assert np.array_equal(res.raw_tally, tally), (knobs, i)
+            for k in STATS:
+                assert res.stats[k] == st[k], (knobs, i, k)
+            if i < len(deep):
+                assert res.stats["divisions"] > 100000, (knobs, i)
+                _same(res, _single(procell_lib, probs[i], seeds[i]), f"knobs {knobs} set {i}")
+
+
 def test_autotune_keeps_results(procell_lib):
     probs = _sets(procell_lib, 3, 8, 0.02)
     seeds = [5 + i for i in range(len(probs))]
