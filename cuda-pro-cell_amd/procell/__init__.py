@@ -606,7 +606,7 @@ class Calibration:
         return res
 
     def loss(self):
-        """(loss of each set at the last step's forward pass, steps taken)."""
+        """(loss of each set at the last step's parameters, steps taken)."""
         out = np.zeros(max(len(self.problems), 1), dtype=np.float64)
         steps, exp = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().procell_calib_loss(self.handle, out.ctypes.data_as(_dp), len(out), ctypes.byref(steps),
